@@ -1,0 +1,1502 @@
+// gammahip.cpp -- MI355X (gfx950) backend of GAMMA's per-step hydro hot path:
+// device storage, kernels, stage/step driver and the C-ABI of include/gammahip.h.
+//
+// Layout in HBM: structure of arrays over tracks.  A cell field is a
+// [ntracks][nax] plane (nax = nmax + 2*ngst), a multi-variable field is
+// [nvar][ntracks][nax]; consecutive cells of a track are consecutive doubles,
+// so every per-cell kernel reads and writes fully coalesced 64-bit lanes.
+// In one dimension the unused momentum slot (SS2/UU2) is not stored.
+//
+// Kernel bodies are __host__ __device__ functions of (track j, index i); the
+// __global__ wrappers map blockIdx.y -> track and blockIdx.x*blockDim.x +
+// threadIdx.x -> cell.  Built with -DGH_EMU the same bodies run in plain loops
+// on the CPU (test tool for index checks); the product library is the HIP build.
+#include "hydro.hpp"
+#include "../../include/gammahip.h"
+
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <climits>
+#include <string>
+#include <vector>
+#include <algorithm>
+
+#ifdef GH_EMU
+#define __global__
+#define __shared__ static
+#define GH_ATOMIC_ADD(p, v) (*(p) += (v))
+#define GH_ATOMIC_MIN(p, v) (*(p) = (*(p) < (v) ? *(p) : (v)))
+#define GH_ATOMIC_MAX(p, v) (*(p) = (*(p) > (v) ? *(p) : (v)))
+#else
+#define GH_ATOMIC_ADD(p, v) atomicAdd((p), (v))
+#define GH_ATOMIC_MIN(p, v) atomicMin((p), (v))
+#define GH_ATOMIC_MAX(p, v) atomicMax((p), (v))
+#endif
+
+using namespace gh;
+
+namespace {
+
+enum { CNT_C2P_CALLS = 0, CNT_C2P_ITERS, CNT_NOBRACKET, CNT_MAXITER, CNT_PHICAP, CNT_NEGDT,
+       CNT_NANFLUX, CNT_F1OVF, CNT_SPLITS, CNT_MERGES, CNT_MLP, CNT_N = 16 };
+enum { KIN_NONE = 0, KIN_FIXED = 1 };
+enum { BND_NONE = 0, BND_BM = 1 };
+enum { RG_NONE = 0, RG_AR2D = 1, RG_BM1D = 2 };
+
+// ---------------------------------------------------------------------------
+// device view (passed by value to every kernel)
+struct Dev {
+  int nd, nq, nqs, ngst, nj, nax, jL, jR, fcap, shock, sync, solver, guess, maxit, rg_mode;
+  long long cs;            // nj * nax
+  long long fs1;           // nj * fcap
+  Phys P;
+  double rg[6];
+  // per track
+  int *ntrack, *tgeom, *tkin, *tbnd, *tfirst, *tiS;
+  double *tvin, *tvout, *tbm, *ty, *tdy, *tdt, *tt, *tmin;
+  // cells
+  double *prim, *cx, *cdx, *dtloc, *U0, *dV0, *gmv, *gf1, *gf1n, *sd, *pspec;
+  int *shk, *nbdj, *nbdlo, *nbdcnt, *nbuj, *nbulo, *nbucnt, *f1off, *f1tot;
+  // faces
+  double *fx, *fxn, *fx0, *fv, *flL, *flR, *fflux;
+  double *ff1, *ff1sd;
+  unsigned long long* cnt;
+  long long *nsplit, *nmerge;
+};
+
+GH_HD long long CI(const Dev& d, int j, int i) { return (long long)j * d.nax + i; }
+
+template <int ND>
+GH_HD int sq(int q) { return (ND == 1 && q > 3) ? q - 1 : q; }
+
+template <int ND, int NQ>
+GH_HD void load_v(const Dev& d, const double* a, long long k, double* v) {
+  for (int q = 0; q < NQ; ++q) v[q] = (ND == 1 && q == 3) ? 0. : a[sq<ND>(q) * d.cs + k];
+}
+template <int ND, int NQ>
+GH_HD void store_v(const Dev& d, double* a, long long k, const double* v) {
+  for (int q = 0; q < NQ; ++q)
+    if (!(ND == 1 && q == 3)) a[sq<ND>(q) * d.cs + k] = v[q];
+}
+
+template <int ND>
+GH_HD Geo geo_of(const Dev& d, int j, long long k) {
+  return cell_geo<ND>(d.cx[k], d.cdx[k], d.ty[j], d.tdy[j], d.tgeom[j]);
+}
+// r argument of the grid sweeps: centroid in 1D (1d.cpp:915), centre in 2D (2d.cpp:1127)
+template <int ND>
+GH_HD double sweep_r(const Dev& d, long long k, const Geo& g) { return ND == 1 ? g.cen0 : d.cx[k]; }
+
+// state of a cell as the prepForUpdate sweeps leave it: prim, cons, flux along dim
+template <int ND, int NQ>
+GH_HD void cell_state(const Dev& d, int j, int i, int dim, double* p, double* c, double* f) {
+  long long k = CI(d, j, i);
+  load_v<ND, NQ>(d, d.prim, k, p);
+  Geo g = geo_of<ND>(d, j, k);
+  double r = sweep_r<ND>(d, k, g);
+  prim2cons<ND, NQ>(p, c, r, d.tgeom[j], d.P.gamma);
+  state2flux<ND, NQ>(p, c, f, dim, r, d.tgeom[j]);
+}
+
+GH_HD bool active_track(const Dev& d, int j) { return d.nd == 1 || (j > d.jL && j < d.jR); }
+
+// ---------------------------------------------------------------------------
+// Grid::updateGhosts (1d.cpp:217-246, 2d.cpp:241-321)
+
+// copy every persistent per-cell field of (js,is) into (j,i)
+template <int ND>
+GH_HD void copy_cell(const Dev& d, long long kd, long long ks) {
+  for (int s = 0; s < d.nqs; ++s) d.prim[s * d.cs + kd] = d.prim[s * d.cs + ks];
+  d.cx[kd] = d.cx[ks];
+  d.cdx[kd] = d.cdx[ks];
+  d.dtloc[kd] = d.dtloc[ks];
+  if (ND == 2) {
+    for (int s = 0; s < d.nqs; ++s) {
+      d.gmv[s * d.cs + kd] = d.gmv[s * d.cs + ks];
+      d.gf1[s * d.cs + kd] = d.gf1[s * d.cs + ks];
+    }
+    d.nbdj[kd] = d.nbdj[ks]; d.nbdlo[kd] = d.nbdlo[ks]; d.nbdcnt[kd] = d.nbdcnt[ks];
+    d.nbuj[kd] = d.nbuj[ks]; d.nbulo[kd] = d.nbulo[ks]; d.nbucnt[kd] = d.nbucnt[ks];
+  }
+  if (d.shock) { d.sd[kd] = d.sd[ks]; d.pspec[kd] = d.pspec[ks]; d.shk[kd] = d.shk[ks]; }
+}
+
+// ghost tracks: whole-track copy of the edge active track (2d.cpp:246-289)
+GH_HD void body_ghost_tracks(const Dev& d, int g, int i) {
+  int j = g < d.ngst ? g : d.nj - d.ngst + (g - d.ngst);
+  int js = j <= d.jL ? d.jL + 1 : d.jR - 1;
+  int nts = d.ntrack[js];
+  if (i == 0) d.ntrack[j] = nts;
+  if (i < nts) copy_cell<2>(d, CI(d, j, i), CI(d, js, i));
+  if (i < nts - 1) d.fx[CI(d, j, i)] = d.fx[CI(d, js, i)];
+}
+
+// ghost cells and faces along the track + userBoundaries (BM1D.cpp:254-274)
+template <int ND, int NQ>
+GH_HD void body_ghost_mv(const Dev& d, int j, int kk) {
+  if (kk >= 2 * d.ngst) return;
+  int nt = d.ntrack[j], iL = d.ngst - 1, iR = nt - d.ngst;
+  if (kk < d.ngst) {
+    int i = kk;
+    long long ke = CI(d, j, iL + 1), kc = CI(d, j, i);
+    double dxe = d.cdx[ke];
+    copy_cell<ND>(d, kc, ke);
+    d.cx[kc] = d.cx[ke] - (iL - i + 1) * dxe;
+    if (i <= iL - 1) d.fx[kc] = d.fx[ke] - (iL - i + 1) * dxe;
+    if (d.tbnd[j] == BND_BM) {
+      double rho, u, p;
+      bm_profile(d.tbm + 4 * j, d.cx[kc], d.tt[j], d.P.gamma, &rho, &u, &p);
+      d.prim[RHO * d.cs + kc] = rho;
+      d.prim[UU1 * d.cs + kc] = u;
+      if (ND == 2) d.prim[UU2 * d.cs + kc] = 0;
+      d.prim[PPP * d.cs + kc] = p;
+      d.prim[sq<ND>(TR1) * d.cs + kc] = 2.;
+    }
+  } else {
+    int i = iR + (kk - d.ngst);
+    long long ke = CI(d, j, iR - 1), kc = CI(d, j, i);
+    double dxe = d.cdx[ke];
+    copy_cell<ND>(d, kc, ke);
+    d.cx[kc] = d.cx[ke] + (i - iR + 1) * dxe;
+    if (i <= nt - 2) d.fx[kc] = d.fx[ke] + (i - iR) * dxe;   // (i-iRbnd): 1d.cpp:241
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Grid::movDir_ComputeLambda + updateKinematics + computeFluxes, MV faces
+// (1d.cpp:370-461, 731-776; 2d.cpp:451-490, 835-888; hllc.cpp:12-39)
+template <int ND, int NQ>
+GH_HD void body_faces(const Dev& d, int j, int i) {
+  int nt = d.ntrack[j];
+  if (i > nt - 2) return;
+  int geom = d.tgeom[j];
+  double G = d.P.gamma;
+  long long k = CI(d, j, i);
+  double xf = d.fx[k];
+  double pL[NQ], cL[NQ], fL[NQ], pR[NQ], cR[NQ], fR[NQ];
+  if (i == 0 || i == nt - 2) {
+    cell_state<ND, NQ>(d, j, i, 0, pL, cL, fL);
+    cell_state<ND, NQ>(d, j, i + 1, 0, pR, cR, fR);
+  } else {
+    double p0[NQ], p3[NQ];
+    load_v<ND, NQ>(d, d.prim, k - 1, p0);
+    load_v<ND, NQ>(d, d.prim, k, pL);
+    load_v<ND, NQ>(d, d.prim, k + 1, pR);
+    load_v<ND, NQ>(d, d.prim, k + 2, p3);
+    double x0 = geo_of<ND>(d, j, k - 1).cen0, x1 = geo_of<ND>(d, j, k).cen0;
+    double x2 = geo_of<ND>(d, j, k + 1).cen0, x3 = geo_of<ND>(d, j, k + 2).cen0;
+    double gLs[NQ], gRs[NQ];
+    for (int q = 0; q < NQ; ++q) {
+      double gl = (pL[q] - p0[q]) / (x1 - x0);
+      double gn = (pR[q] - pL[q]) / (x2 - x1);
+      double gr = (p3[q] - pR[q]) / (x3 - x2);
+      gLs[q] = minmod(gl, gn);
+      gRs[q] = minmod(gr, gn);
+    }
+    if (ND == 2) {
+      store_v<ND, NQ>(d, d.gmv, k, gLs);
+      if (i == nt - 3) store_v<ND, NQ>(d, d.gmv, k + 1, gRs);
+    }
+    for (int q = 0; q < NQ; ++q) {
+      pL[q] = pL[q] + gLs[q] * (xf - x1);
+      pR[q] = pR[q] + gRs[q] * (xf - x2);
+    }
+    prim2cons<ND, NQ>(pL, cL, xf, geom, G); state2flux<ND, NQ>(pL, cL, fL, 0, xf, geom);
+    prim2cons<ND, NQ>(pR, cR, xf, geom, G); state2flux<ND, NQ>(pR, cR, fR, 0, xf, geom);
+  }
+  Waves w = lambdas<ND, NQ>(pL, cL, pR, cR, 0, xf, geom, G);
+  double v = d.P.vi * w.lS;
+  if (d.tkin[j] == KIN_FIXED) {
+    for (int n = 0; n < d.ngst; ++n) {
+      if (i == n) v = d.tvin[j];
+      if (i == nt - 2 - n) v = d.tvout[j];
+    }
+  }
+  double dA = face_area<ND>(0, xf, d.ty[j], d.tdy[j], geom);
+  double fl[NQ];
+  hllc_flux<ND, NQ>(pL, cL, fL, pR, cR, fR, w, v, dA, 0, xf, geom, fl);
+  if (fl[0] != fl[0]) GH_ATOMIC_ADD(&d.cnt[CNT_NANFLUX], 1ull);
+  d.fv[k] = v;
+  d.flL[k] = w.lL;
+  d.flR[k] = w.lR;
+  store_v<ND, NQ>(d, d.fflux, k, fl);
+  if (d.shock) {
+    double ps = -1.e300;
+    double Sd = shock_sd<ND, NQ>(pL, pR, 0, false, G, &ps);
+    d.sd[k] = fmax(d.sd[k], Sd);
+    if (ps > d.pspec[k]) d.pspec[k] = ps;
+  }
+}
+
+// Cell::update_dt, MV candidates (cell.cpp:47-99)
+GH_HD void body_dt(const Dev& d, int j, int i) {
+  int nt = d.ntrack[j];
+  if (i < 1 || i > nt - 2) return;
+  long long k = CI(d, j, i);
+  double a = d.cdx[k];
+  double l, v, dtL, dtR;
+  l = d.flR[k - 1]; v = d.fv[k];
+  if (l > v) dtL = a / (l - v); else dtL = 1.e15;
+  l = d.flL[k]; v = d.fv[k - 1];
+  if (l < v) dtR = a / (v - l); else dtR = 1.e15;
+  double cand = fmin(dtL, dtR);
+  if (cand < 0) GH_ATOMIC_ADD(&d.cnt[CNT_NEGDT], 1ull);
+  d.dtloc[k] = fmin(d.dtloc[k], cand);
+}
+
+// ---------------------------------------------------------------------------
+// 2D transverse (F1) coupling (2d.cpp:366-430, 491-527, 534-606, 890-950)
+
+// first face index k in [0, nt-2] of track ja with x >= xr, nt-1 if none
+GH_HD int lb_face(const Dev& d, int ja, double xr) {
+  int lo = 0, hi = d.ntrack[ja] - 1;
+  const double* X = d.fx + (long long)ja * d.nax;
+  while (lo < hi) {
+    int mid = (lo + hi) >> 1;
+    if (X[mid] >= xr) hi = mid; else lo = mid + 1;
+  }
+  return lo;
+}
+
+// Grid::computeNeighbors: the two-pointer sweep of 2d.cpp:551-576 evaluated
+// per cell.  With face positions non-decreasing along each track, the sweep
+// pointer after cell i is min(lb(xR_i), n-2); the list of cell i starts one
+// step back from the previous pointer when that face lies right of xL_i.
+GH_HD void nb_side(const Dev& d, int j, int i, int ja, int* lo, int* cnt) {
+  int nta = d.ntrack[ja];
+  long long k = CI(d, j, i);
+  double xjL = d.cx[k] - d.cdx[k] / 2., xjR = d.cx[k] + d.cdx[k] / 2.;
+  int fin = 0;
+  if (i > 1) {
+    double xpR = d.cx[k - 1] + d.cdx[k - 1] / 2.;
+    int l = lb_face(d, ja, xpR);
+    fin = l < nta - 2 ? l : nta - 2;
+  }
+  int s = fin;
+  if (d.fx[CI(d, ja, s)] > xjL && s > 0) s--;
+  int L = lb_face(d, ja, xjR);
+  if (L < s) L = s;
+  *lo = s + 1;
+  *cnt = L - s;
+}
+
+GH_HD void body_nb(const Dev& d, int j, int i) {
+  int nt = d.ntrack[j];
+  if (i < 1 || i > nt - 2) return;
+  long long k = CI(d, j, i);
+  int lo = 0, cnt = 0;
+  if (j != 0) nb_side(d, j, i, j - 1, &lo, &cnt);
+  d.nbdj[k] = j - 1; d.nbdlo[k] = lo; d.nbdcnt[k] = cnt;
+  lo = 0; cnt = 0;
+  if (j != d.nj - 1) nb_side(d, j, i, j + 1, &lo, &cnt);
+  d.nbuj[k] = j + 1; d.nbulo[k] = lo; d.nbucnt[k] = cnt;
+}
+
+// Grid::computeTransGradients (2d.cpp:366-430)
+template <int NQ>
+GH_HD void body_gradf1(const Dev& d, int j, int i) {
+  if (j > d.nj - 2) return;
+  int nt = d.ntrack[j];
+  if (i >= nt) return;
+  int geom = d.tgeom[j];
+  long long k = CI(d, j, i);
+  double x0 = d.cx[k];
+  double xL0 = x0 - d.cdx[k] / 2., xR0 = x0 + d.cdx[k] / 2.;
+  Geo g0 = geo_of<2>(d, j, k);
+  double p0[NQ], m0[NQ], gavg[NQ], gf[NQ];
+  load_v<2, NQ>(d, d.prim, k, p0);
+  load_v<2, NQ>(d, d.gmv, k, m0);
+  for (int q = 0; q < NQ; ++q) gavg[q] = 0.;
+  double dAtot = 0;
+  for (int pass = 0; pass < 2; ++pass) {
+    for (int side = 0; side < 2; ++side) {
+      int tj = side == 0 ? d.nbdj[k] : d.nbuj[k];
+      int lo = side == 0 ? d.nbdlo[k] : d.nbulo[k];
+      int cnt = side == 0 ? d.nbdcnt[k] : d.nbucnt[k];
+      for (int n = 0; n < cnt; ++n) {
+        long long kn = CI(d, tj, lo + n);
+        double xn = d.cx[kn];
+        double xLn = xn - d.cdx[kn] / 2., xRn = xn + d.cdx[kn] / 2.;
+        double xf1I = d.ty[j];
+        if (side == 0) xf1I -= d.tdy[j] / 2.;
+        double xm = (fmax(xL0, xLn) + fmin(xR0, xRn)) / 2.;
+        double ov = fmax(0, fmin(xR0, xRn) - fmax(xL0, xLn));
+        double dA = face_area<2>(1, xm, xf1I, ov, geom);
+        Geo gn = geo_of<2>(d, tj, kn);
+        double dxmL = xm - g0.cen0, dxmR = xm - gn.cen0;
+        double pn[NQ], mn[NQ];
+        load_v<2, NQ>(d, d.prim, kn, pn);
+        load_v<2, NQ>(d, d.gmv, kn, mn);
+        if (pass == 0) dAtot += dA;
+        for (int q = 0; q < NQ; ++q) {
+          double qL = p0[q] + m0[q] * dxmL;
+          double qR = pn[q] + mn[q] * dxmR;
+          double gr = (qR - qL) / (gn.cen1 - g0.cen1);
+          if (pass == 0) gavg[q] += gr * dA;
+          else gf[q] = minmod(gf[q], gr);
+        }
+      }
+    }
+    if (pass == 0)
+      for (int q = 0; q < NQ; ++q) { gavg[q] /= dAtot; gf[q] = gavg[q]; }
+  }
+  store_v<2, NQ>(d, d.gf1n, k, gf);
+}
+
+// F1 faces of one cell with its upper neighbours (2d.cpp:901-950)
+template <int NQ>
+GH_HD void body_f1faces(const Dev& d, int j, int i) {
+  if (j > d.nj - 2) return;
+  int nt = d.ntrack[j];
+  if (i >= nt) return;
+  int geom = d.tgeom[j];
+  double G = d.P.gamma;
+  long long k = CI(d, j, i);
+  int tj = d.nbuj[k], lo = d.nbulo[k], cnt = d.nbucnt[k], off = d.f1off[k];
+  if (cnt <= 0) return;
+  double x0 = d.cx[k];
+  double xL0 = x0 - d.cdx[k] / 2., xR0 = x0 + d.cdx[k] / 2.;
+  double jpos = d.ty[j] + d.tdy[j] / 2.;
+  Geo g0 = geo_of<2>(d, j, k);
+  bool pc0 = (j == 0 || j == d.nj - 2 || d.nbdcnt[k] == 0);
+  double pc[NQ], mc[NQ], fc[NQ];
+  load_v<2, NQ>(d, d.prim, k, pc);
+  load_v<2, NQ>(d, d.gmv, k, mc);
+  load_v<2, NQ>(d, d.gf1n, k, fc);
+  for (int n = 0; n < cnt; ++n) {
+    long long slot = off + n;
+    if (slot >= d.fcap) { GH_ATOMIC_ADD(&d.cnt[CNT_F1OVF], 1ull); break; }
+    long long kn = CI(d, tj, lo + n);
+    double xn = d.cx[kn];
+    double xLn = xn - d.cdx[kn] / 2., xRn = xn + d.cdx[kn] / 2.;
+    double xm = (fmax(xL0, xLn) + fmin(xR0, xRn)) / 2.;
+    double ov = fmax(0, fmin(xR0, xRn) - fmax(xL0, xLn));
+    double dA = face_area<2>(1, xm, jpos, ov, geom);
+    double pL[NQ], cL[NQ], fL[NQ], pR[NQ], cR[NQ], fR[NQ];
+    if (pc0 || d.nbucnt[kn] == 0) {
+      cell_state<2, NQ>(d, j, i, 1, pL, cL, fL);
+      cell_state<2, NQ>(d, tj, lo + n, 1, pR, cR, fR);
+    } else {
+      Geo gn = geo_of<2>(d, tj, kn);
+      double dxmL = xm - g0.cen0, dxmR = xm - gn.cen0;
+      double dxfL = jpos - g0.cen1, dxfR = jpos - gn.cen1;
+      double pn[NQ], mn[NQ], fn[NQ];
+      load_v<2, NQ>(d, d.prim, kn, pn);
+      load_v<2, NQ>(d, d.gmv, kn, mn);
+      // the upper track has not recomputed its transverse gradients yet (2d.cpp:901 loops j upward)
+      load_v<2, NQ>(d, tj > j ? d.gf1 : d.gf1n, kn, fn);
+      for (int q = 0; q < NQ; ++q) {
+        double qL = pc[q] + mc[q] * dxmL;
+        double qR = pn[q] + mn[q] * dxmR;
+        pL[q] = qL + fc[q] * dxfL;
+        pR[q] = qR + fn[q] * dxfR;
+      }
+      prim2cons<2, NQ>(pL, cL, xm, geom, G);
+      prim2cons<2, NQ>(pR, cR, xm, geom, G);
+      state2flux<2, NQ>(pL, cL, fL, 1, xm, geom);
+      state2flux<2, NQ>(pR, cR, fR, 1, xm, geom);
+    }
+    Waves w = lambdas<2, NQ>(pL, cL, pR, cR, 1, xm, geom, G);
+    double fl[NQ];
+    hllc_flux<2, NQ>(pL, cL, fL, pR, cR, fR, w, 0., dA, 1, xm, geom, fl);
+    long long fk = (long long)j * d.fcap + slot;
+    for (int q = 0; q < NQ; ++q) d.ff1[q * d.fs1 + fk] = fl[q];
+    if (d.shock) {
+      double psf = -1.e300, psr = -1.e300;
+      double sdf = shock_sd<2, NQ>(pL, pR, 1, false, G, &psf);
+      double sdr = shock_sd<2, NQ>(pR, pL, 1, true, G, &psr);
+      d.ff1sd[0 * d.fs1 + fk] = sdf; d.ff1sd[1 * d.fs1 + fk] = psf;
+      d.ff1sd[2 * d.fs1 + fk] = sdr; d.ff1sd[3 * d.fs1 + fk] = psr;
+    }
+  }
+}
+
+// Visit, in the reference's accumulation order (track j' ascending, cell i'
+// ascending, 2d.cpp:901-906), every F1 face whose upper cell is (J, i).
+// Fresh lists of track J-1 are non-decreasing ranges (binary search); the
+// stale lists that ghost copies carry on cells 0 and nt-1 are checked one by one.
+template <class F>
+GH_HD void for_incoming_f1(const Dev& d, int J, int i, F&& fn) {
+  auto stale = [&](int jp, int ip) {
+    long long kp = CI(d, jp, ip);
+    int c = d.nbucnt[kp], l = d.nbulo[kp];
+    if (d.nbuj[kp] == J && c > 0 && l <= i && i < l + c) {
+      long long slot = d.f1off[kp] + (i - l);
+      if (slot < d.fcap) fn((long long)jp * d.fcap + slot);
+    }
+  };
+  for (int jp = 0; jp < J; ++jp) {
+    if (jp > d.nj - 2) break;
+    int ntp = d.ntrack[jp];
+    if (jp != J - 1) {
+      if (jp <= d.jL) { stale(jp, 0); stale(jp, ntp - 1); }
+      continue;
+    }
+    stale(jp, 0);
+    // first i' in [1, ntp-2] with lo+cnt > i
+    int a = 1, b = ntp - 1;
+    while (a < b) {
+      int m = (a + b) >> 1;
+      long long km = CI(d, jp, m);
+      if (d.nbulo[km] + d.nbucnt[km] > i) b = m; else a = m + 1;
+    }
+    for (int ip = a; ip <= ntp - 2; ++ip) {
+      long long kp = CI(d, jp, ip);
+      int l = d.nbulo[kp], c = d.nbucnt[kp];
+      if (l > i) break;
+      if (c > 0 && i < l + c) {
+        long long slot = d.f1off[kp] + (i - l);
+        if (slot < d.fcap) fn((long long)jp * d.fcap + slot);
+      }
+    }
+    stale(jp, ntp - 1);
+  }
+}
+
+// measureShock of F1 faces onto their cells (2d.cpp:936-938), after the face pass
+GH_HD void body_f1_shock(const Dev& d, int j, int i) {
+  int nt = d.ntrack[j];
+  if (i >= nt) return;
+  long long k = CI(d, j, i);
+  double Sd = d.sd[k], ps = d.pspec[k];
+  if (j <= d.nj - 2) {
+    int cnt = d.nbucnt[k], off = d.f1off[k];
+    for (int n = 0; n < cnt && off + n < d.fcap; ++n) {
+      long long fk = (long long)j * d.fcap + off + n;
+      Sd = fmax(Sd, d.ff1sd[fk]);
+      if (d.ff1sd[d.fs1 + fk] > ps) ps = d.ff1sd[d.fs1 + fk];
+    }
+  }
+  for_incoming_f1(d, j, i, [&](long long fk) {
+    Sd = fmax(Sd, d.ff1sd[2 * d.fs1 + fk]);
+    if (d.ff1sd[3 * d.fs1 + fk] > ps) ps = d.ff1sd[3 * d.fs1 + fk];
+  });
+  d.sd[k] = Sd;
+  d.pspec[k] = ps;
+}
+
+// ---------------------------------------------------------------------------
+// Grid::update + Cell::update + RK3 averages (1d.cpp:798-841, 2d.cpp:974-1036,
+// cell.cpp:102-127, rk3.cpp:12-195).  One thread owns face i and cell i.
+template <int ND, int NQ>
+GH_HD void body_update(const Dev& d, int j, int i, int stage) {
+  if (!active_track(d, j)) return;
+  int nt = d.ntrack[j];
+  if (i >= nt) return;
+  int iL = d.ngst - 1, iR = nt - d.ngst;
+  int geom = d.tgeom[j];
+  double G = d.P.gamma;
+  double dt = d.tdt[j];
+  double w0 = stage == 2 ? 3. / 4. : 1. / 3.;
+  double w1 = stage == 2 ? 1. / 4. : 2. / 3.;
+  long long k = CI(d, j, i);
+  if (i <= nt - 2) {
+    double xo = d.fx[k];
+    double xn = xo + dt * d.fv[k];
+    if (stage == 1) d.fx0[k] = xo;
+    double xa = xn;
+    if (stage > 1 && i >= iL && i <= iR - 1) xa = w0 * d.fx0[k] + w1 * xn;
+    d.fxn[k] = xa;
+  }
+  if (i < iL + 1 || i > iR - 1) return;
+  double xnL = d.fx[k - 1] + dt * d.fv[k - 1];
+  double xnR = d.fx[k] + dt * d.fv[k];
+  double p[NQ], U[NQ];
+  load_v<ND, NQ>(d, d.prim, k, p);
+  Geo g = geo_of<ND>(d, j, k);
+  prim2cons<ND, NQ>(p, U, sweep_r<ND>(d, k, g), geom, G);
+  if (stage == 1) {
+    store_v<ND, NQ>(d, d.U0, k, U);
+    d.dV0[k] = g.dV;
+  }
+  for (int q = 0; q < NQ; ++q) U[q] *= g.dV;
+  {
+    double fa[NQ], fb[NQ];
+    load_v<ND, NQ>(d, d.fflux, k - 1, fa);
+    load_v<ND, NQ>(d, d.fflux, k, fb);
+    for (int q = 0; q < NQ; ++q) U[q] += (fa[q] - fb[q]) * dt;
+  }
+  if (ND == 2) {
+    double up[NQ], dn[NQ];
+    for (int q = 0; q < NQ; ++q) { up[q] = 0.; dn[q] = 0.; }
+    int cnt = d.nbucnt[k], off = d.f1off[k];
+    for (int n = 0; n < cnt && off + n < d.fcap; ++n) {
+      long long fk = (long long)j * d.fcap + off + n;
+      for (int q = 0; q < NQ; ++q) up[q] += d.ff1[q * d.fs1 + fk];
+    }
+    for_incoming_f1(d, j, i, [&](long long fk) {
+      for (int q = 0; q < NQ; ++q) dn[q] += d.ff1[q * d.fs1 + fk];
+    });
+    for (int q = 0; q < NQ; ++q) U[q] += (dn[q] - up[q]) * dt;
+  }
+  sources<ND, NQ>(p, U, d.cx[k], d.cdx[k], d.ty[j], g.dV, dt, geom, G);
+  if (NQ == 10 && d.sync) radiative(p, U, g.dV, dt, d.P);
+  double cxn = (xnR + xnL) / 2., cdxn = (xnR - xnL);
+  Geo g2 = cell_geo<ND>(cxn, cdxn, d.ty[j], d.tdy[j], geom);
+  for (int q = 0; q < NQ; ++q) U[q] /= g2.dV;
+  C2PStat st = {0, 0};
+  cons2prim<ND, NQ>(p, U, g2.cen0, geom, G, d.solver, -1., d.maxit, &st);
+  int ncalls = 1;
+  if (stage > 1) {
+    double U0[NQ];
+    load_v<ND, NQ>(d, d.U0, k, U0);
+    double dV0 = d.dV0[k];
+    for (int q = 0; q < NQ; ++q) U[q] = w0 * U0[q] * dV0 + w1 * U[q] * g2.dV;
+    double xaL = w0 * d.fx0[k - 1] + w1 * xnL;
+    double xaR = w0 * d.fx0[k] + w1 * xnR;
+    cxn = (xaR + xaL) / 2.;
+    cdxn = (xaR - xaL);
+    Geo g3 = cell_geo<ND>(cxn, cdxn, d.ty[j], d.tdy[j], geom);
+    for (int q = 0; q < NQ; ++q) U[q] /= g3.dV;
+    cons2prim<ND, NQ>(p, U, cxn, geom, G, d.solver, -1., d.maxit, &st);
+    ncalls = 2;
+  }
+  store_v<ND, NQ>(d, d.prim, k, p);
+  d.cx[k] = cxn;
+  d.cdx[k] = cdxn;
+  GH_ATOMIC_ADD(&d.cnt[CNT_C2P_CALLS], (unsigned long long)ncalls);
+  if (st.iters) GH_ATOMIC_ADD(&d.cnt[CNT_C2P_ITERS], (unsigned long long)st.iters);
+  if (st.fail & 1) GH_ATOMIC_ADD(&d.cnt[CNT_NOBRACKET], 1ull);
+  if (st.fail & 2) GH_ATOMIC_ADD(&d.cnt[CNT_MAXITER], 1ull);
+  if (st.fail & 4) GH_ATOMIC_ADD(&d.cnt[CNT_PHICAP], 1ull);
+}
+
+// ---------------------------------------------------------------------------
+// apply() sweeps (radiation_sph.cpp:177-283) and time advance
+GH_HD void body_shock_reset(const Dev& d, int j, int i) {
+  if (i >= d.nax) return;
+  long long k = CI(d, j, i);
+  d.sd[k] = -1; d.pspec[k] = 0; d.shk[k] = 0;
+}
+GH_HD void body_shock_detect(const Dev& d, int j, int i) {
+  if (i >= d.nax) return;
+  long long k = CI(d, j, i);
+  if (d.sd[k] > d.P.shock_thr) d.shk[k] = 1;
+}
+template <int ND, int NQ>
+GH_HD void body_inject(const Dev& d, int j, int i) {
+  if (i >= d.ntrack[j]) return;
+  long long k = CI(d, j, i);
+  double p[NQ];
+  load_v<ND, NQ>(d, d.prim, k, p);
+  inject<ND, NQ>(p, d.shk[k] != 0, d.pspec[k], d.P);
+  store_v<ND, NQ>(d, d.prim, k, p);
+}
+template <int ND, int NQ>
+GH_HD void body_init_gammas(const Dev& d, int j, int i) {
+  if (i >= d.nax) return;
+  long long k = CI(d, j, i);
+  double p[NQ];
+  load_v<ND, NQ>(d, d.prim, k, p);
+  init_gammas<ND, NQ>(p);
+  store_v<ND, NQ>(d, d.prim, k, p);
+}
+
+// ---------------------------------------------------------------------------
+// AMR: Grid::regrid / applyRegrid / split / merge (1d.cpp:484-728,
+// 2d.cpp:609-832) with the checkCellForRegrid hooks of the setups.
+enum { RG_SKIP = 0, RG_MERGE = 1, RG_SPLIT = 2 };
+
+template <int ND, int NQ>
+GH_HD int rg_check(const Dev& d, int j, int i, double iS) {
+  long long k = CI(d, j, i);
+  if (d.rg_mode == RG_AR2D) {
+    double ar = d.cdx[k] / d.tdy[j];
+    if (ar > d.rg[0]) return RG_SPLIT;
+    if (ar < d.rg[1]) return RG_MERGE;
+    return RG_SKIP;
+  }
+  if (d.rg_mode == RG_BM1D) {
+    int nt = d.ntrack[j], iR = nt - d.ngst;
+    double dth = d.rg[2], target = d.rg[4];
+    double rOut = d.cx[CI(d, j, iR - 1)];
+    double ar = d.cdx[k] / (rOut * dth);
+    double dist = i - iS;
+    if (0 < dist && dist < 10) ar *= 10;
+    double p[NQ];
+    load_v<ND, NQ>(d, d.prim, k, p);
+    double lf = lfac<ND, NQ>(p);
+    if (ar > d.rg[0] * target / pow(lf, 3. / 2.)) return RG_SPLIT;
+    if (ar < d.rg[1] * target / pow(lf, 3. / 2.)) return RG_MERGE;
+    return RG_SKIP;
+  }
+  return RG_SKIP;
+}
+
+// shock position of BM1D.cpp:292-301: last active cell with p > 1.5 eta, else iLbnd+1
+GH_HD double rg_shock_index(const Dev& d, int j) {
+  int nt = d.ntrack[j], iL = d.ngst - 1, iR = nt - d.ngst;
+  double eta = d.rg[3];
+  double iS = iL + 1;
+  for (int ii = iR - 1; ii > iL; --ii) {
+    iS = ii;
+    if (d.prim[PPP * d.cs + CI(d, j, ii)] > 1.5 * eta) break;
+  }
+  return iS;
+}
+
+template <int ND>
+GH_HD void rg_move_cell(const Dev& d, long long kd, long long ks) {
+  for (int s = 0; s < d.nqs; ++s) d.prim[s * d.cs + kd] = d.prim[s * d.cs + ks];
+  d.cx[kd] = d.cx[ks];
+  d.cdx[kd] = d.cdx[ks];
+  d.dtloc[kd] = d.dtloc[ks];
+  if (ND == 2) {
+    for (int s = 0; s < d.nqs; ++s) {
+      d.gmv[s * d.cs + kd] = d.gmv[s * d.cs + ks];
+      d.gf1[s * d.cs + kd] = d.gf1[s * d.cs + ks];
+    }
+    d.nbdj[kd] = d.nbdj[ks]; d.nbdlo[kd] = d.nbdlo[ks]; d.nbdcnt[kd] = d.nbdcnt[ks];
+    d.nbuj[kd] = d.nbuj[ks]; d.nbulo[kd] = d.nbulo[ks]; d.nbucnt[kd] = d.nbucnt[ks];
+  }
+}
+
+template <int ND, int NQ>
+GH_HD void rg_split(const Dev& d, int j, int i) {
+  int nt = d.ntrack[j];
+  long long b = CI(d, j, 0);
+  for (int m = nt - 1; m >= i + 1; --m) rg_move_cell<ND>(d, b + m + 1, b + m);
+  for (int m = nt - 2; m >= i; --m) d.fx[b + m + 1] = d.fx[b + m];
+  d.ntrack[j] = nt + 1;
+  int geom = d.tgeom[j];
+  double pLc[NQ], pc[NQ], pRo[NQ], gC[NQ];
+  load_v<ND, NQ>(d, d.prim, b + i - 1, pLc);
+  load_v<ND, NQ>(d, d.prim, b + i, pc);
+  load_v<ND, NQ>(d, d.prim, b + i + 2, pRo);
+  double xLc = geo_of<ND>(d, j, b + i - 1).cen0, xc = geo_of<ND>(d, j, b + i).cen0;
+  double xRo = geo_of<ND>(d, j, b + i + 2).cen0;
+  for (int q = 0; q < NQ; ++q) {
+    double gL = (pc[q] - pLc[q]) / (xc - xLc);
+    double gR = (pRo[q] - pc[q]) / (xRo - xc);
+    if (ND == 1) {
+      double gN = (pRo[q] - pLc[q]) / (xRo - xLc);
+      gL = minmod(gL, gN);
+      gR = minmod(gR, gN);
+    }
+    gC[q] = minmod(gL, gR);
+  }
+  double x_old = d.cx[b + i], dl_old = d.cdx[b + i];
+  d.cx[b + i] = x_old - dl_old / 4.;
+  d.cx[b + i + 1] = x_old + dl_old / 4.;
+  d.cdx[b + i] = dl_old / 2.;
+  d.cdx[b + i + 1] = dl_old / 2.;
+  Geo gc = cell_geo<ND>(d.cx[b + i], d.cdx[b + i], d.ty[j], d.tdy[j], geom);
+  Geo gr = cell_geo<ND>(d.cx[b + i + 1], d.cdx[b + i + 1], d.ty[j], d.tdy[j], geom);
+  double xL = gc.cen0, xR = gr.cen0;
+  double xI = d.cx[b + i] + d.cdx[b + i] / 2.;
+  double pn[NQ], pr[NQ];
+  for (int q = 0; q < NQ; ++q) {
+    double Sc = pc[q];
+    pn[q] = Sc + gC[q] * (xL - xI);
+    pr[q] = Sc + gC[q] * (xR - xI);
+  }
+  store_v<ND, NQ>(d, d.prim, b + i, pn);
+  store_v<ND, NQ>(d, d.prim, b + i + 1, pr);
+  d.fx[b + i] = x_old;
+}
+
+template <int ND, int NQ>
+GH_HD void rg_merge(const Dev& d, int j, int i) {
+  int nt = d.ntrack[j];
+  int nact = nt - 2 * d.ngst;
+  long long b = CI(d, j, 0);
+  int geom = d.tgeom[j];
+  double G = d.P.gamma;
+  bool left = (d.cdx[b + i - 1] < d.cdx[b + i + 1] && i != d.ngst) || i == nact + d.ngst - 1;
+  int v = left ? i - 1 : i + 1;
+  double pc[NQ], pv[NQ], cc[NQ], cv[NQ];
+  load_v<ND, NQ>(d, d.prim, b + i, pc);
+  load_v<ND, NQ>(d, d.prim, b + v, pv);
+  Geo gc = geo_of<ND>(d, j, b + i), gv = geo_of<ND>(d, j, b + v);
+  prim2cons<ND, NQ>(pc, cc, sweep_r<ND>(d, b + i, gc), geom, G);
+  prim2cons<ND, NQ>(pv, cv, sweep_r<ND>(d, b + v, gv), geom, G);
+  double dVl = gc.dV, dVv = gv.dV;
+  for (int q = 0; q < NQ; ++q) cc[q] = (cc[q] * dVl + cv[q] * dVv) / (dVl + dVv);
+  double xc = d.cx[b + i], dxc = d.cdx[b + i], xv = d.cx[b + v], dxv = d.cdx[b + v];
+  double xnew = left ? ((xv - dxv / 2.) + (xc + dxc / 2.)) / 2. : ((xv + dxv / 2.) + (xc - dxc / 2.)) / 2.;
+  double dxnew = dxc + dxv;
+  d.cx[b + i] = xnew;
+  d.cdx[b + i] = dxnew;
+  Geo g2 = cell_geo<ND>(xnew, dxnew, d.ty[j], d.tdy[j], geom);
+  C2PStat st = {0, 0};
+  cons2prim<ND, NQ>(pc, cc, ND == 1 ? g2.cen0 : xnew, geom, G, d.solver, -1., d.maxit, &st);
+  store_v<ND, NQ>(d, d.prim, b + i, pc);
+  if (left) {
+    for (int m = i; m < nt; ++m) rg_move_cell<ND>(d, b + m - 1, b + m);
+    for (int m = i; m < nt - 1; ++m) d.fx[b + m - 1] = d.fx[b + m];
+  } else {
+    for (int m = i + 2; m < nt; ++m) rg_move_cell<ND>(d, b + m - 1, b + m);
+    for (int m = i + 1; m < nt - 1; ++m) d.fx[b + m - 1] = d.fx[b + m];
+  }
+  d.ntrack[j] = nt - 1;
+}
+
+// parallel screen: first active cell of each track whose rule asks for an
+// admissible action (cells before it would be skipped by the sequential loop)
+GH_HD void body_rg_init(const Dev& d, int j, int i) {
+  if (i != 0) return;
+  d.tfirst[j] = INT_MAX;
+  d.tiS[j] = d.ngst;        // iLbnd + 1
+}
+GH_HD void body_rg_shockpos(const Dev& d, int j, int i) {
+  if (!active_track(d, j)) return;
+  int nt = d.ntrack[j], iL = d.ngst - 1, iR = nt - d.ngst;
+  if (i < iL + 1 || i > iR - 1) return;
+  if (d.prim[PPP * d.cs + CI(d, j, i)] > 1.5 * d.rg[3]) GH_ATOMIC_MAX(&d.tiS[j], i);
+}
+template <int ND, int NQ>
+GH_HD void body_rg_screen(const Dev& d, int j, int i) {
+  if (!active_track(d, j)) return;
+  int nt = d.ntrack[j], iL = d.ngst - 1, iR = nt - d.ngst;
+  if (i < iL + 1 || i > iR - 1) return;
+  int a = rg_check<ND, NQ>(d, j, i, (double)d.tiS[j]);
+  bool ok = (a == RG_SPLIT && nt < d.nax - 1) || (a == RG_MERGE && nt > 2 * d.ngst + 2);
+  if (ok) GH_ATOMIC_MIN(&d.tfirst[j], i);
+}
+// sequential applyRegrid loop of one track from its first flagged cell
+template <int ND, int NQ>
+GH_HD void body_rg_apply(const Dev& d, int j, int i0) {
+  if (i0 != 0 || !active_track(d, j)) return;
+  int i = d.tfirst[j];
+  if (i == INT_MAX) return;
+  double iS = d.rg_mode == RG_BM1D ? rg_shock_index(d, j) : 0.;
+  for (; i <= d.ntrack[j] - d.ngst - 1; ++i) {
+    int a = rg_check<ND, NQ>(d, j, i, iS);
+    int nt = d.ntrack[j];
+    if (a == RG_SPLIT && nt < d.nax - 1) {
+      rg_split<ND, NQ>(d, j, i);
+      d.nsplit[j]++;
+      if (d.rg_mode == RG_BM1D) iS = rg_shock_index(d, j);
+    }
+    if (a == RG_MERGE && nt > 2 * d.ngst + 2) {
+      rg_merge<ND, NQ>(d, j, i);
+      d.nmerge[j]++;
+      if (d.rg_mode == RG_BM1D) iS = rg_shock_index(d, j);
+    }
+  }
+}
+
+GH_HD void body_advance_time(const Dev& d, int j, int i) {
+  if (i == 0) d.tt[j] += d.tdt[j];
+}
+
+}  // namespace
+
+// ===========================================================================
+// launch layer
+#ifndef GH_EMU
+#define KB 256
+#define GH_KERNEL2(NAME, BODY)                                                  \
+  __global__ void __launch_bounds__(KB) NAME(Dev d, int ni) {                     \
+    int i = blockIdx.x * blockDim.x + threadIdx.x;                               \
+    int j = blockIdx.y;                                                          \
+    if (i < ni) BODY(d, j, i);                                                   \
+  }
+#define GH_KERNEL2S(NAME, BODY)                                                 \
+  __global__ void __launch_bounds__(KB) NAME(Dev d, int ni, int stage) {        \
+    int i = blockIdx.x * blockDim.x + threadIdx.x;                               \
+    int j = blockIdx.y;                                                          \
+    if (i < ni) BODY(d, j, i, stage);                                            \
+  }
+namespace {
+GH_KERNEL2(k_ghost_tracks, body_ghost_tracks)
+GH_KERNEL2(k_dt, body_dt)
+GH_KERNEL2(k_nb, body_nb)
+GH_KERNEL2(k_f1_shock, body_f1_shock)
+GH_KERNEL2(k_shock_reset, body_shock_reset)
+GH_KERNEL2(k_shock_detect, body_shock_detect)
+GH_KERNEL2(k_rg_init, body_rg_init)
+GH_KERNEL2(k_rg_shockpos, body_rg_shockpos)
+GH_KERNEL2(k_advance_time, body_advance_time)
+template <int ND, int NQ> GH_KERNEL2(k_ghost_mv, (body_ghost_mv<ND, NQ>))
+template <int ND, int NQ> GH_KERNEL2(k_faces, (body_faces<ND, NQ>))
+template <int NQ> GH_KERNEL2(k_gradf1, body_gradf1<NQ>)
+template <int NQ> GH_KERNEL2(k_f1faces, body_f1faces<NQ>)
+template <int ND, int NQ> GH_KERNEL2S(k_update, (body_update<ND, NQ>))
+template <int ND, int NQ> GH_KERNEL2(k_inject, (body_inject<ND, NQ>))
+template <int ND, int NQ> GH_KERNEL2(k_init_gammas, (body_init_gammas<ND, NQ>))
+template <int ND, int NQ> GH_KERNEL2(k_rg_screen, (body_rg_screen<ND, NQ>))
+
+// per-track minimum of dt_loc with reset (1d.cpp:779-795, 2d.cpp:954-971)
+__global__ void __launch_bounds__(KB) k_dt_track_min(Dev d) {
+  __shared__ double sm[KB];
+  int j = blockIdx.x;
+  int nt = d.ntrack[j];
+  double m = 1.e15;
+  for (int i = threadIdx.x; i < nt; i += KB) {
+    long long k = CI(d, j, i);
+    m = fmin(m, d.dtloc[k]);
+    d.dtloc[k] = 1.e15;
+  }
+  sm[threadIdx.x] = m;
+  __syncthreads();
+  for (int s = KB / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) sm[threadIdx.x] = fmin(sm[threadIdx.x], sm[threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) d.tmin[j] = sm[0];
+}
+// dt = CFL * min: one global value for coupled tracks, per track for independent runs
+__global__ void __launch_bounds__(KB) k_dt_finish(Dev d) {
+  __shared__ double sm[KB];
+  double m = 1.e15;
+  for (int j = threadIdx.x; j < d.nj; j += KB) m = fmin(m, d.tmin[j]);
+  sm[threadIdx.x] = m;
+  __syncthreads();
+  for (int s = KB / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) sm[threadIdx.x] = fmin(sm[threadIdx.x], sm[threadIdx.x + s]);
+    __syncthreads();
+  }
+  for (int j = threadIdx.x; j < d.nj; j += KB) d.tdt[j] = d.P.cfl * (d.nd == 2 ? sm[0] : d.tmin[j]);
+}
+// exclusive prefix sum of upper-neighbour counts along each track (F1 face slots)
+__global__ void __launch_bounds__(KB) k_f1_scan(Dev d) {
+  __shared__ int sm[KB];
+  int j = blockIdx.x;
+  int nt = d.ntrack[j];
+  int base = 0;
+  for (int i0 = 0; i0 < nt; i0 += KB) {
+    int i = i0 + threadIdx.x;
+    int v = i < nt ? d.nbucnt[CI(d, j, i)] : 0;
+    if (v < 0) v = 0;
+    sm[threadIdx.x] = v;
+    __syncthreads();
+    for (int s = 1; s < KB; s <<= 1) {
+      int t = (int)threadIdx.x >= s ? sm[threadIdx.x - s] : 0;
+      __syncthreads();
+      sm[threadIdx.x] += t;
+      __syncthreads();
+    }
+    if (i < nt) d.f1off[CI(d, j, i)] = base + sm[threadIdx.x] - v;
+    int tot = sm[KB - 1];
+    __syncthreads();
+    base += tot;
+  }
+  if (threadIdx.x == 0) {
+    d.f1tot[j] = base;
+    if (base > d.fcap) atomicAdd(&d.cnt[CNT_F1OVF], 1ull);
+  }
+}
+template <int ND, int NQ>
+__global__ void k_rg_apply(Dev d) {
+  int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < d.nj) body_rg_apply<ND, NQ>(d, j, 0);
+}
+}  // namespace
+#endif
+
+// ===========================================================================
+// host side
+struct KTimer {
+  const char* name;
+  double ms;
+  long long n;
+};
+
+struct gammahip_ctx {
+  gammahip_config cfg;
+  Dev d;
+  std::string err;
+  int ntr;                 // tracks incl. ghost tracks
+  std::vector<void*> allocs;
+  // events for kernel timing (0 = update, 1 = faces, 2 = f1faces, 3 = gradf1)
+  bool timing = false;
+  KTimer kt[16];
+#ifndef GH_EMU
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0[16], ev1[16];
+  bool ev_pending[16];
+#endif
+};
+
+namespace {
+
+int fail(gammahip_ctx* c, const std::string& m) {
+  if (c) c->err = m;
+  return -1;
+}
+
+#ifndef GH_EMU
+#define HIPCHK(x)                                                                   \
+  do {                                                                              \
+    hipError_t e_ = (x);                                                            \
+    if (e_ != hipSuccess) return fail(c, std::string(#x) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+#endif
+
+template <class T>
+int dalloc(gammahip_ctx* c, T** p, size_t n) {
+  size_t bytes = sizeof(T) * (n ? n : 1);
+#ifdef GH_EMU
+  *p = (T*)calloc(1, bytes);
+  if (!*p) return fail(c, "allocation failed");
+#else
+  HIPCHK(hipMalloc((void**)p, bytes));
+  HIPCHK(hipMemset(*p, 0, bytes));
+#endif
+  c->allocs.push_back((void*)*p);
+  return 0;
+}
+
+int h2d(gammahip_ctx* c, void* dst, const void* src, size_t bytes) {
+#ifdef GH_EMU
+  memcpy(dst, src, bytes);
+  return 0;
+#else
+  HIPCHK(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+  return 0;
+#endif
+}
+int d2h(gammahip_ctx* c, void* dst, const void* src, size_t bytes) {
+#ifdef GH_EMU
+  memcpy(dst, src, bytes);
+  return 0;
+#else
+  HIPCHK(hipStreamSynchronize(c->stream));
+  HIPCHK(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+  return 0;
+#endif
+}
+
+// ---- launches ----
+#ifdef GH_EMU
+#define L2(c, body, ntr, ni)                                   \
+  for (int j_ = 0; j_ < (ntr); ++j_)                           \
+    for (int i_ = 0; i_ < (ni); ++i_) body((c)->d, j_, i_)
+#define L2S(c, body, ntr, ni, st)                              \
+  for (int j_ = 0; j_ < (ntr); ++j_)                           \
+    for (int i_ = 0; i_ < (ni); ++i_) body((c)->d, j_, i_, st)
+#else
+#define L2(c, kern, ntr, ni)                                                         \
+  do {                                                                               \
+    dim3 g_(((ni) + KB - 1) / KB, (ntr));                                            \
+    hipLaunchKernelGGL(kern, g_, dim3(KB), 0, (c)->stream, (c)->d, (int)(ni));     \
+  } while (0)
+#define L2S(c, kern, ntr, ni, st)                                                    \
+  do {                                                                               \
+    dim3 g_(((ni) + KB - 1) / KB, (ntr));                                            \
+    hipLaunchKernelGGL(kern, g_, dim3(KB), 0, (c)->stream, (c)->d, (int)(ni), (int)(st)); \
+  } while (0)
+#endif
+
+#ifdef GH_EMU
+#define KSEL(emu, gpu) emu
+#else
+#define KSEL(emu, gpu) gpu
+#endif
+
+void tbeg(gammahip_ctx* c, int id) {
+#ifndef GH_EMU
+  if (c->timing) hipEventRecord(c->ev0[id], c->stream);
+#else
+  (void)c; (void)id;
+#endif
+}
+void tend(gammahip_ctx* c, int id) {
+#ifndef GH_EMU
+  if (!c->timing) return;
+  hipEventRecord(c->ev1[id], c->stream);
+  // collect the previous measurement lazily at the next sync point
+  hipEventSynchronize(c->ev1[id]);
+  float ms = 0.f;
+  hipEventElapsedTime(&ms, c->ev0[id], c->ev1[id]);
+  c->kt[id].ms += ms;
+  c->kt[id].n++;
+#else
+  (void)c; (void)id;
+#endif
+}
+
+template <int ND, int NQ>
+struct Pipe {
+  static void ghosts(gammahip_ctx* c) {
+    Dev& d = c->d;
+    if (ND == 2) L2(c, KSEL(body_ghost_tracks, k_ghost_tracks), 2 * d.ngst, d.nax);
+    L2(c, KSEL((body_ghost_mv<ND, NQ>), (k_ghost_mv<ND, NQ>)), d.nj, 2 * d.ngst);
+  }
+  static void prep(gammahip_ctx* c) {
+    Dev& d = c->d;
+    tbeg(c, 1);
+    L2(c, KSEL((body_faces<ND, NQ>), (k_faces<ND, NQ>)), d.nj, d.nax);
+    tend(c, 1);
+    L2(c, KSEL(body_dt, k_dt), d.nj, d.nax);
+    if (ND == 2) {
+      L2(c, KSEL(body_nb, k_nb), d.nj, d.nax);
+#ifdef GH_EMU
+      for (int j = 0; j < d.nj; ++j) {
+        int base = 0, nt = d.ntrack[j];
+        for (int i = 0; i < nt; ++i) {
+          long long k = CI(d, j, i);
+          d.f1off[k] = base;
+          base += d.nbucnt[k] > 0 ? d.nbucnt[k] : 0;
+        }
+        d.f1tot[j] = base;
+        if (base > d.fcap) d.cnt[CNT_F1OVF]++;
+      }
+#else
+      hipLaunchKernelGGL(k_f1_scan, dim3(d.nj), dim3(KB), 0, c->stream, d);
+#endif
+      tbeg(c, 3);
+      L2(c, KSEL(body_gradf1<NQ>, k_gradf1<NQ>), d.nj, d.nax);
+      tend(c, 3);
+      tbeg(c, 2);
+      L2(c, KSEL(body_f1faces<NQ>, k_f1faces<NQ>), d.nj, d.nax);
+      tend(c, 2);
+      if (d.shock) L2(c, KSEL(body_f1_shock, k_f1_shock), d.nj, d.nax);
+      std::swap(d.gf1, d.gf1n);
+    }
+  }
+  static void collect_dt(gammahip_ctx* c) {
+    Dev& d = c->d;
+#ifdef GH_EMU
+    double gm = 1.e15;
+    for (int j = 0; j < d.nj; ++j) {
+      double m = 1.e15;
+      for (int i = 0; i < d.ntrack[j]; ++i) {
+        long long k = CI(d, j, i);
+        m = fmin(m, d.dtloc[k]);
+        d.dtloc[k] = 1.e15;
+      }
+      d.tmin[j] = m;
+      gm = fmin(gm, m);
+    }
+    for (int j = 0; j < d.nj; ++j) d.tdt[j] = d.P.cfl * (d.nd == 2 ? gm : d.tmin[j]);
+#else
+    hipLaunchKernelGGL(k_dt_track_min, dim3(d.nj), dim3(KB), 0, c->stream, d);
+    hipLaunchKernelGGL(k_dt_finish, dim3(1), dim3(KB), 0, c->stream, d);
+#endif
+  }
+  static void update(gammahip_ctx* c, int stage) {
+    Dev& d = c->d;
+    tbeg(c, 0);
+    L2S(c, KSEL((body_update<ND, NQ>), (k_update<ND, NQ>)), d.nj, d.nax, stage);
+    tend(c, 0);
+    std::swap(d.fx, d.fxn);
+  }
+  static void regrid(gammahip_ctx* c) {
+    Dev& d = c->d;
+    if (d.rg_mode == RG_NONE) return;
+    L2(c, KSEL(body_rg_init, k_rg_init), d.nj, 1);
+    if (d.rg_mode == RG_BM1D) L2(c, KSEL(body_rg_shockpos, k_rg_shockpos), d.nj, d.nax);
+    L2(c, KSEL((body_rg_screen<ND, NQ>), (k_rg_screen<ND, NQ>)), d.nj, d.nax);
+#ifdef GH_EMU
+    for (int j = 0; j < d.nj; ++j) body_rg_apply<ND, NQ>(d, j, 0);
+#else
+    hipLaunchKernelGGL((k_rg_apply<ND, NQ>), dim3((d.nj + 63) / 64), dim3(64), 0, c->stream, d);
+#endif
+  }
+  static void apply(gammahip_ctx* c, int what) {
+    Dev& d = c->d;
+    switch (what) {
+      case GAMMAHIP_RESET_SHOCK: if (d.shock) L2(c, KSEL(body_shock_reset, k_shock_reset), d.nj, d.nax); break;
+      case GAMMAHIP_DETECT_SHOCK: if (d.shock) L2(c, KSEL(body_shock_detect, k_shock_detect), d.nj, d.nax); break;
+      case GAMMAHIP_INJECT: if (NQ == 10 && d.sync) L2(c, KSEL((body_inject<ND, NQ>), (k_inject<ND, NQ>)), d.nj, d.nax); break;
+      case GAMMAHIP_INIT_GAMMAS: if (NQ == 10 && d.sync) L2(c, KSEL((body_init_gammas<ND, NQ>), (k_init_gammas<ND, NQ>)), d.nj, d.nax); break;
+      case GAMMAHIP_ADVANCE_TIME: L2(c, KSEL(body_advance_time, k_advance_time), d.nj, 1); break;
+      default: break;
+    }
+  }
+  // Grid::evolve (rk3.cpp:12-195)
+  static void evolve(gammahip_ctx* c) {
+    update(c, 1);
+    ghosts(c); prep(c); update(c, 2);
+    ghosts(c); prep(c); update(c, 3);
+  }
+  // one iteration of Simu::run (simu.cpp:47-77)
+  static void step(gammahip_ctx* c) {
+    regrid(c);
+    apply(c, GAMMAHIP_RESET_SHOCK);
+    ghosts(c);
+    prep(c);
+    apply(c, GAMMAHIP_DETECT_SHOCK);
+    collect_dt(c);
+    evolve(c);
+    apply(c, GAMMAHIP_ADVANCE_TIME);
+    apply(c, GAMMAHIP_INJECT);
+  }
+};
+
+template <class F>
+int dispatch(gammahip_ctx* c, F&& f) {
+  int nd = c->cfg.nd, nq = c->cfg.nq;
+  if (nd == 1 && nq == 6) { f(Pipe<1, 6>()); return 0; }
+  if (nd == 1 && nq == 10) { f(Pipe<1, 10>()); return 0; }
+  if (nd == 2 && nq == 6) { f(Pipe<2, 6>()); return 0; }
+  if (nd == 2 && nq == 10) { f(Pipe<2, 10>()); return 0; }
+  return fail(c, "unsupported (nd, nq)");
+}
+
+int post(gammahip_ctx* c) {
+#ifndef GH_EMU
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(c, std::string("kernel launch: ") + hipGetErrorString(e));
+#endif
+  (void)c;
+  return 0;
+}
+
+}  // namespace
+
+// ===========================================================================
+// C-ABI
+extern "C" {
+
+int gammahip_create(const gammahip_config* cfg, gammahip_ctx** out) {
+  if (!cfg || !out) return -1;
+  gammahip_ctx* c = new gammahip_ctx();
+  *out = c;
+  c->cfg = *cfg;
+  for (int k = 0; k < 16; ++k) c->kt[k] = KTimer{"", 0., 0};
+  if (cfg->nd != 1 && cfg->nd != 2) return fail(c, "nd must be 1 or 2");
+  if (cfg->nq != 6 && cfg->nq != 10) return fail(c, "nq must be 6 or 10");
+  if (cfg->sync && cfg->nq != 10) return fail(c, "synchrotron tracers need nq == 10");
+  if (cfg->ngst != 2) return fail(c, "ngst must be 2");
+  if (cfg->ncell_mv < 8 || cfg->nmax < cfg->ncell_mv) return fail(c, "bad ncell_mv / nmax");
+  if (cfg->ntracks_active < 1) return fail(c, "bad ntracks_active");
+  if (cfg->nd == 2 && cfg->geom < 0) return fail(c, "2D needs a single geometry");
+  if (cfg->c2p_guess != 0) return fail(c, "learned guess not loaded in this context");
+#ifndef GH_EMU
+  {
+    int ndev = 0;
+    hipError_t e = hipGetDeviceCount(&ndev);
+    if (e != hipSuccess || ndev < 1) return fail(c, "no HIP device");
+    if (hipStreamCreate(&c->stream) != hipSuccess) return fail(c, "stream creation failed");
+    for (int k = 0; k < 16; ++k) {
+      hipEventCreate(&c->ev0[k]);
+      hipEventCreate(&c->ev1[k]);
+    }
+  }
+#endif
+  Dev& d = c->d;
+  memset(&d, 0, sizeof(d));
+  d.nd = cfg->nd; d.nq = cfg->nq; d.nqs = cfg->nd == 1 ? cfg->nq - 1 : cfg->nq;
+  d.ngst = cfg->ngst;
+  d.nj = cfg->nd == 2 ? cfg->ntracks_active + 2 * cfg->ngst : cfg->ntracks_active;
+  d.nax = cfg->nmax + 2 * cfg->ngst;
+  d.jL = cfg->ngst - 1;
+  d.jR = d.nj - cfg->ngst;
+  d.fcap = 4 * d.nax;
+  d.shock = cfg->shock; d.sync = cfg->sync;
+  d.solver = cfg->c2p_solver; d.guess = cfg->c2p_guess;
+  d.maxit = cfg->c2p_maxiter > 0 ? cfg->c2p_maxiter : 1000;
+  d.rg_mode = cfg->rg_mode;
+  for (int k = 0; k < 6; ++k) d.rg[k] = cfg->rg[k];
+  d.P = Phys{cfg->gamma, cfg->cfl, cfg->vi, cfg->shock_thr, cfg->gamma_max_init,
+             cfg->Nalpha, cfg->Nmp, cfg->Nme};
+  d.cs = (long long)d.nj * d.nax;
+  d.fs1 = (long long)d.nj * d.fcap;
+  c->ntr = d.nj;
+  size_t nj = d.nj, cs = d.cs, nqs = d.nqs;
+  int e = 0;
+  e |= dalloc(c, &d.ntrack, nj); e |= dalloc(c, &d.tgeom, nj); e |= dalloc(c, &d.tkin, nj);
+  e |= dalloc(c, &d.tbnd, nj); e |= dalloc(c, &d.tfirst, nj); e |= dalloc(c, &d.tiS, nj);
+  e |= dalloc(c, &d.tvin, nj); e |= dalloc(c, &d.tvout, nj); e |= dalloc(c, &d.tbm, 4 * nj);
+  e |= dalloc(c, &d.ty, nj); e |= dalloc(c, &d.tdy, nj); e |= dalloc(c, &d.tdt, nj);
+  e |= dalloc(c, &d.tt, nj); e |= dalloc(c, &d.tmin, nj);
+  e |= dalloc(c, &d.prim, nqs * cs); e |= dalloc(c, &d.cx, cs); e |= dalloc(c, &d.cdx, cs);
+  e |= dalloc(c, &d.dtloc, cs); e |= dalloc(c, &d.U0, nqs * cs); e |= dalloc(c, &d.dV0, cs);
+  e |= dalloc(c, &d.fx, cs); e |= dalloc(c, &d.fxn, cs); e |= dalloc(c, &d.fx0, cs);
+  e |= dalloc(c, &d.fv, cs); e |= dalloc(c, &d.flL, cs); e |= dalloc(c, &d.flR, cs);
+  e |= dalloc(c, &d.fflux, nqs * cs);
+  e |= dalloc(c, &d.cnt, CNT_N); e |= dalloc(c, &d.nsplit, nj); e |= dalloc(c, &d.nmerge, nj);
+  if (d.nd == 2) {
+    e |= dalloc(c, &d.gmv, nqs * cs); e |= dalloc(c, &d.gf1, nqs * cs); e |= dalloc(c, &d.gf1n, nqs * cs);
+    e |= dalloc(c, &d.nbdj, cs); e |= dalloc(c, &d.nbdlo, cs); e |= dalloc(c, &d.nbdcnt, cs);
+    e |= dalloc(c, &d.nbuj, cs); e |= dalloc(c, &d.nbulo, cs); e |= dalloc(c, &d.nbucnt, cs);
+    e |= dalloc(c, &d.f1off, cs); e |= dalloc(c, &d.f1tot, nj);
+    e |= dalloc(c, &d.ff1, nqs * (size_t)d.fs1);
+    if (d.shock) e |= dalloc(c, &d.ff1sd, 4 * (size_t)d.fs1);
+  }
+  if (d.shock) { e |= dalloc(c, &d.sd, cs); e |= dalloc(c, &d.pspec, cs); e |= dalloc(c, &d.shk, cs); }
+  if (e) return -1;
+  return 0;
+}
+
+int gammahip_destroy(gammahip_ctx* c) {
+  if (!c) return 0;
+#ifdef GH_EMU
+  for (void* p : c->allocs) free(p);
+#else
+  if (c->stream) hipStreamSynchronize(c->stream);
+  for (void* p : c->allocs) hipFree(p);
+  if (c->stream) {
+    for (int k = 0; k < 16; ++k) { hipEventDestroy(c->ev0[k]); hipEventDestroy(c->ev1[k]); }
+    hipStreamDestroy(c->stream);
+  }
+#endif
+  delete c;
+  return 0;
+}
+
+const char* gammahip_last_error(gammahip_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int gammahip_load(gammahip_ctx* c, const double* x, const double* dx, const double* y,
+                  const double* dy, const double* prim, const gammahip_track_hooks* hooks,
+                  double t0) {
+  if (!c) return -1;
+  Dev& d = c->d;
+  const gammahip_config& cf = c->cfg;
+  int nj = d.nj, nax = d.nax, ng = d.ngst, nm = cf.ncell_mv, na = cf.ntracks_active;
+  size_t cs = d.cs;
+  // host images of the initial state (Cinit + mpi_distribute, 1d.cpp:956-969, 2d.cpp:1175-1195)
+  std::vector<int> ntrack(nj, nm + 2 * ng), tgeom(nj), tkin(nj), tbnd(nj);
+  std::vector<double> tvin(nj), tvout(nj), tbm(4 * nj), ty(nj), tdy(nj), tt(nj, t0);
+  std::vector<double> hp((size_t)d.nqs * cs, 0.), hx(cs, 0.), hdx(cs, 0.), hfx(cs, 0.), hdt(cs, 1.e15);
+  auto trk = [&](int a) { return cf.nd == 2 ? a + ng : a; };
+  for (int a = 0; a < na; ++a) {
+    int j = trk(a);
+    const gammahip_track_hooks& h = hooks[cf.nd == 2 ? 0 : a];
+    tgeom[j] = cf.geom >= 0 ? cf.geom : h.geom;
+    tkin[j] = h.kin_mode; tvin[j] = h.v_in; tvout[j] = h.v_out; tbnd[j] = h.bnd_mode;
+    for (int q = 0; q < 4; ++q) tbm[4 * j + q] = h.bm[q];
+    ty[j] = cf.nd == 2 ? y[a] : 0.;
+    tdy[j] = cf.nd == 2 ? dy[a] : 0.;
+    for (int ii = 0; ii < nm; ++ii) {
+      size_t k = (size_t)j * nax + ii + ng, s = (size_t)a * nm + ii;
+      hx[k] = x[s]; hdx[k] = dx[s];
+      double pv[10];
+      for (int q = 0; q < cf.nq; ++q) pv[q] = prim[((size_t)q * na + a) * nm + ii];
+      // Grid::v2u (1d.cpp:934-951)
+      double v = 0;
+      for (int dd = 0; dd < cf.nd; ++dd) v += pv[UU1 + dd] * pv[UU1 + dd];
+      v = sqrt(v);
+      double lf = 1. / sqrt(1. - v * v);
+      for (int dd = 0; dd < cf.nd; ++dd) pv[UU1 + dd] *= lf;
+      for (int q = 0; q < cf.nq; ++q) {
+        if (cf.nd == 1 && q == 3) continue;
+        int s2 = (cf.nd == 1 && q > 3) ? q - 1 : q;
+        hp[(size_t)s2 * cs + k] = pv[q];
+      }
+    }
+  }
+  if (cf.nd == 2) {
+    // ghost tracks: positions shifted from the edge tracks (2d.cpp:259, 281)
+    double dyL = tdy[ng], dyR = tdy[nj - ng - 1];
+    for (int j = 0; j <= d.jL; ++j) {
+      int js = d.jL + 1;
+      ty[j] = ty[js] - (d.jL - j + 1) * dyL; tdy[j] = tdy[js];
+      tgeom[j] = tgeom[js]; tkin[j] = tkin[js]; tvin[j] = tvin[js]; tvout[j] = tvout[js]; tbnd[j] = tbnd[js];
+    }
+    for (int j = d.jR; j < nj; ++j) {
+      int js = d.jR - 1;
+      ty[j] = ty[js] + (j - d.jR + 1) * dyR; tdy[j] = tdy[js];
+      tgeom[j] = tgeom[js]; tkin[j] = tkin[js]; tvin[j] = tvin[js]; tvout[j] = tvout[js]; tbnd[j] = tbnd[js];
+    }
+  }
+  // Grid::interfaceGeomFromCellPos (1d.cpp:844-856, 2d.cpp:1039-1058), active tracks
+  for (int a = 0; a < na; ++a) {
+    int j = trk(a), iL = ng - 1, nt = ntrack[j];
+    for (int i = 0; i < nt - 1; ++i) {
+      size_t k = (size_t)j * nax + i;
+      if (i != iL) hfx[k] = hx[k] + hdx[k] / 2.;
+      else hfx[k] = hx[k + 1] - hdx[k + 1] / 2.;
+    }
+  }
+  int e = 0;
+  e |= h2d(c, d.ntrack, ntrack.data(), sizeof(int) * nj);
+  e |= h2d(c, d.tgeom, tgeom.data(), sizeof(int) * nj);
+  e |= h2d(c, d.tkin, tkin.data(), sizeof(int) * nj);
+  e |= h2d(c, d.tbnd, tbnd.data(), sizeof(int) * nj);
+  e |= h2d(c, d.tvin, tvin.data(), sizeof(double) * nj);
+  e |= h2d(c, d.tvout, tvout.data(), sizeof(double) * nj);
+  e |= h2d(c, d.tbm, tbm.data(), sizeof(double) * 4 * nj);
+  e |= h2d(c, d.ty, ty.data(), sizeof(double) * nj);
+  e |= h2d(c, d.tdy, tdy.data(), sizeof(double) * nj);
+  e |= h2d(c, d.tt, tt.data(), sizeof(double) * nj);
+  e |= h2d(c, d.prim, hp.data(), sizeof(double) * hp.size());
+  e |= h2d(c, d.cx, hx.data(), sizeof(double) * cs);
+  e |= h2d(c, d.cdx, hdx.data(), sizeof(double) * cs);
+  e |= h2d(c, d.fx, hfx.data(), sizeof(double) * cs);
+  e |= h2d(c, d.dtloc, hdt.data(), sizeof(double) * cs);
+  if (e) return -1;
+  if (d.shock) {
+    std::vector<double> m1(cs, -1.);
+    e |= h2d(c, d.sd, m1.data(), sizeof(double) * cs);
+  }
+  if (e) return -1;
+  // radiation_initGammas (simu.cpp:34-36), then the first updateGhosts (simu.cpp:43)
+  int r = dispatch(c, [&](auto P) {
+    P.apply(c, GAMMAHIP_INIT_GAMMAS);
+    P.ghosts(c);
+  });
+  if (r) return r;
+  return post(c);
+}
+
+int gammahip_step(gammahip_ctx* c, int nsteps) {
+  if (!c) return -1;
+  for (int s = 0; s < nsteps; ++s) {
+    int r = dispatch(c, [&](auto P) { P.step(c); });
+    if (r) return r;
+    if (post(c)) return -1;
+  }
+  return 0;
+}
+
+int gammahip_regrid(gammahip_ctx* c) { return dispatch(c, [&](auto P) { P.regrid(c); }) || post(c); }
+int gammahip_update_ghosts(gammahip_ctx* c) { return dispatch(c, [&](auto P) { P.ghosts(c); }) || post(c); }
+int gammahip_prep_for_update(gammahip_ctx* c) { return dispatch(c, [&](auto P) { P.prep(c); }) || post(c); }
+int gammahip_evolve(gammahip_ctx* c) { return dispatch(c, [&](auto P) { P.evolve(c); }) || post(c); }
+int gammahip_apply(gammahip_ctx* c, int what) { return dispatch(c, [&](auto P) { P.apply(c, what); }) || post(c); }
+
+int gammahip_collect_dt(gammahip_ctx* c, double* dt_out) {
+  if (dispatch(c, [&](auto P) { P.collect_dt(c); }) || post(c)) return -1;
+  if (dt_out) return d2h(c, dt_out, c->d.tdt, sizeof(double));
+  return 0;
+}
+
+int gammahip_sync(gammahip_ctx* c) {
+#ifndef GH_EMU
+  HIPCHK(hipStreamSynchronize(c->stream));
+#endif
+  (void)c;
+  return 0;
+}
+
+int gammahip_dims(gammahip_ctx* c, int* ntracks, int* nax) {
+  *ntracks = c->d.nj;
+  *nax = c->d.nax;
+  return 0;
+}
+
+int gammahip_track_info(gammahip_ctx* c, int* ntrack, int* nact, int* iL, int* iR) {
+  Dev& d = c->d;
+  std::vector<int> nt(d.nj);
+  if (d2h(c, nt.data(), d.ntrack, sizeof(int) * d.nj)) return -1;
+  for (int j = 0; j < d.nj; ++j) {
+    if (ntrack) ntrack[j] = nt[j];
+    if (nact) nact[j] = nt[j] - 2 * d.ngst;
+    if (iL) iL[j] = d.ngst - 1;
+    if (iR) iR[j] = nt[j] - d.ngst;
+  }
+  return 0;
+}
+
+int gammahip_time(gammahip_ctx* c, double* t, double* dt) {
+  Dev& d = c->d;
+  if (t && d2h(c, t, d.tt, sizeof(double) * d.nj)) return -1;
+  if (dt && d2h(c, dt, d.tdt, sizeof(double) * d.nj)) return -1;
+  return 0;
+}
+
+int gammahip_download(gammahip_ctx* c, int field, double* out) {
+  Dev& d = c->d;
+  size_t cs = d.cs;
+  int nq = c->cfg.nq, nd = d.nd;
+  auto sidx = [&](int q) { return (nd == 1 && q > 3) ? q - 1 : q; };
+  auto plane = [&](const double* src) { return d2h(c, out, src, sizeof(double) * cs); };
+  if (field >= 0 && field < nq) {
+    if (nd == 1 && field == 3) { memset(out, 0, sizeof(double) * cs); return 0; }
+    return plane(d.prim + (size_t)sidx(field) * cs);
+  }
+  if (field >= 100 && field < 100 + nq) {
+    int q = field - 100;
+    std::vector<double> p((size_t)d.nqs * cs), x(cs), dx(cs), ty(d.nj), tdy(d.nj);
+    std::vector<int> tg(d.nj);
+    if (d2h(c, p.data(), d.prim, sizeof(double) * p.size()) || d2h(c, x.data(), d.cx, sizeof(double) * cs) ||
+        d2h(c, dx.data(), d.cdx, sizeof(double) * cs) || d2h(c, ty.data(), d.ty, sizeof(double) * d.nj) ||
+        d2h(c, tdy.data(), d.tdy, sizeof(double) * d.nj) || d2h(c, tg.data(), d.tgeom, sizeof(int) * d.nj))
+      return -1;
+    for (int j = 0; j < d.nj; ++j)
+      for (int i = 0; i < d.nax; ++i) {
+        size_t k = (size_t)j * d.nax + i;
+        double pv[10], cv[10];
+        for (int qq = 0; qq < nq; ++qq) pv[qq] = (nd == 1 && qq == 3) ? 0. : p[(size_t)sidx(qq) * cs + k];
+        if (nd == 1) {
+          Geo g = cell_geo<1>(x[k], dx[k], ty[j], tdy[j], tg[j]);
+          if (nq == 6) prim2cons<1, 6>(pv, cv, g.cen0, tg[j], d.P.gamma);
+          else prim2cons<1, 10>(pv, cv, g.cen0, tg[j], d.P.gamma);
+        } else {
+          if (nq == 6) prim2cons<2, 6>(pv, cv, x[k], tg[j], d.P.gamma);
+          else prim2cons<2, 10>(pv, cv, x[k], tg[j], d.P.gamma);
+        }
+        out[k] = cv[q];
+      }
+    return 0;
+  }
+  switch (field) {
+    case 200: return plane(d.cx);
+    case 201: return plane(d.cdx);
+    case 206: return plane(d.dtloc);
+    case 207: if (d.shock) return plane(d.sd); break;
+    case 209: if (d.shock) return plane(d.pspec); break;
+    case 1000: return plane(d.fx);
+    case 1001: return plane(d.fv);
+    case 1002: return plane(d.flL);
+    case 1003: return plane(d.flR);
+    default: break;
+  }
+  if (field == 208 && d.shock) {
+    std::vector<int> s(cs);
+    if (d2h(c, s.data(), d.shk, sizeof(int) * cs)) return -1;
+    for (size_t k = 0; k < cs; ++k) out[k] = s[k];
+    return 0;
+  }
+  if (field == 202 || field == 203) {
+    std::vector<double> t(d.nj);
+    if (d2h(c, t.data(), field == 202 ? d.ty : d.tdy, sizeof(double) * d.nj)) return -1;
+    for (int j = 0; j < d.nj; ++j)
+      for (int i = 0; i < d.nax; ++i) out[(size_t)j * d.nax + i] = t[j];
+    return 0;
+  }
+  if (field == 204 || field == 205) {
+    std::vector<double> x(cs), dx(cs), ty(d.nj), tdy(d.nj);
+    std::vector<int> tg(d.nj);
+    if (d2h(c, x.data(), d.cx, sizeof(double) * cs) || d2h(c, dx.data(), d.cdx, sizeof(double) * cs) ||
+        d2h(c, ty.data(), d.ty, sizeof(double) * d.nj) || d2h(c, tdy.data(), d.tdy, sizeof(double) * d.nj) ||
+        d2h(c, tg.data(), d.tgeom, sizeof(int) * d.nj))
+      return -1;
+    for (int j = 0; j < d.nj; ++j)
+      for (int i = 0; i < d.nax; ++i) {
+        size_t k = (size_t)j * d.nax + i;
+        Geo g = nd == 1 ? cell_geo<1>(x[k], dx[k], ty[j], tdy[j], tg[j]) : cell_geo<2>(x[k], dx[k], ty[j], tdy[j], tg[j]);
+        out[k] = field == 204 ? g.dV : g.cen0;
+      }
+    return 0;
+  }
+  if ((field >= 300 && field < 300 + nq) || (field >= 400 && field < 400 + nq)) {
+    if (nd != 2) { memset(out, 0, sizeof(double) * cs); return 0; }
+    int q = field % 100;
+    return plane((field < 400 ? d.gmv : d.gf1) + (size_t)q * cs);
+  }
+  if (field >= 1010 && field < 1010 + nq) {
+    int q = field - 1010;
+    if (nd == 1 && q == 3) { memset(out, 0, sizeof(double) * cs); return 0; }
+    return plane(d.fflux + (size_t)sidx(q) * cs);
+  }
+  return fail(c, "unknown field");
+}
+
+int gammahip_stats_get(gammahip_ctx* c, gammahip_stats* st) {
+  unsigned long long h[CNT_N];
+  if (d2h(c, h, c->d.cnt, sizeof(h))) return -1;
+  memset(st, 0, sizeof(*st));
+  st->c2p_calls = h[CNT_C2P_CALLS]; st->c2p_iters = h[CNT_C2P_ITERS];
+  st->c2p_nobracket = h[CNT_NOBRACKET]; st->c2p_maxiter = h[CNT_MAXITER]; st->c2p_phicap = h[CNT_PHICAP];
+  st->negative_dt = h[CNT_NEGDT]; st->nan_flux = h[CNT_NANFLUX]; st->f1_overflow = h[CNT_F1OVF];
+  std::vector<long long> s(c->d.nj), m(c->d.nj);
+  if (d2h(c, s.data(), c->d.nsplit, sizeof(long long) * c->d.nj)) return -1;
+  if (d2h(c, m.data(), c->d.nmerge, sizeof(long long) * c->d.nj)) return -1;
+  for (int j = 0; j < c->d.nj; ++j) { st->splits += s[j]; st->merges += m[j]; }
+  st->mlp_used = h[CNT_MLP];
+  return 0;
+}
+
+int gammahip_regrid_counts(gammahip_ctx* c, long long* splits, long long* merges) {
+  if (d2h(c, splits, c->d.nsplit, sizeof(long long) * c->d.nj)) return -1;
+  return d2h(c, merges, c->d.nmerge, sizeof(long long) * c->d.nj);
+}
+
+int gammahip_kernel_times(gammahip_ctx* c, double* ms, long long* n) {
+  c->timing = true;     // the first call switches event timing on
+  for (int k = 0; k < 16; ++k) {
+    ms[k] = c->kt[k].n ? c->kt[k].ms / c->kt[k].n : 0.;
+    n[k] = c->kt[k].n;
+    c->kt[k].ms = 0.;
+    c->kt[k].n = 0;
+  }
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,559 @@
+// hydro.hpp -- pointwise FP64 physics of GAMMA's hot path for gfx950 kernels.
+//
+// Register-resident state vectors (prim/cons/flux of one state), templated on
+// the track dimensionality ND (1 or 2) and the number of advected variables NQ
+// (6, or 10 with the synchrotron tracers).  Geometry is a run-time value that
+// is uniform across a track (cartesian: rel_cart + cartesian modules;
+// spherical: rel_sph + spherical[1D] modules), so branches on it never
+// diverge inside a wavefront.
+//
+// Formulas follow the reference (eliotayache-nv/GAMMA, src/) term for term and
+// in the same evaluation order, so that with contraction disabled the results
+// are bit-identical to the CPU restatement; each function cites its source.
+#pragma once
+#ifdef GH_EMU
+// CPU emulation build (test tool): kernel bodies compile as plain C++.
+#define __host__
+#define __device__
+#define __forceinline__ inline
+#else
+#include <hip/hip_runtime.h>
+#endif
+#include <math.h>
+
+#define GH_HD __host__ __device__ __forceinline__
+
+namespace gh {
+
+enum { RHO = 0, PPP = 1, UU1 = 2, UU2 = 3 };
+enum { DEN = 0, TAU = 1, SS1 = 2, SS2 = 3 };
+enum { TR1 = 4, GMN = 6, GMX = 7, GMD = 8, PSN = 9 };
+enum { CART = 0, SPH = 1 };
+
+constexpr double kPi = 3.14159265358979323846;
+constexpr double kPFloor = 1.e-10;       // constants.h:6 P_FLOOR_
+constexpr double kPspec = 2.22;          // constants.h:24 p_ (VARIABLE_PSPEC_ disabled)
+constexpr double kEpsE = 0.1;            // constants.h:25
+constexpr double kEpsB = 0.01;           // constants.h:26
+constexpr double kZeta = 1.;             // constants.h:27
+constexpr double kBig = 1.e15;           // dt_loc reset value (cell.h:32)
+
+struct Phys {          // run-time constants shared by all kernels
+  double gamma, cfl, vi, shock_thr, gamma_max_init;
+  double Nalpha, Nmp, Nme;
+};
+
+template <int NQ>
+struct State {
+  double prim[NQ];
+  double cons[NQ];
+};
+
+// FluidState::lfac  fluid.cpp:10-23 (sqrt of the norm, then squared again)
+template <int ND, int NQ>
+GH_HD double lfac(const double* prim) {
+  double u = 0;
+  for (int i = 0; i < ND; ++i) u += prim[UU1 + i] * prim[UU1 + i];
+  u = sqrt(u);
+  return sqrt(1 + u * u);
+}
+
+// FluidState::prim2cons  rel_sph.cpp:55-82 / rel_cart.cpp:37-64
+template <int ND, int NQ>
+GH_HD void prim2cons(const double* prim, double* cons, double r, int geom, double G) {
+  double rho = prim[RHO], p = prim[PPP];
+  double u = 0;
+  for (int i = 0; i < ND; ++i) u += prim[UU1 + i] * prim[UU1 + i];
+  u = sqrt(u);
+  double lf = sqrt(1 + u * u);
+  double h = 1. + p * G / (G - 1.) / rho;
+  double D = lf * rho;
+  cons[DEN] = D;
+  cons[TAU] = D * h * lf - p - D;
+  for (int i = 0; i < ND; ++i) cons[SS1 + i] = D * h * prim[UU1 + i];
+  if (ND == 1) cons[SS2] = 0.;               // unused slot in one dimension
+  else if (geom == SPH) cons[SS2] *= r;
+  for (int t = TR1; t < NQ; ++t) cons[t] = prim[t] * D;
+}
+
+// FluidState::state2flux along direction n  rel_sph.cpp:85-113 / rel_cart.cpp:67-95
+template <int ND, int NQ>
+GH_HD void state2flux(const double* prim, const double* cons, double* flux, int n,
+                      double r, int geom) {
+  double p = prim[PPP];
+  double u = 0;
+  for (int i = 0; i < ND; ++i) u += prim[UU1 + i] * prim[UU1 + i];
+  u = sqrt(u);
+  double lf = sqrt(1 + u * u);
+  double D = cons[DEN];
+  double ss[2] = {cons[SS1], ND == 2 ? cons[SS2] : 0.};
+  if (ND == 2 && geom == SPH) ss[1] /= r;
+  double un = prim[UU1 + n];
+  flux[DEN] = D * un / lf;
+  for (int i = 0; i < ND; ++i) {
+    if (i == n) flux[SS1 + i] = ss[i] * un / lf + p;
+    else flux[SS1 + i] = ss[i] * un / lf;
+  }
+  if (ND == 1) flux[SS2] = 0.;
+  else if (geom == SPH) flux[SS2] *= r;
+  flux[TAU] = ss[n] - D * un / lf;
+  for (int t = TR1; t < NQ; ++t) flux[t] = cons[t] * un / lf;
+}
+
+// cons2prim residual  rel_sph.cpp:191-210 (fabs) / rel_cart.cpp:108-123
+GH_HD double c2p_f(double p, double D, double S, double E, double G, int geom) {
+  double w = 1. - (S * S) / ((E + p) * (E + p));
+  double lf = 1. / sqrt(geom == SPH ? fabs(w) : w);
+  return (E + p - D * lf - (G * p * lf * lf) / (G - 1.));
+}
+
+// d f / d p for the Newton solver (analytic derivative of c2p_f)
+GH_HD double c2p_df(double p, double D, double S, double E, double G, int geom, double* fval) {
+  double Ep = E + p;
+  double w = 1. - (S * S) / (Ep * Ep);
+  double aw = geom == SPH ? fabs(w) : w;
+  double lf = 1. / sqrt(aw);
+  double gg = G / (G - 1.);
+  *fval = (E + p - D * lf - (G * p * lf * lf) / (G - 1.));
+  // d lf / dp = -lf^3 S^2 / (E+p)^3 (sign of w folded in for |w|)
+  double dw = 2. * (S * S) / (Ep * Ep * Ep);
+  double dlf = -0.5 * lf * lf * lf * (geom == SPH && w < 0 ? -dw : dw);
+  return 1. - D * dlf - gg * (lf * lf + 2. * p * lf * dlf);
+}
+
+enum { C2P_BRENT = 0, C2P_NEWTON = 1 };
+
+struct C2PStat { int iters; int fail; };   // fail: 1 no bracket, 2 max iter, 4 p_hi cap
+
+// Brent's method (Brent 1973, ch. 4) with the GSL fsolver contract used by
+// rel_sph.cpp:273-285: root = best iterate b, bracket = [min(b,c), max(b,c)],
+// stop on |hi-lo| < 1e-13 * min(|lo|,|hi|) (0 if the bracket straddles 0).
+GH_HD double brent(double D, double S, double E, double G, int geom, double lo, double hi,
+                   int maxit, C2PStat* st) {
+  double a = lo, b = hi;
+  double fa = c2p_f(lo, D, S, E, G, geom), fb = c2p_f(hi, D, S, E, G, geom);
+  double c = b, fc = fb, d = b - a, e = b - a;
+  double root = 0.5 * (lo + hi), xl = lo, xu = hi;
+  int it = 0;
+  for (;;) {
+    it++;
+    bool ac_equal = false;
+    if ((fb < 0 && fc < 0) || (fb > 0 && fc > 0)) { ac_equal = true; c = a; fc = fa; d = b - a; e = b - a; }
+    if (fabs(fc) < fabs(fb)) {
+      ac_equal = true;
+      a = b; b = c; c = a;
+      fa = fb; fb = fc; fc = fa;
+    }
+    double tol = 0.5 * 2.2204460492503131e-16 * fabs(b);
+    double m = 0.5 * (c - b);
+    bool done = false;
+    if (fb == 0) { root = b; xl = b; xu = b; done = true; }
+    else if (fabs(m) <= tol) {
+      root = b;
+      if (b < c) { xl = b; xu = c; } else { xl = c; xu = b; }
+      done = true;
+    }
+    if (!done) {
+      if (fabs(e) < tol || fabs(fa) <= fabs(fb)) { d = m; e = m; }
+      else {
+        double pp, q, rr, s = fb / fa;
+        if (ac_equal) { pp = 2 * m * s; q = 1 - s; }
+        else {
+          q = fa / fc; rr = fb / fc;
+          pp = s * (2 * m * q * (q - rr) - (b - a) * (rr - 1));
+          q = (q - 1) * (rr - 1) * (s - 1);
+        }
+        if (pp > 0) q = -q; else pp = -pp;
+        double l1 = 3 * m * q - fabs(tol * q), l2 = fabs(e * q);
+        if (2 * pp < (l1 < l2 ? l1 : l2)) { e = d; d = pp / q; }
+        else { d = m; e = m; }
+      }
+      a = b; fa = fb;
+      if (fabs(d) > tol) b += d; else b += (m > 0 ? +tol : -tol);
+      fb = c2p_f(b, D, S, E, G, geom);
+      root = b;
+      if ((fb < 0 && fc < 0) || (fb > 0 && fc > 0)) c = a;
+      if (b < c) { xl = b; xu = c; } else { xl = c; xu = b; }
+    }
+    double mn = 0.;
+    if ((xl > 0.0 && xu > 0.0) || (xl < 0.0 && xu < 0.0)) mn = fabs(xl) < fabs(xu) ? fabs(xl) : fabs(xu);
+    if (fabs(xu - xl) < 1.e-13 * mn) break;
+    if (it >= maxit) { st->fail |= 2; break; }
+  }
+  st->iters += it;
+  return root;
+}
+
+// Safeguarded Newton on the same bracket: Newton steps from the guess, a
+// bisection step whenever Newton leaves the bracket; converged when the step
+// is below 1e-14 relative (one quadratic step past the reference's 1e-13).
+GH_HD double newton(double D, double S, double E, double G, int geom, double lo, double hi,
+                    double x0, int maxit, C2PStat* st) {
+  double x = (x0 > lo && x0 < hi) ? x0 : 0.5 * (lo + hi);
+  int it = 0;
+  for (;;) {
+    it++;
+    double f;
+    double df = c2p_df(x, D, S, E, G, geom, &f);
+    if (f == 0.) break;
+    if (f > 0) lo = x; else hi = x;      // f decreases through the root
+    double xn = x - f / df;
+    if (!(xn > lo && xn < hi)) xn = 0.5 * (lo + hi);
+    double dx = fabs(xn - x);
+    x = xn;
+    if (dx <= 1.e-14 * fabs(x) || hi - lo <= 1.e-14 * fabs(lo)) break;
+    if (it >= maxit) { st->fail |= 2; break; }
+  }
+  st->iters += it;
+  return x;
+}
+
+// FluidState::cons2prim  rel_sph.cpp:213-320 / rel_cart.cpp:125-229.
+// prim[PPP] on entry is the previous pressure (the reference's guess);
+// guess_override > 0 replaces it as the starting point (learned guess).
+// cons on exit = prim2cons(prim) (rel_sph.cpp:318).
+template <int ND, int NQ>
+GH_HD void cons2prim(double* prim, double* cons, double r, int geom, double G, int solver,
+                     double guess_override, int maxit, C2PStat* st) {
+  double D = cons[DEN], tau = cons[TAU], E = D + tau;
+  double ss[2] = {cons[SS1], ND == 2 ? cons[SS2] : 0.};
+  if (ND == 2 && geom == SPH) ss[1] /= r;
+  double S2 = 0.;
+  for (int d = 0; d < ND; ++d) S2 += ss[d] * ss[d];
+  double S = sqrt(S2);
+  double pprev = geom == SPH ? fmax(1.e-13, prim[PPP]) : prim[PPP];
+  double pCand = pprev;
+  double start = guess_override > 0. ? guess_override : pCand;
+  double f_init = c2p_f(solver == C2P_BRENT ? pCand : start, D, S, E, G, geom);
+  if (solver == C2P_NEWTON) pCand = start;
+  if (fabs(f_init) > 0.) {
+    double p_lo = fmax(0., (1. + 1e-13) * S - E), p_hi;
+    bool find_p = true;
+    if (c2p_f(p_lo, D, S, E, G, geom) < 0) find_p = false;
+    if (f_init < 0) p_hi = 1. * pCand;
+    else {
+      p_hi = pCand;
+      int guard = 0;
+      while (c2p_f(p_hi, D, S, E, G, geom) > 0) {
+        p_hi *= 10;
+        if (++guard > 400) { find_p = false; st->fail |= 4; break; }
+      }
+    }
+    if (p_hi < p_lo) find_p = false;
+    if (find_p) {
+      if (solver == C2P_BRENT) pCand = brent(D, S, E, G, geom, p_lo, p_hi, maxit, st);
+      else pCand = newton(D, S, E, G, geom, p_lo, p_hi, pCand, 100, st);
+    } else {
+      st->fail |= 1;
+      pCand = pprev;
+    }
+  } else if (solver == C2P_NEWTON) {
+    pCand = start;
+  }
+  double p = pCand;
+  double Ep2 = (E + p) * (E + p);
+  double v2 = S2 / Ep2;
+  double lf = 1. / sqrt(fabs(1. - v2));
+  double rho = D / lf;
+  if (fabs(S) == 0.) {
+    for (int d = 0; d < ND; ++d) prim[UU1 + d] = 0;
+  } else {
+    double v = sqrt(1. - 1. / (lf * lf));
+    for (int d = 0; d < ND; ++d) prim[UU1 + d] = lf * v * (ss[d] / S);
+  }
+  if (geom == SPH) p = fmax(p, kPFloor);
+  prim[PPP] = p;
+  prim[RHO] = rho;
+  for (int t = TR1; t < NQ; ++t) prim[t] = cons[t] / D;
+  prim2cons<ND, NQ>(prim, cons, r, geom, G);
+}
+
+// ---- HLLC (Mignone & Bodo 2006) through a moving face ----
+struct Waves { double lL, lR, lS; };
+
+// Interface::wavespeedEstimates + computeLambda  rel_sph.cpp:322-394 / rel_cart.cpp:232-302
+template <int ND, int NQ>
+GH_HD Waves lambdas(const double* pL, const double* cL, const double* pR, const double* cR,
+                    int dim, double rface, int geom, double G) {
+  int u = UU1 + dim, s = SS1 + dim;
+  double lfL = lfac<ND, NQ>(pL), lfR = lfac<ND, NQ>(pR);
+  double vL = pL[u] / lfL, vR = pR[u] / lfR;
+  double cSL = sqrt(G * pL[PPP] / (pL[RHO] + pL[PPP] * G / (G - 1.)));
+  double cSR = sqrt(G * pR[PPP] / (pR[RHO] + pR[PPP] * G / (G - 1.)));
+  double sgSL = cSL * cSL / (lfL * lfL * (1. - cSL * cSL));
+  double sgSR = cSR * cSR / (lfR * lfR * (1. - cSR * cSR));
+  double l1, l2;
+  Waves w;
+  l1 = (vR - sqrt(sgSR * (1. - vR * vR + sgSR))) / (1. + sgSR);
+  l2 = (vL - sqrt(sgSL * (1. - vL * vL + sgSL))) / (1. + sgSL);
+  w.lL = fmin(l1, l2);
+  l1 = (vR + sqrt(sgSR * (1. - vR * vR + sgSR))) / (1. + sgSR);
+  l2 = (vL + sqrt(sgSL * (1. - vL * vL + sgSL))) / (1. + sgSL);
+  w.lR = fmax(l1, l2);
+  bool sph = geom == SPH;
+  double d = (sph && dim == 1) ? rface : 1.;
+  double mL = sph ? cL[s] / d : cL[s];
+  double mR = sph ? cR[s] / d : cR[s];
+  double EL = cL[TAU] + cL[DEN], ER = cR[TAU] + cR[DEN];
+  double lL = w.lL, lR = w.lR;
+  double AL = lL * EL - mL, AR = lR * ER - mR;
+  double BL = mL * (lL - vL) - pL[PPP], BR = mR * (lR - vR) - pR[PPP];
+  double FhllE = (lL * AR - lR * AL) / (lR - lL);
+  double Ehll = (AR - AL) / (lR - lL);
+  double Fhllm = (lL * BR - lR * BL) / (lR - lL);
+  double mhll = (BR - BL) / (lR - lL);
+  bool degenerate = sph ? (fabs(FhllE) == 0) : (fabs(FhllE) < 1.e-15);
+  if (degenerate) { w.lS = mhll / (Ehll + Fhllm); return w; }
+  double ef = Ehll + Fhllm;
+  double delta = ef * ef - 4. * FhllE * mhll;
+  w.lS = ((Ehll + Fhllm) - sqrt(delta)) / (2. * FhllE);
+  return w;
+}
+
+// Interface::starState  rel_sph.cpp:397-450 / rel_cart.cpp:305-355 -> star cons
+template <int ND, int NQ>
+GH_HD void star_state(const double* pin, const double* cin, double lbda, double lS, int dim,
+                      double rface, int geom, double* out) {
+  int u = UU1 + dim, s = SS1 + dim;
+  bool sph = geom == SPH;
+  double mm[2] = {cin[SS1], ND == 2 ? cin[SS2] : 0.};
+  if (ND == 2 && sph) mm[1] /= rface;
+  double lf = lfac<ND, NQ>(pin);
+  double v = pin[u] / lf;
+  double m = mm[dim];
+  double p = pin[PPP];
+  double D = cin[DEN];
+  double E = cin[TAU] + cin[DEN];
+  double A = lbda * E - m;
+  double B = m * (lbda - v) - p;
+  double pS = (A * lS - B) / (1. - lbda * lS);
+  double k = (lbda - v) / (lbda - lS);
+  for (int q = 0; q < NQ; ++q) out[q] = 0.;
+  out[DEN] = D * k;
+  out[s] = (m * (lbda - v) + pS - p) / (lbda - lS);
+  if (ND == 2) { int o = 1 - dim; out[SS1 + o] = mm[o] * k; }
+  if (ND == 2 && sph) out[SS2] *= rface;
+  out[TAU] = (E * (lbda - v) + pS * lS - p * v) / (lbda - lS) - D * k;
+  for (int t = TR1; t < NQ; ++t) out[t] = cin[t] * k;
+}
+
+// Interface::computeFlux  hllc.cpp:12-39: sequential region tests against the
+// face speed v (last matching one wins), F - v U*, times dA.  fL/fR are the
+// physical fluxes of the two states along dim.  NaN wave speeds leave the
+// flux NaN (flagged by the caller).
+template <int ND, int NQ>
+GH_HD void hllc_flux(const double* pL, const double* cL, const double* fL,
+                     const double* pR, const double* cR, const double* fR,
+                     const Waves& w, double v, double dA, int dim, double rface, int geom,
+                     double* flux) {
+  int region = -1;
+  if (v < w.lL) region = 0;
+  if (w.lL <= v && v < w.lS) region = 1;
+  if (w.lS <= v && v < w.lR) region = 2;
+  if (w.lR <= v) region = 3;
+  if (region == 0 || region == 3) {
+    const double* c = region == 0 ? cL : cR;
+    const double* f = region == 0 ? fL : fR;
+    for (int q = 0; q < NQ; ++q) flux[q] = f[q] - v * c[q];
+  } else if (region == 1 || region == 2) {
+    const double* pK = region == 1 ? pL : pR;
+    const double* cK = region == 1 ? cL : cR;
+    const double* fK = region == 1 ? fL : fR;
+    double lK = region == 1 ? w.lL : w.lR;
+    double st[NQ];
+    star_state<ND, NQ>(pK, cK, lK, w.lS, dim, rface, geom, st);
+    for (int q = 0; q < NQ; ++q) flux[q] = fK[q] + lK * (st[q] - cK[q]) - v * st[q];
+  } else {
+    for (int q = 0; q < NQ; ++q) flux[q] = __builtin_nan("");
+  }
+  for (int q = 0; q < NQ; ++q) flux[q] *= dA;
+}
+
+// ---- shock detector: radiation_sph.cpp:37-158 (Rezzolla & Zanotti 2013) ----
+// Returns -1 without touching *pspec when no shock is possible (:56-57).
+template <int ND, int NQ>
+GH_HD double shock_sd(const double* S1, const double* S2, int dim, bool reverse, double G,
+                      double* pspec) {
+  int uux = UU1 + dim, uut = UU2 - dim;
+  const int n_evals = 10;
+  double chi = 0.5;
+  double p1 = S1[PPP], p2 = S2[PPP];
+  if (fabs((p1 - p2) / p1) < 1.e-10) return (-1);
+  if (p1 < p2) return (-1);
+  double delta_p = p2 - p1;
+  double dp = delta_p / (double)n_evals;
+  double rho1 = S1[RHO], rho2 = S2[RHO];
+  double h1 = 1 + p1 * G / (G - 1.) / rho1;
+  double h2 = 1 + p2 * G / (G - 1.) / rho2;
+  double lfac1 = lfac<ND, NQ>(S1), lfac2 = lfac<ND, NQ>(S2);
+  double vx1 = S1[uux] / lfac1, vx2 = S2[uux] / lfac2;
+  if (reverse) { vx1 *= -1; vx2 *= -1; }
+  double ut1 = 0;
+  if (ND > 1) ut1 = S1[uut];
+  double v12 = (vx1 - vx2) / (1 - vx1 * vx2);
+  double A1 = h1 * ut1;
+  double I = 0;
+  for (int ip = 0; ip < n_evals; ++ip) {
+    double p = p1 + (ip + .5) * dp;
+    double rho = rho1 * pow(p / p1, 1. / G);
+    double h = 1 + p * G / (G - 1.) / rho;
+    double cs = sqrt(G * p / (rho + p * G / (G - 1.)));
+    double dI = sqrt(h * h + A1 * A1 * (1. - cs * cs)) / ((h * h + A1 * A1) * rho * cs) * dp;
+    I += dI;
+  }
+  double vSR = tanh(I);
+  double rho22 = rho2 * rho2, lfac22 = lfac2 * lfac2, vx22 = vx2 * vx2;
+  double g = G, gm1 = (g - 1.), gm12 = gm1 * gm1;
+  double Da = 4. * g * p1 * ((g - 1) * p2 + p1) / (gm12 * (p1 - p2) * (p1 - p2));
+  double Db = h2 * (p2 - p1) / rho2 - h2 * h2;
+  double Dd = 1. - Da * Db;
+  double h3 = (sqrt(Dd) - 1) * gm1 * (p1 - p2) / (2 * (gm1 * p2 + p1));
+  double J232a = -g / gm1 * (p1 - p2);
+  double J232b = h3 * (h3 - 1) / p1 - h2 * (h2 - 1) / p2;
+  double J232 = J232a / J232b;
+  double J23 = sqrt(J232);
+  double Vsa = rho22 * lfac22 * vx2 + fabs(J23) * sqrt(J232 + rho22 * lfac22 * (1. - vx22));
+  double Vsb = rho22 * lfac22 + J232;
+  double Vs = Vsa / Vsb;
+  double v2Sa = (p1 - p2) * (1. - vx2 * Vs);
+  double v2Sb = (Vs - vx2) * (h2 * rho2 * lfac22 * (1. - vx22) + p1 - p2);
+  double v2S = v2Sa / v2Sb;
+  double vlim = vSR + chi * (v2S - vSR);
+  *pspec = kPspec;
+  return v12 - vlim;
+}
+
+// ---- geometry (Geometry/*.cpp) ----
+struct Geo { double dV, dl0, dl1, cen0, cen1; };
+
+template <int ND>
+GH_HD Geo cell_geo(double x, double dx, double y, double dy, int geom) {
+  Geo g;
+  if (geom == CART) {
+    g.dl0 = dx; g.dl1 = dy;
+    double dV = 1.;
+    dV *= dx;
+    if (ND == 2) dV *= dy;
+    g.dV = dV;
+    g.cen0 = x; g.cen1 = y;
+  } else if (ND == 1) {                        // spherical1D.cpp:12-48
+    g.dl0 = dx; g.dl1 = 0.;
+    g.dV = 4. * kPi * x * x * dx;
+    double r2 = x * x, dr2 = dx * dx;
+    g.cen0 = 3. * x * (r2 + dr2) / (3 * r2 + dr2);
+    g.cen1 = y;
+  } else {                                     // spherical.cpp:12-69
+    double r = x, dr = dx, th = y, dth = dy;
+    g.dl0 = dr; g.dl1 = r * dth;
+    double r1 = r - dr / 2., r2 = r + dr / 2.;
+    double rsq = r1 * r1 + r1 * r2 + r2 * r2;
+    g.dV = 4. / 3. * kPi * rsq * dr * fabs(sin(.5 * dth) * sin(th));
+    double rr = r * r, dr2 = dr * dr;
+    double th1 = th - dth / 2., th2 = th + dth / 2.;
+    g.cen0 = 3. * r * (rr + dr2) / (3 * rr + dr2);
+    g.cen1 = (th1 * cos(th1) - th2 * cos(th2) + fabs(sin(th2)) - fabs(sin(th1))) / (cos(th1) - cos(th2));
+  }
+  return g;
+}
+
+// Interface::computedA for a face normal to dim at (x, y) with transverse
+// extent dxf (cartesian.cpp:135-143, spherical1D.cpp:60-76, spherical.cpp:81-112)
+template <int ND>
+GH_HD double face_area(int dim, double x, double y, double dxf, int geom) {
+  if (geom == CART) return ND == 1 ? 1. : dxf;
+  if (ND == 1) return 4. * kPi * x * x;
+  if (dim == 0) return 4. * kPi * x * x * fabs(sin(.5 * dxf) * sin(y));
+  double dr = dxf, r1 = x - dr / 2., r2 = x + dr / 2.;
+  double rsq = (r1 + r2) * dr;
+  return kPi * fabs(sin(y)) * rsq;
+}
+
+GH_HD double minmod(double a, double b) {      // 1d.cpp:250-260
+  if (fabs(a) < fabs(b) && a * b > 0) return (a);
+  if (fabs(b) < fabs(a) && a * b > 0) return (b);
+  return (0);
+}
+
+// Cell::sourceTerms  rel_sph.cpp:452-496 (cartesian: none)
+template <int ND, int NQ>
+GH_HD void sources(const double* prim, double* U, double x, double dx, double y, double dV,
+                   double dt, int geom, double G) {
+  if (geom != SPH) return;
+  double r = x, dr = dx;
+  double r1 = r - dr / 2., r2 = r + dr / 2.;
+  double rsq = (r2 * r2 + r1 * r1 + r2 * r1) / 3.;
+  if (ND == 1) {
+    U[SS1] += (2. * prim[PPP]) * (r / rsq) * dV * dt;
+  } else {
+    double rho = prim[RHO], ut = prim[UU2], p = prim[PPP];
+    double h = 1. + p * G / (G - 1.) / rho;
+    U[SS1] += (rho * h * ut * ut + 2. * p) * (r / rsq) * dV * dt;
+    U[SS2] += p * cos(y) / fabs(sin(y)) * dV * dt;
+  }
+}
+
+// Cell::radiativeSourceTerms  radiation_sph.cpp:285-300
+GH_HD void radiative(const double* prim, double* U, double dV, double dt, const Phys& P) {
+  double rho = prim[RHO], p = prim[PPP], gma = P.gamma;
+  double h = 1 + p * gma / (gma - 1.) / rho;
+  double eps = rho * (h - 1.) / gma;
+  double eB = kEpsB * eps;
+  double B = sqrt(8. * kPi * eB);
+  double dgma = P.Nalpha * pow(rho, 4. / 3.) * B * B * dV * dt;
+  U[GMX] += dgma;
+  U[GMN] += dgma;
+}
+
+// Cell::radiation_injectParticles  radiation_sph.cpp:196-283
+template <int ND, int NQ>
+GH_HD void inject(double* prim, bool shocked, double pspec, const Phys& P) {
+  double lf = lfac<ND, NQ>(prim), rho = prim[RHO];
+  double r43 = pow(rho, 4. / 3.);
+  double lim = lf * r43 / (lf * rho);
+  if (shocked) {
+    if (pspec > prim[PSN] || isnan(prim[PSN])) prim[PSN] = pspec;
+    double gmax = P.gamma_max_init;
+    prim[GMX] = (gmax != 0. ? lf * r43 / gmax : 0.) / (lf * rho);
+    double gma = P.gamma, p = prim[PPP], psyn = prim[PSN];
+    double h = 1. + p * gma / (gma - 1.) / rho;
+    double eps = rho * (h - 1.) / gma;
+    double ee = kEpsE * eps;
+    double ne = kZeta * rho / P.Nmp;
+    double lfac_av = ee / (ne * P.Nme);
+    double gmin = (psyn - 2.) / (psyn - 1.) * lfac_av;
+    prim[GMN] = (gmin != 0. ? lf * r43 / gmin : 0.) / (lf * rho);
+  }
+  if (isnan(prim[GMX])) prim[GMX] = lim;
+  if (isnan(prim[GMN])) prim[GMN] = lim;
+}
+
+// Cell::radiation_initGammas  radiation_sph.cpp:246-260 (prim side)
+template <int ND, int NQ>
+GH_HD void init_gammas(double* prim) {
+  double lf = lfac<ND, NQ>(prim), rho = prim[RHO];
+  double lim = lf * pow(rho, 4. / 3.);
+  if (prim[GMX] > lim / (lf * rho) || prim[GMX] <= 0.) prim[GMX] = lim / (lf * rho);
+  if (prim[GMN] > lim / (lf * rho) || prim[GMN] <= 0.) prim[GMN] = lim / (lf * rho);
+}
+
+// BM analytic profile in code units (BM1D.cpp:62-86, c = 1, k = 0):
+// bm = {lfacShock2(t0), t0, rho_ambient, eta}
+GH_HD void bm_profile(const double* bm, double r, double t, double G, double* rho, double* u,
+                      double* p) {
+  double L0 = bm[0], t0 = bm[1], rhoa = bm[2], eta = bm[3];
+  double tr = t0 / t;
+  double L = L0 * tr * tr * tr;
+  double pa = rhoa * eta;
+  double ha = 1. + pa * G / (G - 1.) / rhoa;
+  double pf = 2. / 3. * L * ha * rhoa;
+  double lfacf = sqrt(fmax(1., .5 * L));
+  double Df = 2. * L * rhoa;
+  double chi = (1. + 8. * L) * (1. - r / t);
+  *p = pf * pow(chi, -17. / 12.);
+  double lf = sqrt(lfacf * lfacf / chi + 1);
+  double Dn = Df * pow(chi, -7. / 4.);
+  *rho = Dn / lf;
+  *u = sqrt(1. - 1. / (lf * lf)) * lf;
+}
+
+}  // namespace gh

@@ -1,0 +1,212 @@
+"""Problem setups written against GAMMA's hook API.
+
+A setup plays the role of one ``src/Initial/Tests/*.cpp`` file of the
+reference: ``loadParams`` (sizes), ``Grid::initialGeometry`` /
+``Grid::initialValues`` (the initial active cells), ``userKinematics`` /
+``userBoundaries`` (boundary faces and ghost cells) and
+``checkCellForRegrid`` (AMR rule).  Initial data is produced here as
+structure-of-arrays numpy buffers; the run-time hooks are selected by kind and
+parameters in :class:`RunConfig` and evaluated by the solver itself (device
+code in the HIP backend).
+
+Templates followed (reference files):
+  * 1D spherical Blandford-McKee blast wave -- Initial/Tests/BM1D.cpp:62-332
+    (code units c = 1, ambient rho = 1, R_shock = 1 at t0)
+  * planar relativistic shock tube -- Initial/Tests/1DST.cpp:19-78
+  * 2D cartesian Riemann problem on x tracks -- Initial/Tests/2DST.cpp:166-322
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+# variable slots (reference environment.h:33-35, 70-82)
+RHO, PPP, UU1, UU2 = 0, 1, 2, 3
+TR1, GMN, GMX, GMD, PSN = 4, 6, 7, 8, 9
+
+GEOM_CART, GEOM_SPH = 0, 1
+KIN_NONE, KIN_FIXED = 0, 1
+BND_NONE, BND_BM = 0, 1
+RG_NONE, RG_AR2D, RG_BM1D = 0, 1, 2
+
+# physical constants (reference constants.h:8-28)
+MP = 1.6726219e-24
+ME = 9.1093835e-28
+C_LIGHT = 2.99792458e10
+ALPHA_SYN = 1.29251816e-09
+
+
+def normalized_constants(rho_norm: float, v_norm: float, l_norm: float):
+    """normalizeConstants (reference constants.cpp:12-20): (Nalpha, Nmp, Nme)."""
+    n_alpha = ALPHA_SYN / (1. / (rho_norm * v_norm * l_norm))
+    n_mp = MP / (rho_norm * math.pow(l_norm, 3))
+    n_me = ME / (rho_norm * math.pow(l_norm, 3))
+    return n_alpha, n_mp, n_me
+
+
+@dataclass
+class RunConfig:
+    """Compile-time switches of environment.h plus the setup's hook choices."""
+    nd: int = 1                   # NUM_D
+    geom: int = GEOM_CART         # Hydro + Geometry module pair
+    shock: int = 0                # SHOCK_DETECTION_
+    sync: int = 0                 # LOCAL_SYNCHROTRON_
+    ngst: int = 2
+    nmax: int = 0                 # max active cells per track
+    ncell_mv: int = 0             # initial active cells per track
+    ncell_f1: int = 1             # active tracks (2D)
+    gamma: float = 5. / 3.        # GAMMA_
+    cfl: float = 0.2              # CFL_
+    vi: float = 1.                # VI
+    shock_thr: float = 0.01       # DETECT_SHOCK_THRESHOLD_
+    gamma_max_init: float = 1.e8  # GAMMA_MAX_INIT_
+    n_alpha: float = 0.
+    n_mp: float = 1.
+    n_me: float = 1.
+    kin_mode: int = KIN_NONE
+    v_in: float = 0.
+    v_out: float = 0.
+    bnd_mode: int = BND_NONE
+    bm: tuple = (0., 0., 0., 0., 0., 0.)
+    rg_mode: int = RG_NONE
+    rg: tuple = (0., 0., 0., 0., 0., 0.)
+    c2p_maxiter: int = 1000
+
+    @property
+    def nq(self) -> int:          # NUM_Q = NUM_C(4) + NUM_TR(2) + NUM_S
+        return 6 + (4 if self.sync else 0)
+
+    def replace(self, **kw) -> "RunConfig":
+        return dataclasses.replace(self, **kw)
+
+
+@dataclass
+class Setup:
+    """Initial active cells: x,dx [ncell_f1][ncell_mv], y,dy [ncell_f1],
+    prim [nq][ncell_f1][ncell_mv] with 3-velocities in the UU slots
+    (prepForRun converts them, reference 1d.cpp:934-951)."""
+    name: str
+    cfg: RunConfig
+    t0: float
+    x: np.ndarray
+    dx: np.ndarray
+    y: np.ndarray
+    dy: np.ndarray
+    prim: np.ndarray
+    meta: dict = field(default_factory=dict)
+
+
+# ---------------------------------------------------------------------------
+# counter-based PRNG (SplitMix64, Steele/Lea/Flood 2014), identical on host
+# and device: u(k) = mix(seed + (k+1) * golden) -> uniform [-1, 1)
+_GOLD = np.uint64(0x9E3779B97F4A7C15)
+
+
+def splitmix_uniform(seed: int, idx: np.ndarray) -> np.ndarray:
+    with np.errstate(over="ignore"):
+        z = np.uint64(seed) + (idx.astype(np.uint64) + np.uint64(1)) * _GOLD
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        z = z ^ (z >> np.uint64(31))
+    u = (z >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+    return 2.0 * u - 1.0
+
+
+DEFAULT_SEED = 20261202
+
+
+# ---------------------------------------------------------------------------
+def shock_tube_1d(n: int = 400, rho_l=10., p_l=13.33, rho_r=1., p_r=1.e-6,
+                  gamma=5. / 3., xmin=0., xmax=1., xmid=0.5, geom=GEOM_CART,
+                  nmax=None, shock=0) -> Setup:
+    """Planar relativistic shock tube (Marti & Mueller problem 1 by default).
+    Static boundary faces as in 2DST.cpp:261-278, no AMR."""
+    cfg = RunConfig(nd=1, geom=geom, shock=shock, nmax=nmax or n + 8, ncell_mv=n,
+                    gamma=gamma, kin_mode=KIN_FIXED, v_in=0., v_out=0.)
+    dxc = (xmax - xmin) / n
+    x = xmin + (np.arange(n) + 0.5) * dxc
+    prim = np.zeros((cfg.nq, 1, n))
+    left = x <= xmid
+    prim[RHO, 0] = np.where(left, rho_l, rho_r)
+    prim[PPP, 0] = np.where(left, p_l, p_r)
+    prim[TR1, 0] = np.where(left, 1., 2.)
+    return Setup("shock_tube_1d", cfg, 0., x[None, :], np.full((1, n), dxc),
+                 np.zeros(1), np.ones(1), prim)
+
+
+def bm_profile(r, t, lfac_shock2_0, t0, rho_a, eta, gamma):
+    """Blandford-McKee self-similar profile in code units (c = 1, k = 0);
+    BM1D.cpp:62-86 with lfacShock2(t) = lfacShock2(t0) (t0/t)^3."""
+    tr = t0 / t
+    L = lfac_shock2_0 * tr * tr * tr
+    pa = rho_a * eta
+    ha = 1. + pa * gamma / (gamma - 1.) / rho_a
+    pf = 2. / 3. * L * ha * rho_a
+    lfacf = math.sqrt(max(1., .5 * L))
+    Df = 2. * L * rho_a
+    chi = (1. + 8. * L) * (1. - r / t)
+    p = pf * np.power(chi, -17. / 12.)
+    lfac = np.sqrt(lfacf * lfacf / chi + 1)
+    D = Df * np.power(chi, -7. / 4.)
+    rho = D / lfac
+    v = np.sqrt(1. - 1. / (lfac * lfac))
+    return rho, v, p
+
+
+def blandford_mckee_1d(n: int = 4000, lfac_shock=100., eta=1.e-10, gamma=4. / 3.,
+                       nmax=None, sync=1, shock=1, dth=5.e-4) -> Setup:
+    """Config: 1D spherical BM blast wave, shock Lorentz factor 100, ambient
+    rho = 1, p = eta rho c^2, cells geometrically spaced on [0.5, 1.5] R_sh
+    with r_{i+1/2} = 0.5 * 3^(i/n).  Boundary faces as BM1D.cpp:239-244
+    (v_in = 0, v_out = 1.5), inner ghosts from the analytic solution
+    (BM1D.cpp:254-274), AMR rule of BM1D.cpp:277-319."""
+    L0 = lfac_shock * lfac_shock
+    t0 = 1. / (1. - 1. / (1. + 8. * L0))          # R_shock(t0) = 1 (BM1D.cpp:42)
+    rho_norm, v_norm, l_norm = MP, C_LIGHT, C_LIGHT  # BM1D.cpp:20-23
+    na, nmp, nme = normalized_constants(rho_norm, v_norm, l_norm)
+    cfg = RunConfig(nd=1, geom=GEOM_SPH, shock=shock, sync=sync, nmax=nmax or int(n * 1.25),
+                    ncell_mv=n, gamma=gamma, n_alpha=na, n_mp=nmp, n_me=nme,
+                    kin_mode=KIN_FIXED, v_in=0., v_out=1.5,
+                    bnd_mode=BND_BM, bm=(L0, t0, 1., eta, 0., 0.),
+                    rg_mode=RG_BM1D, rg=(3., 0.1, dth, eta, 1., 0.))
+    i = np.arange(n + 1)
+    edges = 0.5 * np.power(3., i / n)
+    x = 0.5 * (edges[1:] + edges[:-1])
+    dx = edges[1:] - edges[:-1]
+    prim = np.zeros((cfg.nq, 1, n))
+    rshock = 1.
+    inside = x <= rshock
+    rho_b, v_b, p_b = bm_profile(np.minimum(x, rshock * 0.999999), t0, L0, t0, 1., eta, gamma)
+    prim[RHO, 0] = np.where(inside, rho_b, 1.)
+    prim[UU1, 0] = np.where(inside, v_b, 0.)
+    prim[PPP, 0] = np.where(inside, p_b, eta)
+    prim[TR1, 0] = np.where(inside, 2., 1.)
+    return Setup("blandford_mckee_1d", cfg, t0, x[None, :], dx[None, :],
+                 np.zeros(1), np.ones(1), prim, meta=dict(L0=L0, eta=eta))
+
+
+def riemann_2d(nx: int = 2048, ny: int = 2048, rho_l=10., p_l=13.33, rho_r=1.,
+               p_r=1.e-6, gamma=5. / 3., amp=1.e-3, seed=DEFAULT_SEED,
+               nmax=None, regrid=True) -> Setup:
+    """Config: 2D cartesian relativistic Riemann problem on x-directed tracks
+    over [0,1]^2, diaphragm at x = 0.5, density seeded with a 1e-3 SplitMix64
+    perturbation per cell; static boundary faces and the aspect-ratio AMR rule
+    of 2DST.cpp:261-322 (split above 3, merge below 0.2)."""
+    cfg = RunConfig(nd=2, geom=GEOM_CART, nmax=nmax or (nx * 5) // 4, ncell_mv=nx,
+                    ncell_f1=ny, gamma=gamma, kin_mode=KIN_FIXED, v_in=0., v_out=0.,
+                    rg_mode=RG_AR2D if regrid else RG_NONE, rg=(3., 0.2, 0., 0., 0., 0.))
+    dxc, dyc = 1. / nx, 1. / ny
+    x1 = (np.arange(nx) + 0.5) * dxc
+    y = (np.arange(ny) + 0.5) * dyc
+    x = np.broadcast_to(x1, (ny, nx)).copy()
+    prim = np.zeros((cfg.nq, ny, nx))
+    left = x <= 0.5
+    xi = splitmix_uniform(seed, np.arange(nx * ny, dtype=np.uint64)).reshape(ny, nx)
+    prim[RHO] = np.where(left, rho_l, rho_r) * (1. + amp * xi)
+    prim[PPP] = np.where(left, p_l, p_r)
+    prim[TR1] = np.where(left, 1., 2.)
+    return Setup("riemann_2d", cfg, 0., x, np.full((ny, nx), dxc), y, np.full(ny, dyc), prim,
+                 meta=dict(seed=seed, amp=amp))

@@ -1,0 +1,121 @@
+/*
+ * gammahip.h -- C-ABI of the MI355X (gfx950) backend for GAMMA's per-step
+ * hydro hot path.
+ *
+ * The reference (eliotayache-nv/GAMMA) binds its hot path at link time: the
+ * driver loop Simu::run (src/simu.cpp:41-78) calls Grid methods declared in
+ * src/grid.h:61-112 and defined by the Dimensions/Timestep/Hydro/Solver/
+ * Geometry/Radiation module chosen in the Makefile (Makefile:1-8, 82-92).
+ * This header is the C boundary those Grid methods call when the HIP backend
+ * is selected: plain pointers and sizes, an opaque context, int status codes
+ * (0 ok, <0 error; message from gammahip_last_error), no exceptions across
+ * the ABI.  One host thread per context.  Calls are stream-ordered on the
+ * context's stream; the ones documented "synchronous" wait for it.
+ *
+ * Data layout of host buffers: structure of arrays over tracks.  A "cell
+ * field" buffer is [ntracks][nax] doubles (nax = nmax + 2*ngst, ntracks =
+ * nde_nax[F1] including ghost tracks in 2D, the number of independent runs in
+ * 1D), a "face field" buffer [ntracks][nax] (entry nax-1 unused), matching the
+ * reference's Ctot[j][i] / Itot[j][i] indexing (src/Dimensions/2d.cpp:90-94).
+ */
+#ifndef GAMMAHIP_H_
+#define GAMMAHIP_H_
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct gammahip_ctx gammahip_ctx;
+
+/* Compile-time environment of the reference (src/environment.h:46-66) plus
+ * the setup's hook selection (src/Initial/Tests/ setups) and backend options. */
+typedef struct gammahip_config {
+  int    nd;          /* NUM_D: 1 (independent tracks) or 2 (coupled tracks)          */
+  int    nq;          /* NUM_Q: 6, or 10 with LOCAL_SYNCHROTRON_                      */
+  int    geom;        /* 0 cartesian (rel_cart + cartesian), 1 spherical (rel_sph +   */
+                      /* spherical[1D]), -1 per track (1D ensembles, see tgeom)       */
+  int    shock;       /* SHOCK_DETECTION_                                             */
+  int    sync;        /* LOCAL_SYNCHROTRON_ (requires nq == 10)                       */
+  int    ngst;        /* ghost cells per side (2)                                     */
+  int    nmax;        /* max active cells per track (Grid::initialise, 1d.cpp:60)     */
+  int    ncell_mv;    /* initial active cells per track                               */
+  int    ntracks_active; /* active tracks (2D: ncell[F1]; 1D: independent runs)       */
+  double gamma, cfl, vi, shock_thr, gamma_max_init;
+  double Nalpha, Nmp, Nme;               /* normalizeConstants (constants.cpp:12-20)  */
+  int    rg_mode;     /* checkCellForRegrid: 0 none, 1 dx/dy aspect ratio (2DST.cpp:  */
+                      /* 304-322), 2 Blandford-McKee rule (BM1D.cpp:277-319)          */
+  double rg[6];       /* rule parameters {split, merge, dth, eta, target, -}          */
+  int    c2p_solver;  /* 0 Brent (reference algorithm), 1 safeguarded Newton          */
+  int    c2p_guess;   /* 0 previous pressure (reference), 1 learned MLP guess         */
+  int    c2p_maxiter; /* 1000 in the reference                                        */
+} gammahip_config;
+
+/* Per-track hook parameters (userKinematics / userBoundaries of the setup). */
+typedef struct gammahip_track_hooks {
+  int    geom;        /* used when config.geom == -1                                  */
+  int    kin_mode;    /* 0 none, 1 fixed boundary-face speeds (BM1D.cpp:239-244)      */
+  double v_in, v_out;
+  int    bnd_mode;    /* 0 outflow only, 1 Blandford-McKee inflow (BM1D.cpp:254-274)  */
+  double bm[4];       /* {lfacShock2(t0), t0, rho_ambient, eta}                       */
+} gammahip_track_hooks;
+
+typedef struct gammahip_stats {
+  unsigned long long c2p_calls, c2p_iters, c2p_nobracket, c2p_maxiter, c2p_phicap;
+  unsigned long long negative_dt, nan_flux, f1_overflow, splits, merges, mlp_used;
+  unsigned long long reserved[5];
+} gammahip_stats;
+
+/* Grid::Grid + Grid::initialise (src/Dimensions/1d.cpp:53-81, 2d.cpp:59-96):
+ * allocates device storage for nmax cells per track. */
+int  gammahip_create(const gammahip_config* cfg, gammahip_ctx** out);
+/* Grid::destruct (1d.cpp:870-875) */
+int  gammahip_destroy(gammahip_ctx* ctx);
+const char* gammahip_last_error(gammahip_ctx* ctx);
+
+/* Grid::initialGeometry/initialValues output + mpi_distribute + Grid::prepForRun
+ * (grid.cpp:20-26) + apply(radiation_initGammas) (simu.cpp:34-36) + the first
+ * Grid::updateGhosts of Simu::run (simu.cpp:43).  x,dx: [ntracks_active][ncell_mv];
+ * y,dy: [ntracks_active] (F1 centre and width, 2D); prim: [nq][ntracks_active]
+ * [ncell_mv] with 3-velocities in the UU slots; hooks: [ntracks_active];
+ * t0: start time.  Synchronous. */
+int  gammahip_load(gammahip_ctx* ctx, const double* x, const double* dx, const double* y,
+                   const double* dy, const double* prim, const gammahip_track_hooks* hooks,
+                   double t0);
+
+/* One or more iterations of the body of Simu::run (simu.cpp:47-77): regrid,
+ * resetShock, updateGhosts, prepForUpdate, detectShock, dt = CFL*collect_dt,
+ * evolve (rk3.cpp:12-195), t += dt, radiation_injectParticles.  dt stays on
+ * the device.  Asynchronous. */
+int  gammahip_step(gammahip_ctx* ctx, int nsteps);
+
+/* The same loop body as separate Grid methods (src/grid.h:61-112). */
+int  gammahip_regrid(gammahip_ctx* ctx);                       /* Grid::regrid        */
+int  gammahip_update_ghosts(gammahip_ctx* ctx);                /* Grid::updateGhosts  */
+int  gammahip_prep_for_update(gammahip_ctx* ctx);              /* Grid::prepForUpdate */
+int  gammahip_collect_dt(gammahip_ctx* ctx, double* dt_out);   /* Grid::collect_dt * CFL_, synchronous */
+int  gammahip_evolve(gammahip_ctx* ctx);                       /* Grid::evolve with the collected dt */
+enum { GAMMAHIP_RESET_SHOCK = 0, GAMMAHIP_DETECT_SHOCK = 1, GAMMAHIP_INJECT = 2,
+       GAMMAHIP_INIT_GAMMAS = 3, GAMMAHIP_ADVANCE_TIME = 4 };
+int  gammahip_apply(gammahip_ctx* ctx, int what);              /* Grid::apply(&Cell::...) */
+
+/* Synchronous downloads (Grid::printCols reads the same state).  field:
+ * 0..nq-1 primitive q, 100+q conserved q (prim2cons of the stored state),
+ * 200 x, 201 dx, 202 y, 203 dy, 204 dV, 205 centroid, 206 dt_loc, 207 Sd,
+ * 208 isShocked, 209 pspec, 300+q grad_mv, 400+q grad_f1;
+ * face fields: 1000 x, 1001 v, 1002 lL, 1003 lR, 1010+q flux. */
+int  gammahip_download(gammahip_ctx* ctx, int field, double* out /* [ntracks][nax] */);
+int  gammahip_track_info(gammahip_ctx* ctx, int* ntrack, int* nact, int* iLbnd, int* iRbnd);
+int  gammahip_dims(gammahip_ctx* ctx, int* ntracks, int* nax);
+int  gammahip_time(gammahip_ctx* ctx, double* t /* [ntracks] */, double* dt /* [ntracks] */);
+int  gammahip_stats_get(gammahip_ctx* ctx, gammahip_stats* st);
+int  gammahip_regrid_counts(gammahip_ctx* ctx, long long* splits, long long* merges);
+/* Average duration (ms) of the dominant kernel launches since the last call,
+ * measured with HIP events on the context's stream. */
+int  gammahip_kernel_times(gammahip_ctx* ctx, double* ms_per_launch /* [16] */,
+                           long long* launches /* [16] */);
+int  gammahip_sync(gammahip_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

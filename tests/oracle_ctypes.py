@@ -1,0 +1,230 @@
+"""ctypes binding of the CPU oracle (oracle/libgamma_oracle.so).
+
+Test infrastructure: only tests/, __graft_entry__.smoke() and bench.py's
+cpu_baseline leg import this module, and only as the checker / CPU baseline.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB = os.path.join(ROOT, "oracle", "libgamma_oracle.so")
+
+QMAX = 10
+
+
+class OgConfig(C.Structure):
+    _fields_ = [
+        ("nd", C.c_int), ("geom", C.c_int), ("nq", C.c_int), ("shock", C.c_int),
+        ("sync", C.c_int), ("ngst", C.c_int), ("nmax", C.c_int), ("ncell_mv", C.c_int),
+        ("ncell_f1", C.c_int),
+        ("gamma", C.c_double), ("cfl", C.c_double), ("vi", C.c_double),
+        ("shock_thr", C.c_double), ("gamma_max_init", C.c_double),
+        ("Nalpha", C.c_double), ("Nmp", C.c_double), ("Nme", C.c_double),
+        ("kin_mode", C.c_int), ("v_in", C.c_double), ("v_out", C.c_double),
+        ("bnd_mode", C.c_int), ("bm", C.c_double * 6),
+        ("rg_mode", C.c_int), ("rg", C.c_double * 6),
+        ("c2p_maxiter", C.c_int),
+    ]
+
+
+def make_config(rc) -> OgConfig:
+    c = OgConfig()
+    c.nd, c.geom, c.nq, c.shock, c.sync = rc.nd, rc.geom, rc.nq, rc.shock, rc.sync
+    c.ngst, c.nmax, c.ncell_mv, c.ncell_f1 = rc.ngst, rc.nmax, rc.ncell_mv, rc.ncell_f1
+    c.gamma, c.cfl, c.vi = rc.gamma, rc.cfl, rc.vi
+    c.shock_thr, c.gamma_max_init = rc.shock_thr, rc.gamma_max_init
+    c.Nalpha, c.Nmp, c.Nme = rc.n_alpha, rc.n_mp, rc.n_me
+    c.kin_mode, c.v_in, c.v_out = rc.kin_mode, rc.v_in, rc.v_out
+    c.bnd_mode = rc.bnd_mode
+    for k in range(6):
+        c.bm[k] = rc.bm[k]
+        c.rg[k] = rc.rg[k]
+    c.rg_mode = rc.rg_mode
+    c.c2p_maxiter = rc.c2p_maxiter
+    return c
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            subprocess.check_call(["make", "-s", "-C", os.path.join(ROOT, "oracle")])
+        L = C.CDLL(LIB)
+        P, D, I, V = C.c_void_p, C.c_double, C.c_int, None
+        cfgp = C.POINTER(OgConfig)
+        sig = {
+            "og_create": (P, [cfgp]), "og_destroy": (V, [P]),
+            "og_load": (I, [P, P, P, P, P, P]), "og_init": (I, [P, D]),
+            "og_run": (I, [P, I]), "og_ntracks": (I, [P]), "og_nax": (I, [P]),
+            "og_track_info": (V, [P, P, P, P, P]), "og_time": (D, [P]),
+            "og_iter": (C.c_long, [P]), "og_last_dt": (D, [P]),
+            "og_get_cells": (V, [P, I, P]), "og_get_faces": (V, [P, I, P]),
+            "og_counters": (V, [P, P]), "og_regrid_counts": (V, [P, P, P]),
+            "og_prim2cons": (V, [cfgp, P, D, P]), "og_state2flux": (V, [cfgp, P, P, D, P]),
+            "og_cons2prim": (I, [cfgp, P, P, D]), "og_c2p_residual": (D, [cfgp, D, D, D, D]),
+            "og_face_flux": (V, [cfgp, I, P, P, D, D, D, P, P]),
+            "og_compute_sd": (D, [cfgp, P, P, I, I, P]),
+            "og_cell_geom": (V, [cfgp, D, D, D, D, P]), "og_face_area": (D, [cfgp, I, D, D, D]),
+        }
+        for name, (res, args) in sig.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _p(a: np.ndarray):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+# field ids (gamma_oracle.h)
+F_X, F_DX, F_Y, F_DY, F_DV, F_CEN, F_DTLOC, F_SD, F_SHOCKED, F_PSPEC = range(200, 210)
+F_GRADMV0, F_GRADF10 = 300, 400
+I_X, I_V, I_LL, I_LR, I_LS, I_DA, I_FLUX0 = 0, 1, 2, 3, 4, 5, 10
+
+
+class OracleGrid:
+    """Grid + Simu::run loop of the oracle for one setup."""
+
+    def __init__(self, setup):
+        self.setup = setup
+        self.rc = setup.cfg
+        self.cfg = make_config(self.rc)
+        self.h = lib().og_create(C.byref(self.cfg))
+        if not self.h:
+            raise MemoryError("oracle allocation failed")
+        x = np.ascontiguousarray(setup.x, dtype=np.float64)
+        dx = np.ascontiguousarray(setup.dx, dtype=np.float64)
+        y = np.ascontiguousarray(setup.y, dtype=np.float64)
+        dy = np.ascontiguousarray(setup.dy, dtype=np.float64)
+        prim = np.ascontiguousarray(setup.prim, dtype=np.float64)
+        lib().og_load(self.h, _p(x), _p(dx), _p(y), _p(dy), _p(prim))
+        lib().og_init(self.h, setup.t0)
+        self.nj = lib().og_ntracks(self.h)
+        self.nax = lib().og_nax(self.h)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().og_destroy(self.h)
+            self.h = None
+
+    def run(self, nsteps: int) -> None:
+        e = lib().og_run(self.h, nsteps)
+        if e:
+            raise RuntimeError(f"oracle run failed with code {e}")
+
+    @property
+    def t(self) -> float:
+        return lib().og_time(self.h)
+
+    @property
+    def it(self) -> int:
+        return lib().og_iter(self.h)
+
+    @property
+    def dt(self) -> float:
+        return lib().og_last_dt(self.h)
+
+    def tracks(self):
+        n = [np.zeros(self.nj, dtype=np.int32) for _ in range(4)]
+        lib().og_track_info(self.h, *[_p(a) for a in n])
+        return dict(ntrack=n[0], nact=n[1], iLbnd=n[2], iRbnd=n[3])
+
+    def cells(self, field: int) -> np.ndarray:
+        out = np.zeros((self.nj, self.nax))
+        lib().og_get_cells(self.h, field, _p(out))
+        return out
+
+    def faces(self, field: int) -> np.ndarray:
+        out = np.zeros((self.nj, self.nax - 1))
+        lib().og_get_faces(self.h, field, _p(out))
+        return out
+
+    def counters(self) -> dict:
+        out = np.zeros(8, dtype=np.int64)
+        lib().og_counters(self.h, _p(out))
+        keys = ["c2p_calls", "brent_iters", "no_bracket", "max_iter", "splits", "merges",
+                "phi_cap", "tangled"]
+        return dict(zip(keys, out.tolist()))
+
+    def regrid_counts(self):
+        s = np.zeros(self.nj, dtype=np.int64)
+        m = np.zeros(self.nj, dtype=np.int64)
+        lib().og_regrid_counts(self.h, _p(s), _p(m))
+        return s, m
+
+    def active(self, field: int):
+        """List over active tracks of the active-cell values of a cell field."""
+        tr = self.tracks()
+        a = self.cells(field)
+        rc = self.rc
+        js = range(rc.ngst, self.nj - rc.ngst) if rc.nd == 2 else range(1)
+        return [a[j, tr["iLbnd"][j] + 1: tr["iRbnd"][j]] for j in js]
+
+
+# ---- pointwise helpers ----
+def prim2cons(rc, prim, r=1.0):
+    cfg = make_config(rc)
+    p = np.ascontiguousarray(prim, dtype=np.float64)
+    out = np.zeros(rc.nq)
+    lib().og_prim2cons(C.byref(cfg), _p(p), r, _p(out))
+    return out
+
+
+def state2flux(rc, prim, cons, r=1.0):
+    cfg = make_config(rc)
+    p = np.ascontiguousarray(prim, dtype=np.float64)
+    c = np.ascontiguousarray(cons, dtype=np.float64)
+    out = np.zeros(rc.nd * rc.nq)
+    lib().og_state2flux(C.byref(cfg), _p(p), _p(c), r, _p(out))
+    return out.reshape(rc.nd, rc.nq)
+
+
+def cons2prim(rc, cons, prim_guess, r=1.0):
+    cfg = make_config(rc)
+    p = np.array(prim_guess, dtype=np.float64)
+    c = np.array(cons, dtype=np.float64)
+    it = lib().og_cons2prim(C.byref(cfg), _p(p), _p(c), r)
+    return p, c, it
+
+
+def c2p_residual(rc, p, D, S, E):
+    return lib().og_c2p_residual(C.byref(make_config(rc)), p, D, S, E)
+
+
+def face_flux(rc, dim, primL, primR, xf=1.0, v=float("nan"), dA=1.0):
+    cfg = make_config(rc)
+    a = np.ascontiguousarray(primL, dtype=np.float64)
+    b = np.ascontiguousarray(primR, dtype=np.float64)
+    lam = np.zeros(3)
+    fl = np.zeros(rc.nq)
+    lib().og_face_flux(C.byref(cfg), dim, _p(a), _p(b), xf, v, dA, _p(lam), _p(fl))
+    return lam, fl
+
+
+def compute_sd(rc, prim1, prim2, dim=0, reverse=0):
+    cfg = make_config(rc)
+    a = np.ascontiguousarray(prim1, dtype=np.float64)
+    b = np.ascontiguousarray(prim2, dtype=np.float64)
+    ps = np.zeros(1)
+    sd = lib().og_compute_sd(C.byref(cfg), _p(a), _p(b), dim, reverse, _p(ps))
+    return sd, ps[0]
+
+
+def cell_geom(rc, x, dx, y=0.0, dy=0.0):
+    out = np.zeros(5)
+    lib().og_cell_geom(C.byref(make_config(rc)), x, dx, y, dy, _p(out))
+    return out
+
+
+def face_area(rc, dim, x, y=0.0, dxf=0.0):
+    return lib().og_face_area(C.byref(make_config(rc)), dim, x, y, dxf)
