@@ -1,0 +1,147 @@
+"""Benchmark of the MI355X backend on the 2D relativistic Riemann problem.
+
+Workload (the configuration the headline metric is quoted on): 2D cartesian
+relativistic Riemann problem on x-directed Lagrangian tracks, 2048 tracks x
+2048 cells, FP64, left rho=10 p=13.33, right rho=1 p=1e-6, gamma=5/3, 1e-3
+seeded density perturbation, CFL 0.2, host-rule AMR (dx/dy split 3 / merge
+0.2), plain per-stage kernels.  A step is one full RK3 step (regrid, ghosts,
+three flux/update stages).  Metric: cell-updates/s = sum over timed steps of
+the active cells of every track, divided by wall time; inputs resident in HBM.
+
+Prints one JSON line.  Only one GPU: --gpus N>1 exits.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0      # MI355X HBM3E, GB/s
+
+# HIP-event timed kernels (ids of gammahip_kernel_times)
+KNAMES = {0: "k_update", 1: "k_faces", 2: "k_f1faces"}
+
+
+def algorithmic_bytes(kid: int, cells: float, f1faces: float, nq: int, stage_mix: float) -> float:
+    """Bytes one launch must move at minimum: every array element the kernel
+    reads or writes, once, for `cells` active cells (neighbour stencil reads
+    shared between cells are counted once).  stage_mix: fraction of launches
+    that are RK stages 2/3 (which also read the RK base)."""
+    d, i4 = 8.0, 4.0
+    if kid == 0:   # k_update: prim, x, dx, face x, v, flux in; prim, x, dx, face x out;
+        # RK base (U0, dV0, x0) written in stage 1 / read in stages 2-3; F1 face
+        # fluxes read by both cells of each F1 face; F1 slot offsets and counts
+        per = (nq + 2 + 2 + nq) * d + (nq + 3) * d + (nq + 2) * d
+        per += 2 * (f1faces / cells) * nq * d + 4 * i4
+        return cells * per
+    if kid == 1:   # k_faces: prim, centroid inputs (x, dx), face x in; v, lL, lR, flux, grad_mv out
+        return cells * ((nq + 2 + 1) * d + (3 + nq + nq) * d)
+    if kid == 2:   # k_f1faces: per cell prim, grad_mv, grad_f1 (new) + geometry + slot ints;
+        # per face the neighbour's prim, grad_mv, old grad_f1, geometry; flux out
+        return cells * ((3 * nq + 2) * d + 5 * i4) + f1faces * ((3 * nq + 2) * d + nq * d)
+    if kid == 3:   # k_gradf1: prim, grad_mv, geometry of the cell and of its neighbours; grad_f1 out
+        return cells * ((2 * nq + 2) * d + 6 * i4 + nq * d) + 2 * f1faces * (2 * nq + 2) * d
+    return 0.0
+
+
+def cpu_baseline(sample_tracks: int = 64, nx: int = 2048, steps: int = 30) -> dict:
+    """Oracle (CPU restatement, one core) on a bounded sample of the same
+    workload: the bottom `sample_tracks` tracks of the 2048 x 2048 grid (same
+    cells, same seeded perturbation), `steps` RK3 steps after 2 warm-up steps."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_ctypes as oc
+    from gamma_amd import setups
+    s = setups.riemann_2d(nx, sample_tracks, ly=sample_tracks / nx)
+    g = oc.OracleGrid(s)
+    g.run(2)
+    n0 = sum(len(a) for a in g.active(0))
+    t0 = time.perf_counter()
+    g.run(steps)
+    el = time.perf_counter() - t0
+    n1 = sum(len(a) for a in g.active(0))
+    cu = 0.5 * (n0 + n1) * steps
+    return {"value": cu / el, "unit": "cell-updates/s", "cores": 1, "kind": "port",
+            "sample": f"oracle, riemann_2d {nx} x {sample_tracks} tracks, {steps} RK3 steps "
+                      f"after 2 warm-up steps, {el:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--nx", type=int, default=2048)
+    ap.add_argument("--ny", type=int, default=2048)
+    ap.add_argument("--solver", choices=["brent", "newton"], default="brent")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--timing-steps", type=int, default=2)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; multi-GPU sharding is a "
+                                   "design (see DESIGN.md), not built", "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    from gamma_amd import hip, setups
+    s = setups.riemann_2d(args.nx, args.ny)
+    solver = hip.C2P_BRENT if args.solver == "brent" else hip.C2P_NEWTON
+    g = hip.Grid(s, solver=solver)
+    print(f"[bench] loaded riemann_2d {args.nx}x{args.ny}", file=sys.stderr, flush=True)
+    g.step(args.warmup)
+    g.sync()
+    n0 = int(g.tracks()["nact"][2:-2].sum())
+    g.sync()
+    t0 = time.perf_counter()
+    g.step(args.steps)
+    g.sync()
+    el = time.perf_counter() - t0
+    n1 = int(g.tracks()["nact"][2:-2].sum())
+    cell_updates = 0.5 * (n0 + n1) * args.steps
+    value = cell_updates / el
+    print(f"[bench] {args.steps} steps in {el:.3f} s", file=sys.stderr, flush=True)
+    st = g.stats()
+
+    # per-kernel HIP-event timing on the context's stream (extra steps, outside the timed region)
+    g.kernel_times()                      # switches event timing on and clears
+    g.step(args.timing_steps)
+    g.sync()
+    ms, nl = g.kernel_times()
+    f1faces = float(st.get("f1_faces", 0)) or 2.0 * n1
+    kern = {}
+    for kid, name in KNAMES.items():
+        if nl[kid]:
+            by = algorithmic_bytes(kid, n1, f1faces, s.cfg.nq, 2. / 3.)
+            kern[name] = {"ms": ms[kid], "launches": int(nl[kid]), "bytes": by,
+                          "GBs": by / (ms[kid] * 1e-3) / 1e9}
+    dom = max(kern, key=lambda k: kern[k]["ms"] * kern[k]["launches"])
+    ach = kern[dom]["GBs"]
+    roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
+            "kernel_ms": round(kern[dom]["ms"], 4)}
+    cpu = None if args.no_cpu_baseline else cpu_baseline()
+    out = {
+        "metric": "cell-updates/s (full RK3 step)", "value": value, "unit": "cell-updates/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": 1e3 * el / args.steps, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded SplitMix64 perturbation)",
+        "config": {"workload": f"riemann_2d {args.nx} tracks x {args.nx} cells, 2D cartesian, "
+                               f"plain per-stage kernels, cons2prim={args.solver}",
+                   "tracks": args.ny, "cells_per_track": args.nx, "parallelism": "single GPU"},
+        "roofline": roof, "cpu_baseline": cpu,
+        "kernels": kern,
+        "c2p_iters_per_call": st["c2p_iters"] / max(st["c2p_calls"], 1),
+        "failures": {k: st[k] for k in ("c2p_nobracket", "c2p_maxiter", "negative_dt", "nan_flux",
+                                        "f1_overflow")},
+        "regrid": {"splits": st["splits"], "merges": st["merges"]},
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

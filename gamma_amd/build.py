@@ -1,0 +1,66 @@
+"""Build recipes for the native pieces (run in-tree; outputs are .so files
+next to their sources, git-ignored but shipped with the tree).
+
+* ``gamma_amd/libgammahip.so`` -- the product: HIP for gfx950.
+* ``tests/libgammahip_emu.so`` -- the same kernel bodies compiled for the CPU
+  (GH_EMU); a test tool that checks indexing and arithmetic against the oracle
+  without a GPU.  Never loaded by the package.
+* ``oracle/libgamma_oracle.so`` -- the CPU oracle (test infrastructure).
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+SRC = os.path.join(HERE, "csrc", "gammahip.cpp")
+DEPS = [SRC, os.path.join(HERE, "csrc", "hydro.hpp"), os.path.join(ROOT, "include", "gammahip.h")]
+
+HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+# -ffp-contract=off: no fused multiply-add, every expression rounds as written
+# (the oracle is built the same way); packed FP32 ops disabled as required on gfx950.
+HIP_FLAGS = ["-O3", "-std=c++17", "--offload-arch=gfx950", "-ffp-contract=off",
+             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+             "-fPIC", "-shared", "-x", "hip"]
+EMU_FLAGS = ["-O2", "-std=c++17", "-DGH_EMU", "-x", "c++", "-ffp-contract=off", "-fPIC",
+             "-shared"]
+
+
+def _run(cmd):
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("build failed: " + " ".join(cmd) + "\n" + r.stdout + r.stderr)
+
+
+def build_hip(out: str | None = None) -> str:
+    out = out or os.path.join(HERE, "libgammahip.so")
+    tmp = out + ".tmp"
+    _run([HIPCC, *HIP_FLAGS, SRC, "-o", tmp])
+    os.replace(tmp, out)
+    return out
+
+
+def build_emu(out: str | None = None) -> str:
+    out = out or os.path.join(ROOT, "tests", "libgammahip_emu.so")
+    tmp = out + ".tmp"
+    _run(["g++", *EMU_FLAGS, SRC, "-o", tmp])
+    os.replace(tmp, out)
+    return out
+
+
+def build_oracle() -> str:
+    d = os.path.join(ROOT, "oracle")
+    _run(["make", "-s", "-B", "-C", d])
+    return os.path.join(d, "libgamma_oracle.so")
+
+
+def build_all() -> None:
+    build_hip()
+    build_emu()
+    build_oracle()
+
+
+if __name__ == "__main__":
+    build_all()

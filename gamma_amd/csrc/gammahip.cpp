@@ -293,20 +293,14 @@ GH_HD void body_nb(const Dev& d, int j, int i) {
   d.nbuj[k] = j + 1; d.nbulo[k] = lo; d.nbucnt[k] = cnt;
 }
 
-// Grid::computeTransGradients (2d.cpp:366-430)
+// Grid::computeTransGradients (2d.cpp:366-430) of cell (j,i) into gf
 template <int NQ>
-GH_HD void body_gradf1(const Dev& d, int j, int i) {
-  if (j > d.nj - 2) return;
-  int nt = d.ntrack[j];
-  if (i >= nt) return;
+GH_HD void trans_grad(const Dev& d, int j, long long k, const double* p0, const double* m0,
+                      const Geo& g0, double* gf) {
   int geom = d.tgeom[j];
-  long long k = CI(d, j, i);
   double x0 = d.cx[k];
   double xL0 = x0 - d.cdx[k] / 2., xR0 = x0 + d.cdx[k] / 2.;
-  Geo g0 = geo_of<2>(d, j, k);
-  double p0[NQ], m0[NQ], gavg[NQ], gf[NQ];
-  load_v<2, NQ>(d, d.prim, k, p0);
-  load_v<2, NQ>(d, d.gmv, k, m0);
+  double gavg[NQ];
   for (int q = 0; q < NQ; ++q) gavg[q] = 0.;
   double dAtot = 0;
   for (int pass = 0; pass < 2; ++pass) {
@@ -341,10 +335,14 @@ GH_HD void body_gradf1(const Dev& d, int j, int i) {
     if (pass == 0)
       for (int q = 0; q < NQ; ++q) { gavg[q] /= dAtot; gf[q] = gavg[q]; }
   }
-  store_v<2, NQ>(d, d.gf1n, k, gf);
 }
 
-// F1 faces of one cell with its upper neighbours (2d.cpp:901-950)
+// One cell's pass of the F1 loop of Grid::computeFluxes (2d.cpp:901-950):
+// its transverse gradient (computeTransGradients, stored as the new value),
+// then its faces with the upper neighbours.  The upper track's gradients are
+// the previous stage's (2d.cpp:901 loops j upward, so track j+1 has not been
+// recomputed yet); stale lists of the top ghost track pointing into their own
+// track read the previous values too (their faces only feed ghost cells).
 template <int NQ>
 GH_HD void body_f1faces(const Dev& d, int j, int i) {
   if (j > d.nj - 2) return;
@@ -353,17 +351,18 @@ GH_HD void body_f1faces(const Dev& d, int j, int i) {
   int geom = d.tgeom[j];
   double G = d.P.gamma;
   long long k = CI(d, j, i);
+  Geo g0 = geo_of<2>(d, j, k);
+  double pc[NQ], mc[NQ], fc[NQ];
+  load_v<2, NQ>(d, d.prim, k, pc);
+  load_v<2, NQ>(d, d.gmv, k, mc);
+  trans_grad<NQ>(d, j, k, pc, mc, g0, fc);
+  store_v<2, NQ>(d, d.gf1n, k, fc);
   int tj = d.nbuj[k], lo = d.nbulo[k], cnt = d.nbucnt[k], off = d.f1off[k];
   if (cnt <= 0) return;
   double x0 = d.cx[k];
   double xL0 = x0 - d.cdx[k] / 2., xR0 = x0 + d.cdx[k] / 2.;
   double jpos = d.ty[j] + d.tdy[j] / 2.;
-  Geo g0 = geo_of<2>(d, j, k);
   bool pc0 = (j == 0 || j == d.nj - 2 || d.nbdcnt[k] == 0);
-  double pc[NQ], mc[NQ], fc[NQ];
-  load_v<2, NQ>(d, d.prim, k, pc);
-  load_v<2, NQ>(d, d.gmv, k, mc);
-  load_v<2, NQ>(d, d.gf1n, k, fc);
   for (int n = 0; n < cnt; ++n) {
     long long slot = off + n;
     if (slot >= d.fcap) { GH_ATOMIC_ADD(&d.cnt[CNT_F1OVF], 1ull); break; }
@@ -385,7 +384,7 @@ GH_HD void body_f1faces(const Dev& d, int j, int i) {
       load_v<2, NQ>(d, d.prim, kn, pn);
       load_v<2, NQ>(d, d.gmv, kn, mn);
       // the upper track has not recomputed its transverse gradients yet (2d.cpp:901 loops j upward)
-      load_v<2, NQ>(d, tj > j ? d.gf1 : d.gf1n, kn, fn);
+      load_v<2, NQ>(d, d.gf1, kn, fn);
       for (int q = 0; q < NQ; ++q) {
         double qL = pc[q] + mc[q] * dxmL;
         double qR = pn[q] + mn[q] * dxmR;
@@ -426,13 +425,17 @@ GH_HD void for_incoming_f1(const Dev& d, int J, int i, F&& fn) {
       if (slot < d.fcap) fn((long long)jp * d.fcap + slot);
     }
   };
-  for (int jp = 0; jp < J; ++jp) {
-    if (jp > d.nj - 2) break;
+  // ghost tracks below the domain carry stale lists copied from track jL+1,
+  // which point two tracks up: they are the only sources besides track J-1
+  int jstale = J - 2 < d.jL ? J - 2 : d.jL;
+  for (int jp = 0; jp <= jstale; ++jp) {
+    stale(jp, 0);
+    stale(jp, d.ntrack[jp] - 1);
+  }
+  {
+    int jp = J - 1;
+    if (jp < 0 || jp > d.nj - 2) return;
     int ntp = d.ntrack[jp];
-    if (jp != J - 1) {
-      if (jp <= d.jL) { stale(jp, 0); stale(jp, ntp - 1); }
-      continue;
-    }
     stale(jp, 0);
     // first i' in [1, ntp-2] with lo+cnt > i
     int a = 1, b = ntp - 1;
@@ -811,7 +814,6 @@ GH_KERNEL2(k_rg_shockpos, body_rg_shockpos)
 GH_KERNEL2(k_advance_time, body_advance_time)
 template <int ND, int NQ> GH_KERNEL2(k_ghost_mv, (body_ghost_mv<ND, NQ>))
 template <int ND, int NQ> GH_KERNEL2(k_faces, (body_faces<ND, NQ>))
-template <int NQ> GH_KERNEL2(k_gradf1, body_gradf1<NQ>)
 template <int NQ> GH_KERNEL2(k_f1faces, body_f1faces<NQ>)
 template <int ND, int NQ> GH_KERNEL2S(k_update, (body_update<ND, NQ>))
 template <int ND, int NQ> GH_KERNEL2(k_inject, (body_inject<ND, NQ>))
@@ -1037,9 +1039,6 @@ struct Pipe {
 #else
       hipLaunchKernelGGL(k_f1_scan, dim3(d.nj), dim3(KB), 0, c->stream, d);
 #endif
-      tbeg(c, 3);
-      L2(c, KSEL(body_gradf1<NQ>, k_gradf1<NQ>), d.nj, d.nax);
-      tend(c, 3);
       tbeg(c, 2);
       L2(c, KSEL(body_f1faces<NQ>, k_f1faces<NQ>), d.nj, d.nax);
       tend(c, 2);

@@ -107,14 +107,17 @@ GH_HD double c2p_f(double p, double D, double S, double E, double G, int geom) {
   return (E + p - D * lf - (G * p * lf * lf) / (G - 1.));
 }
 
-// d f / d p for the Newton solver (analytic derivative of c2p_f)
-GH_HD double c2p_df(double p, double D, double S, double E, double G, int geom, double* fval) {
+// d f / d p for the Newton solver (analytic derivative of c2p_f); *fscale is
+// the sum of the magnitudes of the terms of f, i.e. the scale of its rounding error
+GH_HD double c2p_df(double p, double D, double S, double E, double G, int geom, double* fval,
+                    double* fscale) {
   double Ep = E + p;
   double w = 1. - (S * S) / (Ep * Ep);
   double aw = geom == SPH ? fabs(w) : w;
   double lf = 1. / sqrt(aw);
   double gg = G / (G - 1.);
   *fval = (E + p - D * lf - (G * p * lf * lf) / (G - 1.));
+  *fscale = fabs(E) + fabs(p) + fabs(D * lf) + fabs((G * p * lf * lf) / (G - 1.));
   // d lf / dp = -lf^3 S^2 / (E+p)^3 (sign of w folded in for |w|)
   double dw = 2. * (S * S) / (Ep * Ep * Ep);
   double dlf = -0.5 * lf * lf * lf * (geom == SPH && w < 0 ? -dw : dw);
@@ -185,23 +188,25 @@ GH_HD double brent(double D, double S, double E, double G, int geom, double lo, 
 }
 
 // Safeguarded Newton on the same bracket: Newton steps from the guess, a
-// bisection step whenever Newton leaves the bracket; converged when the step
-// is below 1e-14 relative (one quadratic step past the reference's 1e-13).
+// bisection step whenever Newton leaves the bracket.  Converged when |f| is at
+// the rounding level of its own terms (x is a root to working precision), or
+// after a step below 1e-15 relative (quadratic convergence puts the new
+// iterate well inside the reference's 1e-13 tolerance).
 GH_HD double newton(double D, double S, double E, double G, int geom, double lo, double hi,
                     double x0, int maxit, C2PStat* st) {
-  double x = (x0 > lo && x0 < hi) ? x0 : 0.5 * (lo + hi);
+  double x = (x0 >= lo && x0 <= hi) ? x0 : 0.5 * (lo + hi);
   int it = 0;
   for (;;) {
     it++;
-    double f;
-    double df = c2p_df(x, D, S, E, G, geom, &f);
-    if (f == 0.) break;
+    double f, fs;
+    double df = c2p_df(x, D, S, E, G, geom, &f, &fs);
+    if (fabs(f) <= 4. * 2.2204460492503131e-16 * fs) break;
     if (f > 0) lo = x; else hi = x;      // f decreases through the root
     double xn = x - f / df;
     if (!(xn > lo && xn < hi)) xn = 0.5 * (lo + hi);
     double dx = fabs(xn - x);
     x = xn;
-    if (dx <= 1.e-14 * fabs(x) || hi - lo <= 1.e-14 * fabs(lo)) break;
+    if (dx <= 1.e-15 * fabs(x) || hi - lo <= 1.e-15 * fabs(lo)) break;
     if (it >= maxit) { st->fail |= 2; break; }
   }
   st->iters += it;

@@ -190,15 +190,17 @@ def blandford_mckee_1d(n: int = 4000, lfac_shock=100., eta=1.e-10, gamma=4. / 3.
 
 def riemann_2d(nx: int = 2048, ny: int = 2048, rho_l=10., p_l=13.33, rho_r=1.,
                p_r=1.e-6, gamma=5. / 3., amp=1.e-3, seed=DEFAULT_SEED,
-               nmax=None, regrid=True) -> Setup:
+               nmax=None, regrid=True, ly: float = 1.) -> Setup:
     """Config: 2D cartesian relativistic Riemann problem on x-directed tracks
-    over [0,1]^2, diaphragm at x = 0.5, density seeded with a 1e-3 SplitMix64
-    perturbation per cell; static boundary faces and the aspect-ratio AMR rule
-    of 2DST.cpp:261-322 (split above 3, merge below 0.2)."""
+    over [0,1] x [0,ly], diaphragm at x = 0.5, density seeded with a 1e-3
+    SplitMix64 perturbation per cell (counter j*nx + i); static boundary faces
+    and the aspect-ratio AMR rule of 2DST.cpp:261-322 (split above 3, merge
+    below 0.2).  With ly = ny/2048 and nx = 2048 the grid is the bottom strip
+    of the 2048 x 2048 configuration, cell for cell."""
     cfg = RunConfig(nd=2, geom=GEOM_CART, nmax=nmax or (nx * 5) // 4, ncell_mv=nx,
                     ncell_f1=ny, gamma=gamma, kin_mode=KIN_FIXED, v_in=0., v_out=0.,
                     rg_mode=RG_AR2D if regrid else RG_NONE, rg=(3., 0.2, 0., 0., 0., 0.))
-    dxc, dyc = 1. / nx, 1. / ny
+    dxc, dyc = 1. / nx, ly / ny
     x1 = (np.arange(nx) + 0.5) * dxc
     y = (np.arange(ny) + 0.5) * dyc
     x = np.broadcast_to(x1, (ny, nx)).copy()

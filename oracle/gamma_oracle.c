@@ -159,6 +159,7 @@ static double c2p_f(const og_config *c, double p, double D, double S, double E) 
  * test with epsabs 0, epsrel 1e-13, or after maxit iterations. */
 static double brent_root(const og_config *c, double D, double S, double E,
                          double lo, double hi, int maxit, int *iters, int *hitmax) {
+  double tolr = c->c2p_tol > 0 ? c->c2p_tol : 1.e-13;
   double a = lo, b = hi, fa = c2p_f(c, lo, D, S, E), fb = c2p_f(c, hi, D, S, E);
   double cc = b, fc = fb, d = b - a, e = b - a;
   double root = 0.5 * (lo + hi), xl = lo, xu = hi;
@@ -211,7 +212,7 @@ static double brent_root(const og_config *c, double D, double S, double E,
     double mn = 0.;
     if ((xl > 0.0 && xu > 0.0) || (xl < 0.0 && xu < 0.0))
       mn = fabs(xl) < fabs(xu) ? fabs(xl) : fabs(xu);
-    int conv = fabs(xu - xl) < 1.e-13 * mn;
+    int conv = fabs(xu - xl) < tolr * mn;
     if (conv) break;
     if (it >= maxit) { *hitmax = 1; break; }
   }

@@ -51,6 +51,9 @@ typedef struct {
   int    bnd_mode;  double bm[6];              /* userBoundaries: BM params */
   int    rg_mode;   double rg[6];              /* checkCellForRegrid params */
   int    c2p_maxiter;                          /* 1000 in the reference */
+  double c2p_tol;     /* Brent interval tolerance, 1e-13 in the reference (rel_sph.cpp:281);
+                         other values only measure how much the reference's own
+                         tolerance moves its trajectories */
 } og_config;
 
 typedef struct og_grid og_grid;
