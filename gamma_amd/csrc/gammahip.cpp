@@ -58,6 +58,7 @@ struct Dev {
   // cells
   double *prim, *cx, *cdx, *dtloc, *U0, *dV0, *gmv, *gf1, *gf1n, *sd, *pspec;
   int *shk, *nbdj, *nbdlo, *nbdcnt, *nbuj, *nbulo, *nbucnt, *f1off, *f1tot;
+  int *incf;               // first fresh lower-track cell whose F1 list holds this cell
   // faces
   double *fx, *fxn, *fx0, *fv, *flL, *flR, *fflux;
   double *ff1, *ff1sd;
@@ -173,6 +174,10 @@ GH_HD void body_faces(const Dev& d, int j, int i) {
   int geom = d.tgeom[j];
   double G = d.P.gamma;
   long long k = CI(d, j, i);
+  if (ND == 2) {               // reset before the neighbour pass fills it
+    d.incf[k] = INT_MAX;
+    if (i == nt - 2) d.incf[k + 1] = INT_MAX;
+  }
   double xf = d.fx[k];
   double pL[NQ], cL[NQ], fL[NQ], pR[NQ], cR[NQ], fR[NQ];
   if (i == 0 || i == nt - 2) {
@@ -291,6 +296,7 @@ GH_HD void body_nb(const Dev& d, int j, int i) {
   lo = 0; cnt = 0;
   if (j != d.nj - 1) nb_side(d, j, i, j + 1, &lo, &cnt);
   d.nbuj[k] = j + 1; d.nbulo[k] = lo; d.nbucnt[k] = cnt;
+  for (int t = lo; t < lo + cnt; ++t) GH_ATOMIC_MIN(&d.incf[CI(d, j + 1, t)], i);
 }
 
 // Grid::computeTransGradients (2d.cpp:366-430) of cell (j,i) into gf
@@ -437,13 +443,9 @@ GH_HD void for_incoming_f1(const Dev& d, int J, int i, F&& fn) {
     if (jp < 0 || jp > d.nj - 2) return;
     int ntp = d.ntrack[jp];
     stale(jp, 0);
-    // first i' in [1, ntp-2] with lo+cnt > i
-    int a = 1, b = ntp - 1;
-    while (a < b) {
-      int m = (a + b) >> 1;
-      long long km = CI(d, jp, m);
-      if (d.nbulo[km] + d.nbucnt[km] > i) b = m; else a = m + 1;
-    }
+    // fresh lists of track J-1 are non-decreasing ranges: the sources of
+    // (J,i) are consecutive cells starting at the first one (filled by k_nb)
+    int a = d.incf[CI(d, J, i)];
     for (int ip = a; ip <= ntp - 2; ++ip) {
       long long kp = CI(d, jp, ip);
       int l = d.nbulo[kp], c = d.nbucnt[kp];
@@ -1203,7 +1205,7 @@ int gammahip_create(const gammahip_config* cfg, gammahip_ctx** out) {
     e |= dalloc(c, &d.gmv, nqs * cs); e |= dalloc(c, &d.gf1, nqs * cs); e |= dalloc(c, &d.gf1n, nqs * cs);
     e |= dalloc(c, &d.nbdj, cs); e |= dalloc(c, &d.nbdlo, cs); e |= dalloc(c, &d.nbdcnt, cs);
     e |= dalloc(c, &d.nbuj, cs); e |= dalloc(c, &d.nbulo, cs); e |= dalloc(c, &d.nbucnt, cs);
-    e |= dalloc(c, &d.f1off, cs); e |= dalloc(c, &d.f1tot, nj);
+    e |= dalloc(c, &d.f1off, cs); e |= dalloc(c, &d.f1tot, nj); e |= dalloc(c, &d.incf, cs);
     e |= dalloc(c, &d.ff1, nqs * (size_t)d.fs1);
     if (d.shock) e |= dalloc(c, &d.ff1sd, 4 * (size_t)d.fs1);
   }

@@ -1466,3 +1466,19 @@ double og_face_area(const og_config *c, int dim, double x, double y, double dxf)
   face_dA(c, &f);
   return f.dA;
 }
+
+double og_sync_source(const og_config *c, const double *prim, double dV, double dt) {
+  Cell cl; cell_ctor(&cl);
+  memcpy(cl.S.prim, prim, sizeof(double) * c->nq);
+  cl.G.dV = dV;
+  double before = cl.S.cons[OG_GMX];
+  cell_radiative(c, &cl, dt);
+  return cl.S.cons[OG_GMX] - before;
+}
+void og_inject(const og_config *c, double *prim, int shocked, double pspec) {
+  Cell cl; cell_ctor(&cl);
+  memcpy(cl.S.prim, prim, sizeof(double) * c->nq);
+  cl.isShocked = shocked; cl.pspec = pspec;
+  cell_inject(c, &cl);
+  memcpy(prim, cl.S.prim, sizeof(double) * c->nq);
+}

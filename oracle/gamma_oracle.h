@@ -110,6 +110,10 @@ double og_compute_sd(const og_config *c, const double *prim1,
 void og_cell_geom(const og_config *c, double x, double dx, double y, double dy,
                   double *out);
 double og_face_area(const og_config *c, int dim, double x, double y, double dxf);
+/* radiativeSourceTerms increment of cons[GMX] and cons[GMN] for one cell */
+double og_sync_source(const og_config *c, const double *prim, double dV, double dt);
+/* radiation_injectParticles on one cell (prim in/out) */
+void og_inject(const og_config *c, double *prim, int shocked, double pspec);
 
 #ifdef __cplusplus
 }

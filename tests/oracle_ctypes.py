@@ -74,6 +74,7 @@ def lib():
             "og_face_flux": (V, [cfgp, I, P, P, D, D, D, P, P]),
             "og_compute_sd": (D, [cfgp, P, P, I, I, P]),
             "og_cell_geom": (V, [cfgp, D, D, D, D, P]), "og_face_area": (D, [cfgp, I, D, D, D]),
+            "og_sync_source": (D, [cfgp, P, D, D]), "og_inject": (V, [cfgp, P, I, D]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -229,3 +230,14 @@ def cell_geom(rc, x, dx, y=0.0, dy=0.0):
 
 def face_area(rc, dim, x, y=0.0, dxf=0.0):
     return lib().og_face_area(C.byref(make_config(rc)), dim, x, y, dxf)
+
+
+def sync_source(rc, prim, dV, dt):
+    p = np.ascontiguousarray(prim, dtype=np.float64)
+    return lib().og_sync_source(C.byref(make_config(rc)), _p(p), dV, dt)
+
+
+def inject(rc, prim, shocked, pspec):
+    p = np.array(prim, dtype=np.float64)
+    lib().og_inject(C.byref(make_config(rc)), _p(p), int(shocked), pspec)
+    return p
