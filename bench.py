@@ -72,6 +72,44 @@ def cpu_baseline(sample_tracks: int = 64, nx: int = 2048, steps: int = 30) -> di
                       f"after 2 warm-up steps, {el:.1f} s"}
 
 
+def run_form(setup, solver, options, steps, warmup, timing_steps):
+    """Time `steps` RK3 steps after `warmup` on a fresh grid; then per-kernel
+    HIP-event times over `timing_steps` extra steps."""
+    from gamma_amd import hip
+    g = hip.Grid(setup, solver=solver)
+    if options:
+        g.set_options(**options)
+    g.step(warmup)
+    g.sync()
+    n0 = int(g.tracks()["nact"][2:-2].sum())
+    g.sync()
+    t0 = time.perf_counter()
+    g.step(steps)
+    g.sync()
+    el = time.perf_counter() - t0
+    n1 = int(g.tracks()["nact"][2:-2].sum())
+    st = g.stats()
+    g.kernel_times()                      # switches event timing on and clears
+    g.step(timing_steps)
+    g.sync()
+    ms, nl = g.kernel_times()
+    g.close()
+    return {"value": 0.5 * (n0 + n1) * steps / el, "ms_per_step": 1e3 * el / steps,
+            "stats": st, "ms": ms, "launches": nl, "cells": n1}
+
+
+def kernel_table(res, nq):
+    f1faces = 2.0 * res["cells"]
+    kern = {}
+    for kid, name in KNAMES.items():
+        if res["launches"][kid]:
+            by = algorithmic_bytes(kid, res["cells"], f1faces, nq, 2. / 3.)
+            ms = res["ms"][kid]
+            kern[name] = {"ms": ms, "launches": int(res["launches"][kid]), "bytes": by,
+                          "GBs": by / (ms * 1e-3) / 1e9}
+    return kern
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -79,8 +117,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--nx", type=int, default=2048)
     ap.add_argument("--ny", type=int, default=2048)
-    ap.add_argument("--solver", choices=["brent", "newton"], default="brent")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--timing-steps", type=int, default=2)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -88,50 +126,44 @@ def main():
                                    "design (see DESIGN.md), not built", "n_gpus": args.gpus}))
         sys.exit(2)
 
-    from gamma_amd import hip, setups
+    from gamma_amd import autotune, hip, setups
     s = setups.riemann_2d(args.nx, args.ny)
-    solver = hip.C2P_BRENT if args.solver == "brent" else hip.C2P_NEWTON
-    g = hip.Grid(s, solver=solver)
-    print(f"[bench] loaded riemann_2d {args.nx}x{args.ny}", file=sys.stderr, flush=True)
-    g.step(args.warmup)
-    g.sync()
-    n0 = int(g.tracks()["nact"][2:-2].sum())
-    g.sync()
-    t0 = time.perf_counter()
-    g.step(args.steps)
-    g.sync()
-    el = time.perf_counter() - t0
-    n1 = int(g.tracks()["nact"][2:-2].sum())
-    cell_updates = 0.5 * (n0 + n1) * args.steps
-    value = cell_updates / el
-    print(f"[bench] {args.steps} steps in {el:.3f} s", file=sys.stderr, flush=True)
-    st = g.stats()
-
-    # per-kernel HIP-event timing on the context's stream (extra steps, outside the timed region)
-    g.kernel_times()                      # switches event timing on and clears
-    g.step(args.timing_steps)
-    g.sync()
-    ms, nl = g.kernel_times()
-    f1faces = float(st.get("f1_faces", 0)) or 2.0 * n1
-    kern = {}
-    for kid, name in KNAMES.items():
-        if nl[kid]:
-            by = algorithmic_bytes(kid, n1, f1faces, s.cfg.nq, 2. / 3.)
-            kern[name] = {"ms": ms[kid], "launches": int(nl[kid]), "bytes": by,
-                          "GBs": by / (ms[kid] * 1e-3) / 1e9}
+    nq = s.cfg.nq
+    print(f"[bench] riemann_2d {args.nx}x{args.ny}: plain form", file=sys.stderr, flush=True)
+    plain = run_form(s, hip.C2P_BRENT, None, args.steps, args.warmup, args.timing_steps)
+    print(f"[bench] plain {plain['ms_per_step']:.3f} ms/step", file=sys.stderr, flush=True)
+    kern = kernel_table(plain, nq)
     dom = max(kern, key=lambda k: kern[k]["ms"] * kern[k]["launches"])
     ach = kern[dom]["GBs"]
     roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
             "kernel_ms": round(kern[dom]["ms"], 4)}
+
+    batched = None
+    if not args.no_batched:
+        print("[bench] autotuning the batched form", file=sys.stderr, flush=True)
+        scratch = hip.Grid(s, solver=hip.C2P_NEWTON)
+        scratch.step(args.warmup)
+        tuned = autotune.tune(scratch)
+        scratch.close()
+        opts = {k: tuned["best"][k] for k in ("block", "remap", "graph")}
+        b = run_form(s, hip.C2P_NEWTON, opts, args.steps, args.warmup, args.timing_steps)
+        print(f"[bench] batched {b['ms_per_step']:.3f} ms/step with {opts}", file=sys.stderr, flush=True)
+        batched = {"value": b["value"], "ms_per_step": b["ms_per_step"], "options": opts,
+                   "cons2prim": "safeguarded Newton, previous-pressure guess",
+                   "speedup_vs_plain": b["value"] / plain["value"],
+                   "c2p_iters_per_call": b["stats"]["c2p_iters"] / max(b["stats"]["c2p_calls"], 1),
+                   "kernels": kernel_table(b, nq), "autotune": tuned["table"]}
+
     cpu = None if args.no_cpu_baseline else cpu_baseline()
+    st = plain["stats"]
     out = {
-        "metric": "cell-updates/s (full RK3 step)", "value": value, "unit": "cell-updates/s",
+        "metric": "cell-updates/s (full RK3 step)", "value": plain["value"], "unit": "cell-updates/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": 1e3 * el / args.steps, "higher_is_better": True, "scaling": "weak",
+        "ms_per_step": plain["ms_per_step"], "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded SplitMix64 perturbation)",
-        "config": {"workload": f"riemann_2d {args.nx} tracks x {args.nx} cells, 2D cartesian, "
-                               f"plain per-stage kernels, cons2prim={args.solver}",
+        "config": {"workload": f"riemann_2d {args.ny} tracks x {args.nx} cells, 2D cartesian, "
+                               f"plain per-stage kernels, Brent cons2prim",
                    "tracks": args.ny, "cells_per_track": args.nx, "parallelism": "single GPU"},
         "roofline": roof, "cpu_baseline": cpu,
         "kernels": kern,
@@ -139,6 +171,7 @@ def main():
         "failures": {k: st[k] for k in ("c2p_nobracket", "c2p_maxiter", "negative_dt", "nan_flux",
                                         "f1_overflow")},
         "regrid": {"splits": st["splits"], "merges": st["merges"]},
+        "batched": batched,
     }
     print(json.dumps(out))
 

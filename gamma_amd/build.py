@@ -6,6 +6,8 @@ next to their sources, git-ignored but shipped with the tree).
   (GH_EMU); a test tool that checks indexing and arithmetic against the oracle
   without a GPU.  Never loaded by the package.
 * ``oracle/libgamma_oracle.so`` -- the CPU oracle (test infrastructure).
+* ``gamma_amd/copy_probe`` -- a plain copy kernel of known size, the
+  calibration point for the HBM traffic counters.
 """
 from __future__ import annotations
 
@@ -16,7 +18,8 @@ import subprocess
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 SRC = os.path.join(HERE, "csrc", "gammahip.cpp")
-DEPS = [SRC, os.path.join(HERE, "csrc", "hydro.hpp"), os.path.join(ROOT, "include", "gammahip.h")]
+DEPS = [SRC, os.path.join(HERE, "csrc", "hydro.hpp"), os.path.join(HERE, "csrc", "mlp.hpp"),
+        os.path.join(ROOT, "include", "gammahip.h")]
 
 HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
 # -ffp-contract=off: no fused multiply-add, every expression rounds as written
@@ -56,10 +59,18 @@ def build_oracle() -> str:
     return os.path.join(d, "libgamma_oracle.so")
 
 
+def build_copy_probe(out: str | None = None) -> str:
+    out = out or os.path.join(HERE, "copy_probe")
+    flags = [f for f in HIP_FLAGS if f not in ("-fPIC", "-shared")]
+    _run([HIPCC, *flags, os.path.join(HERE, "csrc", "copy_probe.cpp"), "-o", out])
+    return out
+
+
 def build_all() -> None:
     build_hip()
     build_emu()
     build_oracle()
+    build_copy_probe()
 
 
 if __name__ == "__main__":

@@ -12,6 +12,7 @@
 // threadIdx.x -> cell.  Built with -DGH_EMU the same bodies run in plain loops
 // on the CPU (test tool for index checks); the product library is the HIP build.
 #include "hydro.hpp"
+#include "mlp.hpp"
 #include "../../include/gammahip.h"
 
 #include <stdio.h>
@@ -253,10 +254,30 @@ GH_HD void body_dt(const Dev& d, int j, int i) {
 // ---------------------------------------------------------------------------
 // 2D transverse (F1) coupling (2d.cpp:366-430, 491-527, 534-606, 890-950)
 
-// first face index k in [0, nt-2] of track ja with x >= xr, nt-1 if none
-GH_HD int lb_face(const Dev& d, int ja, double xr) {
-  int lo = 0, hi = d.ntrack[ja] - 1;
+// first face index k in [0, nt-2] of track ja with x >= xr, nt-1 if none.
+// Galloping search from the hint h (adjacent tracks have nearly the same
+// structure, so the answer is usually within a few faces of the cell's own
+// index), then bisection inside the bracket found: the same lower bound as a
+// full binary search on a sorted track.
+GH_HD int lb_face(const Dev& d, int ja, double xr, int h) {
+  int n = d.ntrack[ja] - 1;                  // searchable faces [0, n)
   const double* X = d.fx + (long long)ja * d.nax;
+  if (h < 0) h = 0;
+  if (h > n) h = n;
+  int lo, hi;
+  if (h < n && X[h] < xr) {                  // answer right of h
+    lo = h + 1;
+    int step = 1;
+    hi = lo;
+    while (hi < n && X[hi] < xr) { lo = hi + 1; hi += step; step <<= 1; }
+    if (hi > n) hi = n;
+  } else {                                   // answer at or left of h
+    hi = h;
+    int step = 1;
+    lo = h - 1;
+    while (lo >= 0 && X[lo] >= xr) { hi = lo; lo -= step; step <<= 1; }
+    lo = lo < 0 ? 0 : lo + 1;
+  }
   while (lo < hi) {
     int mid = (lo + hi) >> 1;
     if (X[mid] >= xr) hi = mid; else lo = mid + 1;
@@ -275,12 +296,12 @@ GH_HD void nb_side(const Dev& d, int j, int i, int ja, int* lo, int* cnt) {
   int fin = 0;
   if (i > 1) {
     double xpR = d.cx[k - 1] + d.cdx[k - 1] / 2.;
-    int l = lb_face(d, ja, xpR);
+    int l = lb_face(d, ja, xpR, i - 1);
     fin = l < nta - 2 ? l : nta - 2;
   }
   int s = fin;
   if (d.fx[CI(d, ja, s)] > xjL && s > 0) s--;
-  int L = lb_face(d, ja, xjR);
+  int L = lb_face(d, ja, xjR, i);
   if (L < s) L = s;
   *lo = s + 1;
   *cnt = L - s;
@@ -503,6 +524,18 @@ GH_HD void body_update(const Dev& d, int j, int i, int stage) {
     double xa = xn;
     if (stage > 1 && i >= iL && i <= iR - 1) xa = w0 * d.fx0[k] + w1 * xn;
     d.fxn[k] = xa;
+  }
+  if (stage > 1 && (i == iL || i == iR)) {
+    // CellGeomFromInterfacePos covers cells 1..ntrack-2 (1d.cpp:834), so the
+    // ghost cells next to the active range follow their faces too (split at
+    // the first / last active cell reads them)
+    double xoL = d.fx[k - 1] + dt * d.fv[k - 1];
+    double xoR = d.fx[k] + dt * d.fv[k];
+    if (i - 1 >= iL && i - 1 <= iR - 1) xoL = w0 * d.fx0[k - 1] + w1 * xoL;
+    if (i >= iL && i <= iR - 1) xoR = w0 * d.fx0[k] + w1 * xoR;
+    d.cx[k] = (xoR + xoL) / 2.;
+    d.cdx[k] = (xoR - xoL);
+    return;
   }
   if (i < iL + 1 || i > iR - 1) return;
   double xnL = d.fx[k - 1] + dt * d.fv[k - 1];
@@ -792,17 +825,36 @@ GH_HD void body_advance_time(const Dev& d, int j, int i) {
 // launch layer
 #ifndef GH_EMU
 #define KB 256
+// Block -> (track, cell-chunk) mapping.  remap == 0: blockIdx.y is the track,
+// blockIdx.x the chunk.  remap == 1 (XCD-aware): workgroups are dispatched
+// round-robin over the 8 XCDs by linear id, so linear id L runs on XCD L % 8;
+// L is renumbered so that each XCD walks a contiguous band of tiles (whole
+// tracks in order), keeping adjacent tracks -- the F1 neighbours -- in one L2.
+__device__ __forceinline__ void tile_of(int remap, int& j, int& i) {
+  int gx = gridDim.x, gy = gridDim.y;
+  int L = blockIdx.x + gx * blockIdx.y;
+  if (remap) {
+    // XCD x runs blocks x, x+8, x+16, ...: n_x = N/8 + (x < N%8) of them.  Give
+    // it the logical range [x*(N/8) + min(x, N%8), +n_x): a bijection on [0, N).
+    int N = gx * gy;
+    int x = L % 8, r = L / 8;
+    int q = N / 8, m = N % 8;
+    L = x * q + (x < m ? x : m) + r;
+  }
+  j = L / gx;
+  i = (L % gx) * blockDim.x + threadIdx.x;
+}
 #define GH_KERNEL2(NAME, BODY)                                                  \
-  __global__ void __launch_bounds__(KB) NAME(Dev d, int ni) {                     \
-    int i = blockIdx.x * blockDim.x + threadIdx.x;                               \
-    int j = blockIdx.y;                                                          \
-    if (i < ni) BODY(d, j, i);                                                   \
+  __global__ void __launch_bounds__(KB) NAME(Dev d, int ni, int remap) {        \
+    int i, j;                                                                    \
+    tile_of(remap, j, i);                                                        \
+    if (i < ni && j < d.nj) BODY(d, j, i);                                       \
   }
 #define GH_KERNEL2S(NAME, BODY)                                                 \
-  __global__ void __launch_bounds__(KB) NAME(Dev d, int ni, int stage) {        \
-    int i = blockIdx.x * blockDim.x + threadIdx.x;                               \
-    int j = blockIdx.y;                                                          \
-    if (i < ni) BODY(d, j, i, stage);                                            \
+  __global__ void __launch_bounds__(KB) NAME(Dev d, int ni, int remap, int stage) { \
+    int i, j;                                                                    \
+    tile_of(remap, j, i);                                                        \
+    if (i < ni && j < d.nj) BODY(d, j, i, stage);                                \
   }
 namespace {
 GH_KERNEL2(k_ghost_tracks, body_ghost_tracks)
@@ -890,6 +942,153 @@ __global__ void k_rg_apply(Dev d) {
 }  // namespace
 #endif
 
+
+// ===========================================================================
+// learned cons2prim guess: training and batch inference kernels
+#ifndef GH_EMU
+namespace {
+using namespace gh::mlp;
+
+// normalised features and targets of raw states (D, |S|, tau, p)
+__global__ void k_mlp_prepare(const float* __restrict__ w, const double* __restrict__ st, long long n,
+                              float* __restrict__ X, float* __restrict__ T) {
+  long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  double D = st[4 * s], S = st[4 * s + 1], tau = st[4 * s + 2], p = st[4 * s + 3];
+  features(w, D, S, tau, X + 3 * s);
+  double lpd = log(p / (D > 1e-300 ? D : 1e-300));
+  T[s] = (float)((lpd - w[ONORM + 6]) * w[ONORM + 7]);
+}
+
+// gradient of the mean squared error over the samples idx[0..nb) (FP32
+// forward/backward, one wavefront of 64 samples per block), accumulated into g
+__global__ void __launch_bounds__(64) k_mlp_grad(const float* __restrict__ w, const float* __restrict__ X,
+                                                 const float* __restrict__ T, const long long* __restrict__ idx,
+                                                 int nb, float* __restrict__ g, float* __restrict__ loss) {
+  __shared__ float h1s[64][H + 1], zs[64][H + 1], dys[64], xs[64][3];
+  int t = threadIdx.x;
+  int s = blockIdx.x * 64 + t;
+  bool valid = s < nb;
+  float x[3] = {0.f, 0.f, 0.f}, h1[H], h2[H], dy = 0.f;
+  if (valid) {
+    long long k = idx[s];
+    x[0] = X[3 * k]; x[1] = X[3 * k + 1]; x[2] = X[3 * k + 2];
+    float y = forward_f32(w, x, h1, h2);
+    float e = y - T[k];
+    dy = 2.f * e / nb;
+    atomicAdd(loss, e * e / nb);
+  } else {
+    for (int o = 0; o < H; ++o) { h1[o] = 0.f; h2[o] = 0.f; }
+  }
+  for (int o = 0; o < H; ++o) { h1s[t][o] = h1[o]; zs[t][o] = h2[o]; }
+  dys[t] = dy;
+  xs[t][0] = x[0]; xs[t][1] = x[1]; xs[t][2] = x[2];
+  __syncthreads();
+  {   // output layer: dw3[c], db3
+    float a = 0.f;
+    for (int q = 0; q < 64; ++q) a += dys[q] * zs[q][t];
+    atomicAdd(&g[OW3 + t], a);
+    if (t == 0) {
+      float b = 0.f;
+      for (int q = 0; q < 64; ++q) b += dys[q];
+      atomicAdd(&g[OB3], b);
+    }
+  }
+  __syncthreads();
+  for (int o = 0; o < H; ++o) zs[t][o] = zs[t][o] > 0.f ? dys[t] * w[OW3 + o] : 0.f;   // dz2
+  __syncthreads();
+  {   // hidden layer: dW2[o=t][i], db2[t]
+    float b = 0.f;
+    for (int q = 0; q < 64; ++q) b += zs[q][t];
+    atomicAdd(&g[OB2 + t], b);
+    for (int i = 0; i < H; ++i) {
+      float a = 0.f;
+      for (int q = 0; q < 64; ++q) a += zs[q][t] * h1s[q][i];
+      atomicAdd(&g[OW2 + t * H + i], a);
+    }
+  }
+  __syncthreads();
+  {   // dz1 of this sample (row t), then dW1, db1
+    float dz1[H];
+    for (int i = 0; i < H; ++i) {
+      float a = 0.f;
+      for (int o = 0; o < H; ++o) a += w[OW2 + o * H + i] * zs[t][o];
+      dz1[i] = h1s[t][i] > 0.f ? a : 0.f;
+    }
+    __syncthreads();
+    for (int i = 0; i < H; ++i) h1s[t][i] = dz1[i];
+  }
+  __syncthreads();
+  {
+    float b = 0.f, a0 = 0.f, a1 = 0.f, a2 = 0.f;
+    for (int q = 0; q < 64; ++q) {
+      float z = h1s[q][t];
+      b += z; a0 += z * xs[q][0]; a1 += z * xs[q][1]; a2 += z * xs[q][2];
+    }
+    atomicAdd(&g[OB1 + t], b);
+    atomicAdd(&g[OW1 + t * 3], a0);
+    atomicAdd(&g[OW1 + t * 3 + 1], a1);
+    atomicAdd(&g[OW1 + t * 3 + 2], a2);
+  }
+}
+
+// Adam (Kingma & Ba 2015) on the trained entries; clears the gradient
+__global__ void k_mlp_adam(float* w, float* g, float* m, float* v, int step, float lr) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= NTRAIN) return;
+  const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+  float gg = g[k];
+  m[k] = b1 * m[k] + (1.f - b1) * gg;
+  v[k] = b2 * v[k] + (1.f - b2) * gg * gg;
+  float mh = m[k] / (1.f - powf(b1, (float)step)), vh = v[k] / (1.f - powf(b2, (float)step));
+  w[k] -= lr * mh / (sqrtf(vh) + eps);
+  g[k] = 0.f;
+}
+
+// minibatch indices drawn with replacement (SplitMix64 counter stream)
+__global__ void k_mlp_sample(long long* idx, int nb, long long n, unsigned long long seed, long long step) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nb) return;
+  unsigned long long z = seed + (unsigned long long)(step * (long long)nb + s + 1) * 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  z ^= z >> 31;
+  idx[s] = (long long)(z % (unsigned long long)n);
+}
+
+// batch inference: pressure guesses of raw states; mfma = 1 runs the BF16 MFMA
+// wavefront path, 0 the FP32 reference path
+__global__ void __launch_bounds__(64) k_mlp_infer(const float* __restrict__ w, const bf16x8* __restrict__ w2b,
+                                                  const double* __restrict__ st, long long n,
+                                                  double* __restrict__ out, int mfma) {
+  __shared__ __bf16 lds[64 * H];
+  __shared__ float rows[64];
+  long long s = (long long)blockIdx.x * 64 + threadIdx.x;
+  bool valid = s < n;
+  double D = valid ? st[3 * s] : 1., S = valid ? st[3 * s + 1] : 0., tau = valid ? st[3 * s + 2] : 1.;
+  float x[3];
+  features(w, D, S, tau, x);
+  float y;
+  if (mfma) y = forward_wave(w, w2b, x, lds, rows);
+  else { float h1[H], h2[H]; y = forward_f32(w, x, h1, h2); }
+  if (valid) out[s] = pressure_of(w, D, y);
+}
+
+// lane-layout probe of v_mfma_f32_32x32x16_bf16: D = A B for row-major A (32x16), B (16x32)
+__global__ void __launch_bounds__(64) k_mfma_probe(const float* A, const float* B, float* Dm) {
+  int l = threadIdx.x;
+  bf16x8 a, b;
+  for (int e = 0; e < 8; ++e) {
+    a[e] = (__bf16)A[(l % 32) * 16 + 8 * (l / 32) + e];
+    b[e] = (__bf16)B[(8 * (l / 32) + e) * 32 + l % 32];
+  }
+  floatx16 acc = {};
+  acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+  for (int r = 0; r < 16; ++r) Dm[d_row(l, r) * 32 + l % 32] = acc[r];
+}
+}  // namespace
+#endif
+
 // ===========================================================================
 // host side
 struct KTimer {
@@ -907,10 +1106,15 @@ struct gammahip_ctx {
   // events for kernel timing (0 = update, 1 = faces, 2 = f1faces, 3 = gradf1)
   bool timing = false;
   KTimer kt[16];
+  // launch options (gammahip_set_option; the autotuner picks them)
+  int block = 256;         // threads per workgroup of the per-cell kernels (64..256)
+  int remap = 0;           // XCD-aware tile order
+  int use_graph = 0;       // replay steps from a captured hipGraph
 #ifndef GH_EMU
   hipStream_t stream = nullptr;
   hipEvent_t ev0[16], ev1[16];
   bool ev_pending[16];
+  hipGraphExec_t graph = nullptr;   // two steps (an even number of buffer swaps)
 #endif
 };
 
@@ -974,13 +1178,15 @@ int d2h(gammahip_ctx* c, void* dst, const void* src, size_t bytes) {
 #else
 #define L2(c, kern, ntr, ni)                                                         \
   do {                                                                               \
-    dim3 g_(((ni) + KB - 1) / KB, (ntr));                                            \
-    hipLaunchKernelGGL(kern, g_, dim3(KB), 0, (c)->stream, (c)->d, (int)(ni));     \
+    int bs_ = (c)->block;                                                            \
+    dim3 g_(((ni) + bs_ - 1) / bs_, (ntr));                                          \
+    hipLaunchKernelGGL(kern, g_, dim3(bs_), 0, (c)->stream, (c)->d, (int)(ni), (c)->remap); \
   } while (0)
 #define L2S(c, kern, ntr, ni, st)                                                    \
   do {                                                                               \
-    dim3 g_(((ni) + KB - 1) / KB, (ntr));                                            \
-    hipLaunchKernelGGL(kern, g_, dim3(KB), 0, (c)->stream, (c)->d, (int)(ni), (int)(st)); \
+    int bs_ = (c)->block;                                                            \
+    dim3 g_(((ni) + bs_ - 1) / bs_, (ntr));                                          \
+    hipLaunchKernelGGL(kern, g_, dim3(bs_), 0, (c)->stream, (c)->d, (int)(ni), (c)->remap, (int)(st)); \
   } while (0)
 #endif
 
@@ -1141,6 +1347,31 @@ int post(gammahip_ctx* c) {
 
 // ===========================================================================
 // C-ABI
+static void drop_graph(gammahip_ctx* c) {
+#ifndef GH_EMU
+  if (c->graph) { (void)hipGraphExecDestroy(c->graph); c->graph = nullptr; }
+#else
+  (void)c;
+#endif
+}
+
+#ifndef GH_EMU
+namespace {
+struct DevBuf {
+  std::vector<void*> p;
+  ~DevBuf() { for (void* q : p) (void)hipFree(q); }
+  template <class T> T* get(size_t n) {
+    void* q = nullptr;
+    if (hipMalloc(&q, sizeof(T) * (n ? n : 1)) != hipSuccess) return nullptr;
+    p.push_back(q);
+    return (T*)q;
+  }
+};
+int plain_fail(const char* m) { fprintf(stderr, "gammahip: %s\n", m); return -1; }
+#define CK(x) do { if ((x) != hipSuccess) return plain_fail(#x); } while (0)
+}  // namespace
+#endif
+
 extern "C" {
 
 int gammahip_create(const gammahip_config* cfg, gammahip_ctx** out) {
@@ -1219,8 +1450,9 @@ int gammahip_destroy(gammahip_ctx* c) {
 #ifdef GH_EMU
   for (void* p : c->allocs) free(p);
 #else
-  if (c->stream) hipStreamSynchronize(c->stream);
-  for (void* p : c->allocs) hipFree(p);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  drop_graph(c);
+  for (void* p : c->allocs) (void)hipFree(p);
   if (c->stream) {
     for (int k = 0; k < 16; ++k) { hipEventDestroy(c->ev0[k]); hipEventDestroy(c->ev1[k]); }
     hipStreamDestroy(c->stream);
@@ -1325,9 +1557,42 @@ int gammahip_load(gammahip_ctx* c, const double* x, const double* dx, const doub
   return post(c);
 }
 
+int gammahip_set_option(gammahip_ctx* c, int key, long long value) {
+  if (!c) return -1;
+  drop_graph(c);
+  switch (key) {
+    case GAMMAHIP_OPT_BLOCK:
+      if (value != 64 && value != 128 && value != 256) return fail(c, "block must be 64, 128 or 256");
+      c->block = (int)value; return 0;
+    case GAMMAHIP_OPT_REMAP: c->remap = value ? 1 : 0; return 0;
+    case GAMMAHIP_OPT_GRAPH: c->use_graph = value ? 1 : 0; return 0;
+    default: return fail(c, "unknown option");
+  }
+}
+
 int gammahip_step(gammahip_ctx* c, int nsteps) {
   if (!c) return -1;
-  for (int s = 0; s < nsteps; ++s) {
+  int s = 0;
+#ifndef GH_EMU
+  if (c->use_graph && !c->timing && nsteps >= 2) {
+    if (!c->graph) {
+      // two steps = six stage updates: the face-position and transverse-gradient
+      // double buffers come back to their starting roles, so the pointers baked
+      // into the graph stay valid for every replay
+      hipGraph_t g = nullptr;
+      HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeGlobal));
+      int r = dispatch(c, [&](auto P) { P.step(c); P.step(c); });
+      hipError_t e = hipStreamEndCapture(c->stream, &g);
+      if (r || e != hipSuccess) return fail(c, "graph capture failed");
+      e = hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      if (e != hipSuccess) { c->graph = nullptr; return fail(c, "graph instantiation failed"); }
+    }
+    for (; s + 1 < nsteps; s += 2) HIPCHK(hipGraphLaunch(c->graph, c->stream));
+  }
+#endif
+  for (; s < nsteps; ++s) {
+    drop_graph(c);       // a single step flips the buffer roles the graph baked in
     int r = dispatch(c, [&](auto P) { P.step(c); });
     if (r) return r;
     if (post(c)) return -1;
@@ -1335,13 +1600,14 @@ int gammahip_step(gammahip_ctx* c, int nsteps) {
   return 0;
 }
 
-int gammahip_regrid(gammahip_ctx* c) { return dispatch(c, [&](auto P) { P.regrid(c); }) || post(c); }
-int gammahip_update_ghosts(gammahip_ctx* c) { return dispatch(c, [&](auto P) { P.ghosts(c); }) || post(c); }
-int gammahip_prep_for_update(gammahip_ctx* c) { return dispatch(c, [&](auto P) { P.prep(c); }) || post(c); }
-int gammahip_evolve(gammahip_ctx* c) { return dispatch(c, [&](auto P) { P.evolve(c); }) || post(c); }
-int gammahip_apply(gammahip_ctx* c, int what) { return dispatch(c, [&](auto P) { P.apply(c, what); }) || post(c); }
+int gammahip_regrid(gammahip_ctx* c) { drop_graph(c); return dispatch(c, [&](auto P) { P.regrid(c); }) || post(c); }
+int gammahip_update_ghosts(gammahip_ctx* c) { drop_graph(c); return dispatch(c, [&](auto P) { P.ghosts(c); }) || post(c); }
+int gammahip_prep_for_update(gammahip_ctx* c) { drop_graph(c); return dispatch(c, [&](auto P) { P.prep(c); }) || post(c); }
+int gammahip_evolve(gammahip_ctx* c) { drop_graph(c); return dispatch(c, [&](auto P) { P.evolve(c); }) || post(c); }
+int gammahip_apply(gammahip_ctx* c, int what) { drop_graph(c); return dispatch(c, [&](auto P) { P.apply(c, what); }) || post(c); }
 
 int gammahip_collect_dt(gammahip_ctx* c, double* dt_out) {
+  drop_graph(c);
   if (dispatch(c, [&](auto P) { P.collect_dt(c); }) || post(c)) return -1;
   if (dt_out) return d2h(c, dt_out, c->d.tdt, sizeof(double));
   return 0;
@@ -1487,6 +1753,111 @@ int gammahip_stats_get(gammahip_ctx* c, gammahip_stats* st) {
 int gammahip_regrid_counts(gammahip_ctx* c, long long* splits, long long* merges) {
   if (d2h(c, splits, c->d.nsplit, sizeof(long long) * c->d.nj)) return -1;
   return d2h(c, merges, c->d.nmerge, sizeof(long long) * c->d.nj);
+}
+
+
+// ---------------------------------------------------------------------------
+// learned cons2prim guess (no counterpart in the reference)
+
+int gammahip_mlp_nparams(void) { return gh::mlp::NPARAM; }
+
+int gammahip_mlp_train(const double* states, long long n, float* params, int steps, int batch,
+                       float lr, unsigned long long seed, float* loss_hist) {
+#ifdef GH_EMU
+  (void)states; (void)n; (void)params; (void)steps; (void)batch; (void)lr; (void)seed; (void)loss_hist;
+  return -1;
+#else
+  using namespace gh::mlp;
+  if (n < 1 || batch < 64 || steps < 1) return plain_fail("bad training arguments");
+  DevBuf B;
+  double* st = B.get<double>(4 * n);
+  float *w = B.get<float>(NPARAM), *g = B.get<float>(NPARAM), *m = B.get<float>(NPARAM), *v = B.get<float>(NPARAM);
+  float *X = B.get<float>(3 * n), *T = B.get<float>(n), *loss = B.get<float>(1);
+  long long* idx = B.get<long long>(batch);
+  if (!st || !w || !g || !m || !v || !X || !T || !loss || !idx) return plain_fail("allocation failed");
+  CK(hipMemcpy(st, states, sizeof(double) * 4 * n, hipMemcpyHostToDevice));
+  CK(hipMemcpy(w, params, sizeof(float) * NPARAM, hipMemcpyHostToDevice));
+  CK(hipMemset(g, 0, sizeof(float) * NPARAM)); CK(hipMemset(m, 0, sizeof(float) * NPARAM));
+  CK(hipMemset(v, 0, sizeof(float) * NPARAM));
+  hipLaunchKernelGGL(k_mlp_prepare, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, 0, w, st, n, X, T);
+  for (int k = 0; k < steps; ++k) {
+    CK(hipMemset(loss, 0, sizeof(float)));
+    hipLaunchKernelGGL(k_mlp_sample, dim3((batch + 255) / 256), dim3(256), 0, 0, idx, batch, n, seed, (long long)k);
+    hipLaunchKernelGGL(k_mlp_grad, dim3((batch + 63) / 64), dim3(64), 0, 0, w, X, T, idx, batch, g, loss);
+    hipLaunchKernelGGL(k_mlp_adam, dim3((NTRAIN + 255) / 256), dim3(256), 0, 0, w, g, m, v, k + 1, lr);
+    if (loss_hist) CK(hipMemcpy(&loss_hist[k], loss, sizeof(float), hipMemcpyDeviceToHost));
+  }
+  CK(hipGetLastError());
+  CK(hipMemcpy(params, w, sizeof(float) * NPARAM, hipMemcpyDeviceToHost));
+  return 0;
+#endif
+}
+
+int gammahip_mlp_grad(const float* params, const double* states, int n, float* grad, float* loss_out) {
+#ifdef GH_EMU
+  (void)params; (void)states; (void)n; (void)grad; (void)loss_out;
+  return -1;
+#else
+  using namespace gh::mlp;
+  DevBuf B;
+  double* st = B.get<double>(4 * (size_t)n);
+  float *w = B.get<float>(NPARAM), *g = B.get<float>(NPARAM), *X = B.get<float>(3 * (size_t)n);
+  float *T = B.get<float>(n), *loss = B.get<float>(1);
+  long long* idx = B.get<long long>(n);
+  if (!st || !w || !g || !X || !T || !loss || !idx) return plain_fail("allocation failed");
+  std::vector<long long> hi(n);
+  for (int k = 0; k < n; ++k) hi[k] = k;
+  CK(hipMemcpy(st, states, sizeof(double) * 4 * n, hipMemcpyHostToDevice));
+  CK(hipMemcpy(w, params, sizeof(float) * NPARAM, hipMemcpyHostToDevice));
+  CK(hipMemcpy(idx, hi.data(), sizeof(long long) * n, hipMemcpyHostToDevice));
+  CK(hipMemset(g, 0, sizeof(float) * NPARAM)); CK(hipMemset(loss, 0, sizeof(float)));
+  hipLaunchKernelGGL(k_mlp_prepare, dim3((n + 255) / 256), dim3(256), 0, 0, w, st, (long long)n, X, T);
+  hipLaunchKernelGGL(k_mlp_grad, dim3((n + 63) / 64), dim3(64), 0, 0, w, X, T, idx, n, g, loss);
+  CK(hipGetLastError());
+  CK(hipMemcpy(grad, g, sizeof(float) * NTRAIN, hipMemcpyDeviceToHost));
+  CK(hipMemcpy(loss_out, loss, sizeof(float), hipMemcpyDeviceToHost));
+  return 0;
+#endif
+}
+
+int gammahip_mlp_infer(const float* params, const double* states, long long n, double* p_out, int mfma) {
+#ifdef GH_EMU
+  (void)params; (void)states; (void)n; (void)p_out; (void)mfma;
+  return -1;
+#else
+  using namespace gh::mlp;
+  DevBuf B;
+  double *st = B.get<double>(3 * n), *out = B.get<double>(n);
+  float* w = B.get<float>(NPARAM);
+  bf16x8* w2b = B.get<bf16x8>(8 * 64);
+  if (!st || !out || !w || !w2b) return plain_fail("allocation failed");
+  std::vector<__bf16> hb(8 * 64 * 8);
+  pack_w2(params, hb.data());
+  CK(hipMemcpy(st, states, sizeof(double) * 3 * n, hipMemcpyHostToDevice));
+  CK(hipMemcpy(w, params, sizeof(float) * NPARAM, hipMemcpyHostToDevice));
+  CK(hipMemcpy(w2b, hb.data(), sizeof(__bf16) * hb.size(), hipMemcpyHostToDevice));
+  hipLaunchKernelGGL(k_mlp_infer, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, 0, w, w2b, st, n, out, mfma);
+  CK(hipGetLastError());
+  CK(hipMemcpy(p_out, out, sizeof(double) * n, hipMemcpyDeviceToHost));
+  return 0;
+#endif
+}
+
+int gammahip_mfma_probe(const float* A, const float* Bm, float* Dm) {
+#ifdef GH_EMU
+  (void)A; (void)Bm; (void)Dm;
+  return -1;
+#else
+  DevBuf B;
+  float *a = B.get<float>(32 * 16), *b = B.get<float>(16 * 32), *d = B.get<float>(32 * 32);
+  if (!a || !b || !d) return plain_fail("allocation failed");
+  CK(hipMemcpy(a, A, sizeof(float) * 512, hipMemcpyHostToDevice));
+  CK(hipMemcpy(b, Bm, sizeof(float) * 512, hipMemcpyHostToDevice));
+  hipLaunchKernelGGL(k_mfma_probe, dim3(1), dim3(64), 0, 0, a, b, d);
+  CK(hipGetLastError());
+  CK(hipMemcpy(Dm, d, sizeof(float) * 1024, hipMemcpyDeviceToHost));
+  return 0;
+#endif
 }
 
 int gammahip_kernel_times(gammahip_ctx* c, double* ms, long long* n) {

@@ -24,6 +24,7 @@ F_GRADMV0, F_GRADF10 = 300, 400
 FACE_X, FACE_V, FACE_LL, FACE_LR, FACE_FLUX0 = 1000, 1001, 1002, 1003, 1010
 RESET_SHOCK, DETECT_SHOCK, INJECT, INIT_GAMMAS, ADVANCE_TIME = range(5)
 C2P_BRENT, C2P_NEWTON = 0, 1
+OPT_BLOCK, OPT_REMAP, OPT_GRAPH = 0, 1, 2
 
 
 class Config(C.Structure):
@@ -77,7 +78,12 @@ def load_library(path: str = LIB_PATH):
         "gammahip_track_info": (I, [P, P, P, P, P]), "gammahip_dims": (I, [P, P, P]),
         "gammahip_time": (I, [P, P, P]), "gammahip_stats_get": (I, [P, C.POINTER(Stats)]),
         "gammahip_regrid_counts": (I, [P, P, P]), "gammahip_kernel_times": (I, [P, P, P]),
-        "gammahip_sync": (I, [P]),
+        "gammahip_sync": (I, [P]), "gammahip_set_option": (I, [P, I, C.c_longlong]),
+        "gammahip_mlp_nparams": (I, []),
+        "gammahip_mlp_train": (I, [P, C.c_longlong, P, I, I, C.c_float, C.c_ulonglong, P]),
+        "gammahip_mlp_grad": (I, [P, P, I, P, P]),
+        "gammahip_mlp_infer": (I, [P, P, C.c_longlong, P, I]),
+        "gammahip_mfma_probe": (I, [P, P, P]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -191,6 +197,12 @@ class Grid:
 
     def sync(self):
         self._check(self.L.gammahip_sync(self.h))
+
+    def set_options(self, block=None, remap=None, graph=None):
+        """Launch options (threads per workgroup, XCD-aware tile order, hipGraph replay)."""
+        for key, val in ((OPT_BLOCK, block), (OPT_REMAP, remap), (OPT_GRAPH, graph)):
+            if val is not None:
+                self._check(self.L.gammahip_set_option(self.h, key, int(val)))
 
     # ---- state access ----
     def cells(self, field: int) -> np.ndarray:

@@ -137,6 +137,18 @@ def shock_tube_1d(n: int = 400, rho_l=10., p_l=13.33, rho_r=1., p_r=1.e-6,
                  np.zeros(1), np.ones(1), prim)
 
 
+def expanding_tube_1d(n: int = 200, v_edge=0.5) -> Setup:
+    """Shock tube whose boundary faces move outward (userKinematics v_in = -v,
+    v_out = +v), with the aspect-ratio rule of BM1D.cpp:277-319 on a flat
+    geometry: the edge cells grow and split repeatedly next to the ghosts."""
+    s = shock_tube_1d(n, p_r=1.e-6, xmid=0.1)
+    s.prim[RHO, 0] *= 1. + 0.5 * s.x[0]          # density ramp at uniform pressure
+    s.cfg = s.cfg.replace(nmax=2 * n, kin_mode=KIN_FIXED, v_in=-v_edge, v_out=v_edge,
+                          rg_mode=RG_BM1D, rg=(3., 0.1, 1. / n, 1.e-6, 1., 0.))
+    s.name = "expanding_tube_1d"
+    return s
+
+
 def bm_profile(r, t, lfac_shock2_0, t0, rho_a, eta, gamma):
     """Blandford-McKee self-similar profile in code units (c = 1, k = 0);
     BM1D.cpp:62-86 with lfacShock2(t) = lfacShock2(t0) (t0/t)^3."""

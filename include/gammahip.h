@@ -115,6 +115,29 @@ int  gammahip_kernel_times(gammahip_ctx* ctx, double* ms_per_launch /* [16] */,
                            long long* launches /* [16] */);
 int  gammahip_sync(gammahip_ctx* ctx);
 
+/* Launch options (no counterpart in the reference; chosen by the autotuner in
+ * gamma_amd/autotune.py): threads per workgroup of the per-cell kernels (64,
+ * 128, 256), XCD-aware ordering of the (track, chunk) tiles, and replay of
+ * two-step hipGraphs by gammahip_step. */
+enum { GAMMAHIP_OPT_BLOCK = 0, GAMMAHIP_OPT_REMAP = 1, GAMMAHIP_OPT_GRAPH = 2 };
+int  gammahip_set_option(gammahip_ctx* ctx, int key, long long value);
+
+/* ---- learned cons2prim guess (no counterpart in the reference) ----
+ * A 3 -> 64 -> 64 -> 1 MLP predicting log p of a conserved state from
+ * (log D, |S|/D, tau/D); parameters are one float array of
+ * gammahip_mlp_nparams() entries (weights, then the fixed feature/target
+ * normalisation).  states: [n][4] = (D, |S|, tau, p) for training,
+ * [n][3] = (D, |S|, tau) for inference.  Training runs FP32 forward/backward
+ * kernels with Adam on minibatches drawn with replacement; inference runs the
+ * hidden layer as BF16 MFMA with FP32 accumulation (mfma = 1) or FP32 (0). */
+int  gammahip_mlp_nparams(void);
+int  gammahip_mlp_train(const double* states, long long n, float* params, int steps, int batch,
+                        float lr, unsigned long long seed, float* loss_hist /* [steps] */);
+int  gammahip_mlp_grad(const float* params, const double* states, int n, float* grad, float* loss);
+int  gammahip_mlp_infer(const float* params, const double* states, long long n, double* p_out,
+                        int mfma);
+int  gammahip_mfma_probe(const float* A /* 32x16 */, const float* B /* 16x32 */, float* D /* 32x32 */);
+
 #ifdef __cplusplus
 }
 #endif

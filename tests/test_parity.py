@@ -24,6 +24,8 @@ CASES = {
                                                    geom=setups.GEOM_SPH, shock=1), 80),
     "blandford_mckee": (lambda: setups.blandford_mckee_1d(400), 60),
     "riemann_2d": (lambda: setups.riemann_2d(64, 16), 60),
+    # edge cells grow, split next to the ghosts, which must follow their faces
+    "expanding_tube": (lambda: setups.expanding_tube_1d(200), 300),
 }
 
 
