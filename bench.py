@@ -72,11 +72,13 @@ def cpu_baseline(sample_tracks: int = 64, nx: int = 2048, steps: int = 30) -> di
                       f"after 2 warm-up steps, {el:.1f} s"}
 
 
-def run_form(setup, solver, options, steps, warmup, timing_steps):
+def run_form(setup, solver, options, steps, warmup, timing_steps, surrogate=None):
     """Time `steps` RK3 steps after `warmup` on a fresh grid; then per-kernel
     HIP-event times over `timing_steps` extra steps."""
     from gamma_amd import hip
     g = hip.Grid(setup, solver=solver)
+    if surrogate is not None:
+        g.load_surrogate(surrogate)
     if options:
         g.set_options(**options)
     g.step(warmup)
@@ -141,18 +143,34 @@ def main():
 
     batched = None
     if not args.no_batched:
-        print("[bench] autotuning the batched form", file=sys.stderr, flush=True)
+        from gamma_amd import surrogate as SG
+        print("[bench] training the cons2prim network on solver states", file=sys.stderr, flush=True)
         scratch = hip.Grid(s, solver=hip.C2P_NEWTON)
         scratch.step(args.warmup)
-        tuned = autotune.tune(scratch)
+        t0 = time.perf_counter()
+        train_states = np.concatenate([
+            SG.states_from_grid(scratch, s.cfg.gamma, max_states=1 << 20),
+            SG.synthetic_states(1 << 20, s.cfg.gamma, seed=3)])
+        w, loss = SG.train(train_states, steps=3000, batch=4096, lr=2e-3, seed=5)
+        train_s = time.perf_counter() - t0
+        print("[bench] autotuning the batched form", file=sys.stderr, flush=True)
+        tuned = autotune.tune(scratch, surrogate=w)
         scratch.close()
-        opts = {k: tuned["best"][k] for k in ("block", "remap", "graph")}
-        b = run_form(s, hip.C2P_NEWTON, opts, args.steps, args.warmup, args.timing_steps)
+        opts = {k: tuned["best"][k] for k in ("block", "remap", "graph", "mlp_thr")}
+        b = run_form(s, hip.C2P_NEWTON, opts, args.steps, args.warmup, args.timing_steps,
+                     surrogate=w if opts["mlp_thr"] is not None else None)
         print(f"[bench] batched {b['ms_per_step']:.3f} ms/step with {opts}", file=sys.stderr, flush=True)
+        bst = b["stats"]
         batched = {"value": b["value"], "ms_per_step": b["ms_per_step"], "options": opts,
-                   "cons2prim": "safeguarded Newton, previous-pressure guess",
+                   "cons2prim": "safeguarded Newton, previous-pressure start"
+                                + ("" if opts["mlp_thr"] is None else
+                                   ", learned start judged against it by |f|"),
                    "speedup_vs_plain": b["value"] / plain["value"],
-                   "c2p_iters_per_call": b["stats"]["c2p_iters"] / max(b["stats"]["c2p_calls"], 1),
+                   "c2p_iters_per_call": bst["c2p_iters"] / max(bst["c2p_calls"], 1),
+                   "network_starts": bst["mlp_used"], "c2p_calls": bst["c2p_calls"],
+                   "failures": {k: bst[k] for k in ("c2p_nobracket", "c2p_maxiter", "c2p_phicap")},
+                   "network": {"train_states": int(train_states.shape[0]), "train_steps": 3000,
+                               "final_loss": float(loss[-100:].mean()), "train_s": round(train_s, 2)},
                    "kernels": kernel_table(b, nq), "autotune": tuned["table"]}
 
     cpu = None if args.no_cpu_baseline else cpu_baseline()

@@ -48,13 +48,14 @@ enum { RG_NONE = 0, RG_AR2D = 1, RG_BM1D = 2 };
 // ---------------------------------------------------------------------------
 // device view (passed by value to every kernel)
 struct Dev {
-  int nd, nq, nqs, ngst, nj, nax, jL, jR, fcap, shock, sync, solver, guess, maxit, rg_mode;
+  int nd, nq, nqs, ngst, nj, nax, jL, jR, fcap, shock, sync, solver, guess, maxit;
+  int rg_mode;             // bit set of the tracks' regrid rules (1 << RG_*)
   long long cs;            // nj * nax
   long long fs1;           // nj * fcap
   Phys P;
   double rg[6];
   // per track
-  int *ntrack, *tgeom, *tkin, *tbnd, *tfirst, *tiS;
+  int *ntrack, *tgeom, *tkin, *tbnd, *tfirst, *tiS, *trg;
   double *tvin, *tvout, *tbm, *ty, *tdy, *tdt, *tt, *tmin;
   // cells
   double *prim, *cx, *cdx, *dtloc, *U0, *dV0, *gmv, *gf1, *gf1n, *sd, *pspec;
@@ -65,6 +66,11 @@ struct Dev {
   double *ff1, *ff1sd;
   unsigned long long* cnt;
   long long *nsplit, *nmerge;
+  // learned cons2prim guess: FP32 parameters and the packed BF16 hidden-layer
+  // operands (null when the network is not loaded)
+  const float* mw;
+  const void* mw2b;
+  double mlp_thr;          // relative residual of the previous pressure that asks for the network
 };
 
 GH_HD long long CI(const Dev& d, int j, int i) { return (long long)j * d.nax + i; }
@@ -503,101 +509,136 @@ GH_HD void body_f1_shock(const Dev& d, int j, int i) {
 }
 
 // ---------------------------------------------------------------------------
-// Grid::update + Cell::update + RK3 averages (1d.cpp:798-841, 2d.cpp:974-1036,
-// cell.cpp:102-127, rk3.cpp:12-195).  One thread owns face i and cell i.
-template <int ND, int NQ>
-GH_HD void body_update(const Dev& d, int j, int i, int stage) {
-  if (!active_track(d, j)) return;
-  int nt = d.ntrack[j];
-  if (i >= nt) return;
+// Starting-pressure provider of the update's cons2prim calls: NoGuess keeps
+// the previous pressure alone (the reference's rule).  A provider is called by
+// every thread of the kernel (want = false for threads without a cell), so
+// providers may use wavefront-wide operations.
+struct NoGuess {
+  GH_HD double operator()(bool, double, double, double, double, int) const { return -1.; }
+};
+
+template <int ND, int NQ, class GuessT>
+GH_HD void body_update_g(const Dev& d, int j, int i, int stage, bool valid, const GuessT& guess) {
+  bool run = valid && active_track(d, j);
+  int nt = run ? d.ntrack[j] : 0;
+  run = run && i < nt;
   int iL = d.ngst - 1, iR = nt - d.ngst;
-  int geom = d.tgeom[j];
+  int geom = run ? d.tgeom[j] : 0;
   double G = d.P.gamma;
-  double dt = d.tdt[j];
+  double dt = run ? d.tdt[j] : 0.;
   double w0 = stage == 2 ? 3. / 4. : 1. / 3.;
   double w1 = stage == 2 ? 1. / 4. : 2. / 3.;
-  long long k = CI(d, j, i);
-  if (i <= nt - 2) {
-    double xo = d.fx[k];
-    double xn = xo + dt * d.fv[k];
-    if (stage == 1) d.fx0[k] = xo;
-    double xa = xn;
-    if (stage > 1 && i >= iL && i <= iR - 1) xa = w0 * d.fx0[k] + w1 * xn;
-    d.fxn[k] = xa;
-  }
-  if (stage > 1 && (i == iL || i == iR)) {
-    // CellGeomFromInterfacePos covers cells 1..ntrack-2 (1d.cpp:834), so the
-    // ghost cells next to the active range follow their faces too (split at
-    // the first / last active cell reads them)
-    double xoL = d.fx[k - 1] + dt * d.fv[k - 1];
-    double xoR = d.fx[k] + dt * d.fv[k];
-    if (i - 1 >= iL && i - 1 <= iR - 1) xoL = w0 * d.fx0[k - 1] + w1 * xoL;
-    if (i >= iL && i <= iR - 1) xoR = w0 * d.fx0[k] + w1 * xoR;
-    d.cx[k] = (xoR + xoL) / 2.;
-    d.cdx[k] = (xoR - xoL);
-    return;
-  }
-  if (i < iL + 1 || i > iR - 1) return;
-  double xnL = d.fx[k - 1] + dt * d.fv[k - 1];
-  double xnR = d.fx[k] + dt * d.fv[k];
+  long long k = run ? CI(d, j, i) : 0;
+  bool cell = false;
   double p[NQ], U[NQ];
-  load_v<ND, NQ>(d, d.prim, k, p);
-  Geo g = geo_of<ND>(d, j, k);
-  prim2cons<ND, NQ>(p, U, sweep_r<ND>(d, k, g), geom, G);
-  if (stage == 1) {
-    store_v<ND, NQ>(d, d.U0, k, U);
-    d.dV0[k] = g.dV;
-  }
-  for (int q = 0; q < NQ; ++q) U[q] *= g.dV;
-  {
-    double fa[NQ], fb[NQ];
-    load_v<ND, NQ>(d, d.fflux, k - 1, fa);
-    load_v<ND, NQ>(d, d.fflux, k, fb);
-    for (int q = 0; q < NQ; ++q) U[q] += (fa[q] - fb[q]) * dt;
-  }
-  if (ND == 2) {
-    double up[NQ], dn[NQ];
-    for (int q = 0; q < NQ; ++q) { up[q] = 0.; dn[q] = 0.; }
-    int cnt = d.nbucnt[k], off = d.f1off[k];
-    for (int n = 0; n < cnt && off + n < d.fcap; ++n) {
-      long long fk = (long long)j * d.fcap + off + n;
-      for (int q = 0; q < NQ; ++q) up[q] += d.ff1[q * d.fs1 + fk];
+  for (int q = 0; q < NQ; ++q) { p[q] = 0.; U[q] = 0.; }
+  double xnL = 0., xnR = 0., cxn = 0., cdxn = 0., r1 = 0.;
+  Geo g2 = {};
+  if (run) {
+    if (i <= nt - 2) {
+      double xo = d.fx[k];
+      double xn = xo + dt * d.fv[k];
+      if (stage == 1) d.fx0[k] = xo;
+      double xa = xn;
+      if (stage > 1 && i >= iL && i <= iR - 1) xa = w0 * d.fx0[k] + w1 * xn;
+      d.fxn[k] = xa;
     }
-    for_incoming_f1(d, j, i, [&](long long fk) {
-      for (int q = 0; q < NQ; ++q) dn[q] += d.ff1[q * d.fs1 + fk];
-    });
-    for (int q = 0; q < NQ; ++q) U[q] += (dn[q] - up[q]) * dt;
+    if (stage > 1 && (i == iL || i == iR)) {
+      // CellGeomFromInterfacePos covers cells 1..ntrack-2 (1d.cpp:834), so the
+      // ghost cells next to the active range follow their faces too (split at
+      // the first / last active cell reads them)
+      double xoL = d.fx[k - 1] + dt * d.fv[k - 1];
+      double xoR = d.fx[k] + dt * d.fv[k];
+      if (i - 1 >= iL && i - 1 <= iR - 1) xoL = w0 * d.fx0[k - 1] + w1 * xoL;
+      if (i >= iL && i <= iR - 1) xoR = w0 * d.fx0[k] + w1 * xoR;
+      d.cx[k] = (xoR + xoL) / 2.;
+      d.cdx[k] = (xoR - xoL);
+    } else if (i >= iL + 1 && i <= iR - 1) {
+      cell = true;
+    }
   }
-  sources<ND, NQ>(p, U, d.cx[k], d.cdx[k], d.ty[j], g.dV, dt, geom, G);
-  if (NQ == 10 && d.sync) radiative(p, U, g.dV, dt, d.P);
-  double cxn = (xnR + xnL) / 2., cdxn = (xnR - xnL);
-  Geo g2 = cell_geo<ND>(cxn, cdxn, d.ty[j], d.tdy[j], geom);
-  for (int q = 0; q < NQ; ++q) U[q] /= g2.dV;
-  C2PStat st = {0, 0};
-  cons2prim<ND, NQ>(p, U, g2.cen0, geom, G, d.solver, -1., d.maxit, &st);
+  if (cell) {
+    xnL = d.fx[k - 1] + dt * d.fv[k - 1];
+    xnR = d.fx[k] + dt * d.fv[k];
+    load_v<ND, NQ>(d, d.prim, k, p);
+    Geo g = geo_of<ND>(d, j, k);
+    prim2cons<ND, NQ>(p, U, sweep_r<ND>(d, k, g), geom, G);
+    if (stage == 1) {
+      store_v<ND, NQ>(d, d.U0, k, U);
+      d.dV0[k] = g.dV;
+    }
+    for (int q = 0; q < NQ; ++q) U[q] *= g.dV;
+    {
+      double fa[NQ], fb[NQ];
+      load_v<ND, NQ>(d, d.fflux, k - 1, fa);
+      load_v<ND, NQ>(d, d.fflux, k, fb);
+      for (int q = 0; q < NQ; ++q) U[q] += (fa[q] - fb[q]) * dt;
+    }
+    if (ND == 2) {
+      double up[NQ], dn[NQ];
+      for (int q = 0; q < NQ; ++q) { up[q] = 0.; dn[q] = 0.; }
+      int cnt = d.nbucnt[k], off = d.f1off[k];
+      for (int n = 0; n < cnt && off + n < d.fcap; ++n) {
+        long long fk = (long long)j * d.fcap + off + n;
+        for (int q = 0; q < NQ; ++q) up[q] += d.ff1[q * d.fs1 + fk];
+      }
+      for_incoming_f1(d, j, i, [&](long long fk) {
+        for (int q = 0; q < NQ; ++q) dn[q] += d.ff1[q * d.fs1 + fk];
+      });
+      for (int q = 0; q < NQ; ++q) U[q] += (dn[q] - up[q]) * dt;
+    }
+    sources<ND, NQ>(p, U, d.cx[k], d.cdx[k], d.ty[j], g.dV, dt, geom, G);
+    if (NQ == 10 && d.sync) radiative(p, U, g.dV, dt, d.P);
+    cxn = (xnR + xnL) / 2.;
+    cdxn = (xnR - xnL);
+    g2 = cell_geo<ND>(cxn, cdxn, d.ty[j], d.tdy[j], geom);
+    for (int q = 0; q < NQ; ++q) U[q] /= g2.dV;
+    r1 = g2.cen0;
+  }
+  C2PStat st = {0, 0, 0, 0};
+  {
+    double D = 1., S = 0., tau = 1.;
+    if (cell) c2p_inputs<ND, NQ>(U, r1, geom, &D, &S, &tau);
+    double gs = guess(cell, D, S, tau, p[PPP], geom);
+    if (cell) cons2prim<ND, NQ>(p, U, r1, geom, G, d.solver, gs, d.maxit, &st);
+  }
   int ncalls = 1;
   if (stage > 1) {
-    double U0[NQ];
-    load_v<ND, NQ>(d, d.U0, k, U0);
-    double dV0 = d.dV0[k];
-    for (int q = 0; q < NQ; ++q) U[q] = w0 * U0[q] * dV0 + w1 * U[q] * g2.dV;
-    double xaL = w0 * d.fx0[k - 1] + w1 * xnL;
-    double xaR = w0 * d.fx0[k] + w1 * xnR;
-    cxn = (xaR + xaL) / 2.;
-    cdxn = (xaR - xaL);
-    Geo g3 = cell_geo<ND>(cxn, cdxn, d.ty[j], d.tdy[j], geom);
-    for (int q = 0; q < NQ; ++q) U[q] /= g3.dV;
-    cons2prim<ND, NQ>(p, U, cxn, geom, G, d.solver, -1., d.maxit, &st);
+    if (cell) {
+      double U0[NQ];
+      load_v<ND, NQ>(d, d.U0, k, U0);
+      double dV0 = d.dV0[k];
+      for (int q = 0; q < NQ; ++q) U[q] = w0 * U0[q] * dV0 + w1 * U[q] * g2.dV;
+      double xaL = w0 * d.fx0[k - 1] + w1 * xnL;
+      double xaR = w0 * d.fx0[k] + w1 * xnR;
+      cxn = (xaR + xaL) / 2.;
+      cdxn = (xaR - xaL);
+      Geo g3 = cell_geo<ND>(cxn, cdxn, d.ty[j], d.tdy[j], geom);
+      for (int q = 0; q < NQ; ++q) U[q] /= g3.dV;
+    }
+    double D = 1., S = 0., tau = 1.;
+    if (cell) c2p_inputs<ND, NQ>(U, cxn, geom, &D, &S, &tau);
+    double gs = guess(cell, D, S, tau, p[PPP], geom);
+    if (cell) cons2prim<ND, NQ>(p, U, cxn, geom, G, d.solver, gs, d.maxit, &st);
     ncalls = 2;
   }
+  if (!cell) return;
   store_v<ND, NQ>(d, d.prim, k, p);
   d.cx[k] = cxn;
   d.cdx[k] = cdxn;
   GH_ATOMIC_ADD(&d.cnt[CNT_C2P_CALLS], (unsigned long long)ncalls);
   if (st.iters) GH_ATOMIC_ADD(&d.cnt[CNT_C2P_ITERS], (unsigned long long)st.iters);
+  if (st.used) GH_ATOMIC_ADD(&d.cnt[CNT_MLP], (unsigned long long)st.used);
   if (st.fail & 1) GH_ATOMIC_ADD(&d.cnt[CNT_NOBRACKET], 1ull);
   if (st.fail & 2) GH_ATOMIC_ADD(&d.cnt[CNT_MAXITER], 1ull);
   if (st.fail & 4) GH_ATOMIC_ADD(&d.cnt[CNT_PHICAP], 1ull);
+}
+
+// Grid::update + Cell::update + RK3 averages (1d.cpp:798-841, 2d.cpp:974-1036,
+// cell.cpp:102-127, rk3.cpp:12-195).  One thread owns face i and cell i.
+template <int ND, int NQ>
+GH_HD void body_update(const Dev& d, int j, int i, int stage) {
+  body_update_g<ND, NQ>(d, j, i, stage, true, NoGuess{});
 }
 
 // ---------------------------------------------------------------------------
@@ -639,13 +680,14 @@ enum { RG_SKIP = 0, RG_MERGE = 1, RG_SPLIT = 2 };
 template <int ND, int NQ>
 GH_HD int rg_check(const Dev& d, int j, int i, double iS) {
   long long k = CI(d, j, i);
-  if (d.rg_mode == RG_AR2D) {
+  int mode = d.trg[j];
+  if (mode == RG_AR2D) {
     double ar = d.cdx[k] / d.tdy[j];
     if (ar > d.rg[0]) return RG_SPLIT;
     if (ar < d.rg[1]) return RG_MERGE;
     return RG_SKIP;
   }
-  if (d.rg_mode == RG_BM1D) {
+  if (mode == RG_BM1D) {
     int nt = d.ntrack[j], iR = nt - d.ngst;
     double dth = d.rg[2], target = d.rg[4];
     double rOut = d.cx[CI(d, j, iR - 1)];
@@ -757,7 +799,7 @@ GH_HD void rg_merge(const Dev& d, int j, int i) {
   d.cx[b + i] = xnew;
   d.cdx[b + i] = dxnew;
   Geo g2 = cell_geo<ND>(xnew, dxnew, d.ty[j], d.tdy[j], geom);
-  C2PStat st = {0, 0};
+  C2PStat st = {0, 0, 0, 0};
   cons2prim<ND, NQ>(pc, cc, ND == 1 ? g2.cen0 : xnew, geom, G, d.solver, -1., d.maxit, &st);
   store_v<ND, NQ>(d, d.prim, b + i, pc);
   if (left) {
@@ -778,14 +820,14 @@ GH_HD void body_rg_init(const Dev& d, int j, int i) {
   d.tiS[j] = d.ngst;        // iLbnd + 1
 }
 GH_HD void body_rg_shockpos(const Dev& d, int j, int i) {
-  if (!active_track(d, j)) return;
+  if (!active_track(d, j) || d.trg[j] != RG_BM1D) return;
   int nt = d.ntrack[j], iL = d.ngst - 1, iR = nt - d.ngst;
   if (i < iL + 1 || i > iR - 1) return;
   if (d.prim[PPP * d.cs + CI(d, j, i)] > 1.5 * d.rg[3]) GH_ATOMIC_MAX(&d.tiS[j], i);
 }
 template <int ND, int NQ>
 GH_HD void body_rg_screen(const Dev& d, int j, int i) {
-  if (!active_track(d, j)) return;
+  if (!active_track(d, j) || d.trg[j] == RG_NONE) return;
   int nt = d.ntrack[j], iL = d.ngst - 1, iR = nt - d.ngst;
   if (i < iL + 1 || i > iR - 1) return;
   int a = rg_check<ND, NQ>(d, j, i, (double)d.tiS[j]);
@@ -798,19 +840,20 @@ GH_HD void body_rg_apply(const Dev& d, int j, int i0) {
   if (i0 != 0 || !active_track(d, j)) return;
   int i = d.tfirst[j];
   if (i == INT_MAX) return;
-  double iS = d.rg_mode == RG_BM1D ? rg_shock_index(d, j) : 0.;
+  bool bm = d.trg[j] == RG_BM1D;
+  double iS = bm ? rg_shock_index(d, j) : 0.;
   for (; i <= d.ntrack[j] - d.ngst - 1; ++i) {
     int a = rg_check<ND, NQ>(d, j, i, iS);
     int nt = d.ntrack[j];
     if (a == RG_SPLIT && nt < d.nax - 1) {
       rg_split<ND, NQ>(d, j, i);
       d.nsplit[j]++;
-      if (d.rg_mode == RG_BM1D) iS = rg_shock_index(d, j);
+      if (bm) iS = rg_shock_index(d, j);
     }
     if (a == RG_MERGE && nt > 2 * d.ngst + 2) {
       rg_merge<ND, NQ>(d, j, i);
       d.nmerge[j]++;
-      if (d.rg_mode == RG_BM1D) iS = rg_shock_index(d, j);
+      if (bm) iS = rg_shock_index(d, j);
     }
   }
 }
@@ -1074,6 +1117,71 @@ __global__ void __launch_bounds__(64) k_mlp_infer(const float* __restrict__ w, c
   if (valid) out[s] = pressure_of(w, D, y);
 }
 
+// pointwise cons2prim of raw states (D, |S|, tau) from given starting
+// pressures; with guess2 != null each state starts from whichever of the two
+// guesses has the smaller residual |f| (used[s] = 1 when guess[s] won)
+__global__ void k_c2p_batch(const double* __restrict__ st, const double* __restrict__ guess,
+                            const double* __restrict__ guess2, long long n, double G, int solver, int maxit,
+                            double* __restrict__ p_out, int* __restrict__ iters, int* __restrict__ fail,
+                            int* __restrict__ used) {
+  long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  double D = st[3 * s], S = st[3 * s + 1], tau = st[3 * s + 2];
+  double prim[6] = {0., 0., 0., 0., 0., 0.}, cons[6] = {0., 0., 0., 0., 0., 0.};
+  // with a second start, guess2 plays the previous pressure and guess the challenger
+  prim[PPP] = guess2 ? guess2[s] : guess[s];
+  cons[DEN] = D; cons[SS1] = S; cons[TAU] = tau;
+  C2PStat cs = {0, 0, 0, 0};
+  cons2prim<1, 6>(prim, cons, 1., CART, G, solver, guess2 ? guess[s] : -1., maxit, &cs);
+  p_out[s] = prim[PPP];
+  iters[s] = cs.iters + cs.evals;
+  fail[s] = cs.fail;
+  if (used) used[s] = guess2 ? cs.used : 1;
+}
+
+// Update stage with the learned starting pressure.  The network runs per
+// wavefront (BF16 MFMA), and only in wavefronts where some cell's previous
+// pressure leaves a relative residual above d.mlp_thr; cons2prim then judges
+// the two starts by |f|.  Every thread of the grid calls the body (threads
+// past the track end with valid = false), as the network needs all 64 lanes.
+struct MlpGuess {
+  const float* w;
+  const bf16x8* w2b;
+  __bf16* lds;
+  float* rows;
+  double thr;
+  double G;
+  __device__ double operator()(bool want, double D, double S, double tau, double pprev, int geom) const {
+    bool hard = false;
+    if (want) {
+      double E = D + tau;
+      double pp = geom == SPH ? fmax(1.e-13, pprev) : pprev;
+      double Ep = E + pp;
+      double lf = 1. / sqrt(fabs(1. - S * S / (Ep * Ep)));
+      double f = E + pp - D * lf - (G * pp * lf * lf) / (G - 1.);
+      double sc = fabs(E) + fabs(pp) + fabs(D * lf) + fabs((G * pp * lf * lf) / (G - 1.));
+      hard = !(fabs(f) <= thr * sc);
+    }
+    if (__ballot(hard) == 0) return -1.;
+    float x[3];
+    if (want) features(w, D, S, tau, x);
+    else { x[0] = 0.f; x[1] = 0.f; x[2] = 0.f; }
+    float y = forward_wave(w, w2b, x, lds, rows);
+    return hard ? pressure_of(w, D, y) : -1.;
+  }
+};
+
+template <int ND, int NQ>
+__global__ void __launch_bounds__(KB) k_update_mlp(Dev d, int ni, int remap, int stage) {
+  __shared__ __bf16 lds[KB / 64][64 * H];
+  __shared__ float rows[KB / 64][64];
+  int i, j;
+  tile_of(remap, j, i);
+  int wv = threadIdx.x / 64;
+  MlpGuess g{d.mw, (const bf16x8*)d.mw2b, lds[wv], rows[wv], d.mlp_thr, d.P.gamma};
+  body_update_g<ND, NQ>(d, j, i, stage, i < ni, g);
+}
+
 // lane-layout probe of v_mfma_f32_32x32x16_bf16: D = A B for row-major A (32x16), B (16x32)
 __global__ void __launch_bounds__(64) k_mfma_probe(const float* A, const float* B, float* Dm) {
   int l = threadIdx.x;
@@ -1115,6 +1223,8 @@ struct gammahip_ctx {
   hipEvent_t ev0[16], ev1[16];
   bool ev_pending[16];
   hipGraphExec_t graph = nullptr;   // two steps (an even number of buffer swaps)
+  void* mlp_w = nullptr;            // learned-guess parameters (gammahip_mlp_load)
+  void* mlp_w2b = nullptr;
 #endif
 };
 
@@ -1277,15 +1387,20 @@ struct Pipe {
   static void update(gammahip_ctx* c, int stage) {
     Dev& d = c->d;
     tbeg(c, 0);
-    L2S(c, KSEL((body_update<ND, NQ>), (k_update<ND, NQ>)), d.nj, d.nax, stage);
+#ifdef GH_EMU
+    L2S(c, (body_update<ND, NQ>), d.nj, d.nax, stage);
+#else
+    if (d.mw) L2S(c, (k_update_mlp<ND, NQ>), d.nj, d.nax, stage);
+    else L2S(c, (k_update<ND, NQ>), d.nj, d.nax, stage);
+#endif
     tend(c, 0);
     std::swap(d.fx, d.fxn);
   }
   static void regrid(gammahip_ctx* c) {
     Dev& d = c->d;
-    if (d.rg_mode == RG_NONE) return;
+    if (!d.rg_mode) return;
     L2(c, KSEL(body_rg_init, k_rg_init), d.nj, 1);
-    if (d.rg_mode == RG_BM1D) L2(c, KSEL(body_rg_shockpos, k_rg_shockpos), d.nj, d.nax);
+    if (d.rg_mode & (1 << RG_BM1D)) L2(c, KSEL(body_rg_shockpos, k_rg_shockpos), d.nj, d.nax);
     L2(c, KSEL((body_rg_screen<ND, NQ>), (k_rg_screen<ND, NQ>)), d.nj, d.nax);
 #ifdef GH_EMU
     for (int j = 0; j < d.nj; ++j) body_rg_apply<ND, NQ>(d, j, 0);
@@ -1387,7 +1502,8 @@ int gammahip_create(const gammahip_config* cfg, gammahip_ctx** out) {
   if (cfg->ncell_mv < 8 || cfg->nmax < cfg->ncell_mv) return fail(c, "bad ncell_mv / nmax");
   if (cfg->ntracks_active < 1) return fail(c, "bad ntracks_active");
   if (cfg->nd == 2 && cfg->geom < 0) return fail(c, "2D needs a single geometry");
-  if (cfg->c2p_guess != 0) return fail(c, "learned guess not loaded in this context");
+  if (cfg->rg_mode < RG_NONE || cfg->rg_mode > RG_BM1D) return fail(c, "unknown rg_mode");
+  if (cfg->c2p_guess != 0) return fail(c, "c2p_guess must be 0 at creation; gammahip_mlp_load switches the learned guess on");
 #ifndef GH_EMU
   {
     int ndev = 0;
@@ -1411,8 +1527,9 @@ int gammahip_create(const gammahip_config* cfg, gammahip_ctx** out) {
   d.fcap = 4 * d.nax;
   d.shock = cfg->shock; d.sync = cfg->sync;
   d.solver = cfg->c2p_solver; d.guess = cfg->c2p_guess;
+  d.mw = nullptr; d.mw2b = nullptr; d.mlp_thr = 1e-3;
   d.maxit = cfg->c2p_maxiter > 0 ? cfg->c2p_maxiter : 1000;
-  d.rg_mode = cfg->rg_mode;
+  d.rg_mode = 0;            // set from the tracks' rules by gammahip_load
   for (int k = 0; k < 6; ++k) d.rg[k] = cfg->rg[k];
   d.P = Phys{cfg->gamma, cfg->cfl, cfg->vi, cfg->shock_thr, cfg->gamma_max_init,
              cfg->Nalpha, cfg->Nmp, cfg->Nme};
@@ -1421,7 +1538,7 @@ int gammahip_create(const gammahip_config* cfg, gammahip_ctx** out) {
   c->ntr = d.nj;
   size_t nj = d.nj, cs = d.cs, nqs = d.nqs;
   int e = 0;
-  e |= dalloc(c, &d.ntrack, nj); e |= dalloc(c, &d.tgeom, nj); e |= dalloc(c, &d.tkin, nj);
+  e |= dalloc(c, &d.ntrack, nj); e |= dalloc(c, &d.trg, nj); e |= dalloc(c, &d.tgeom, nj); e |= dalloc(c, &d.tkin, nj);
   e |= dalloc(c, &d.tbnd, nj); e |= dalloc(c, &d.tfirst, nj); e |= dalloc(c, &d.tiS, nj);
   e |= dalloc(c, &d.tvin, nj); e |= dalloc(c, &d.tvout, nj); e |= dalloc(c, &d.tbm, 4 * nj);
   e |= dalloc(c, &d.ty, nj); e |= dalloc(c, &d.tdy, nj); e |= dalloc(c, &d.tdt, nj);
@@ -1453,6 +1570,8 @@ int gammahip_destroy(gammahip_ctx* c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   drop_graph(c);
   for (void* p : c->allocs) (void)hipFree(p);
+  if (c->mlp_w) (void)hipFree(c->mlp_w);
+  if (c->mlp_w2b) (void)hipFree(c->mlp_w2b);
   if (c->stream) {
     for (int k = 0; k < 16; ++k) { hipEventDestroy(c->ev0[k]); hipEventDestroy(c->ev1[k]); }
     hipStreamDestroy(c->stream);
@@ -1473,19 +1592,25 @@ int gammahip_load(gammahip_ctx* c, const double* x, const double* dx, const doub
   int nj = d.nj, nax = d.nax, ng = d.ngst, nm = cf.ncell_mv, na = cf.ntracks_active;
   size_t cs = d.cs;
   // host images of the initial state (Cinit + mpi_distribute, 1d.cpp:956-969, 2d.cpp:1175-1195)
-  std::vector<int> ntrack(nj, nm + 2 * ng), tgeom(nj), tkin(nj), tbnd(nj);
+  std::vector<int> ntrack(nj, nm + 2 * ng), tgeom(nj), tkin(nj), tbnd(nj), trg(nj, RG_NONE);
   std::vector<double> tvin(nj), tvout(nj), tbm(4 * nj), ty(nj), tdy(nj), tt(nj, t0);
   std::vector<double> hp((size_t)d.nqs * cs, 0.), hx(cs, 0.), hdx(cs, 0.), hfx(cs, 0.), hdt(cs, 1.e15);
   auto trk = [&](int a) { return cf.nd == 2 ? a + ng : a; };
   for (int a = 0; a < na; ++a) {
     int j = trk(a);
-    const gammahip_track_hooks& h = hooks[cf.nd == 2 ? 0 : a];
+    const gammahip_track_hooks& h = hooks[a];
+    int nc = h.ncell > 0 ? h.ncell : nm;
+    if (nc < 2 || nc > nm) return fail(c, "track " + std::to_string(a) + ": ncell outside 2..ncell_mv");
+    if (h.rg_mode < -1 || h.rg_mode > RG_BM1D) return fail(c, "track " + std::to_string(a) + ": unknown rg_mode");
+    ntrack[j] = nc + 2 * ng;
+    trg[j] = h.rg_mode >= 0 ? h.rg_mode : cf.rg_mode;
+    d.rg_mode |= trg[j] != RG_NONE ? (1 << trg[j]) : 0;
     tgeom[j] = cf.geom >= 0 ? cf.geom : h.geom;
     tkin[j] = h.kin_mode; tvin[j] = h.v_in; tvout[j] = h.v_out; tbnd[j] = h.bnd_mode;
     for (int q = 0; q < 4; ++q) tbm[4 * j + q] = h.bm[q];
     ty[j] = cf.nd == 2 ? y[a] : 0.;
     tdy[j] = cf.nd == 2 ? dy[a] : 0.;
-    for (int ii = 0; ii < nm; ++ii) {
+    for (int ii = 0; ii < nc; ++ii) {
       size_t k = (size_t)j * nax + ii + ng, s = (size_t)a * nm + ii;
       hx[k] = x[s]; hdx[k] = dx[s];
       double pv[10];
@@ -1510,11 +1635,15 @@ int gammahip_load(gammahip_ctx* c, const double* x, const double* dx, const doub
       int js = d.jL + 1;
       ty[j] = ty[js] - (d.jL - j + 1) * dyL; tdy[j] = tdy[js];
       tgeom[j] = tgeom[js]; tkin[j] = tkin[js]; tvin[j] = tvin[js]; tvout[j] = tvout[js]; tbnd[j] = tbnd[js];
+      ntrack[j] = ntrack[js]; trg[j] = trg[js];
+      for (int q = 0; q < 4; ++q) tbm[4 * j + q] = tbm[4 * js + q];
     }
     for (int j = d.jR; j < nj; ++j) {
       int js = d.jR - 1;
       ty[j] = ty[js] + (j - d.jR + 1) * dyR; tdy[j] = tdy[js];
       tgeom[j] = tgeom[js]; tkin[j] = tkin[js]; tvin[j] = tvin[js]; tvout[j] = tvout[js]; tbnd[j] = tbnd[js];
+      ntrack[j] = ntrack[js]; trg[j] = trg[js];
+      for (int q = 0; q < 4; ++q) tbm[4 * j + q] = tbm[4 * js + q];
     }
   }
   // Grid::interfaceGeomFromCellPos (1d.cpp:844-856, 2d.cpp:1039-1058), active tracks
@@ -1528,6 +1657,7 @@ int gammahip_load(gammahip_ctx* c, const double* x, const double* dx, const doub
   }
   int e = 0;
   e |= h2d(c, d.ntrack, ntrack.data(), sizeof(int) * nj);
+  e |= h2d(c, d.trg, trg.data(), sizeof(int) * nj);
   e |= h2d(c, d.tgeom, tgeom.data(), sizeof(int) * nj);
   e |= h2d(c, d.tkin, tkin.data(), sizeof(int) * nj);
   e |= h2d(c, d.tbnd, tbnd.data(), sizeof(int) * nj);
@@ -1557,6 +1687,35 @@ int gammahip_load(gammahip_ctx* c, const double* x, const double* dx, const doub
   return post(c);
 }
 
+int gammahip_mlp_load(gammahip_ctx* c, const float* params) {
+  if (!c) return -1;
+  Dev& d = c->d;
+#ifdef GH_EMU
+  if (params) return fail(c, "the learned guess needs the HIP build");
+  return 0;
+#else
+  drop_graph(c);
+  if (!params) {
+    d.mw = nullptr; d.mw2b = nullptr; d.guess = 0;
+    return 0;
+  }
+  if (d.solver != gh::C2P_NEWTON) return fail(c, "the learned guess seeds the Newton solver (c2p_solver = 1)");
+  if (!c->mlp_w) {
+    HIPCHK(hipMalloc(&c->mlp_w, sizeof(float) * gh::mlp::NPARAM));
+    HIPCHK(hipMalloc(&c->mlp_w2b, sizeof(__bf16) * 8 * 64 * 8));
+  }
+  std::vector<__bf16> hb(8 * 64 * 8);
+  gh::mlp::pack_w2(params, hb.data());
+  HIPCHK(hipStreamSynchronize(c->stream));
+  HIPCHK(hipMemcpy(c->mlp_w, params, sizeof(float) * gh::mlp::NPARAM, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->mlp_w2b, hb.data(), sizeof(__bf16) * hb.size(), hipMemcpyHostToDevice));
+  d.mw = (const float*)c->mlp_w;
+  d.mw2b = c->mlp_w2b;
+  d.guess = 1;
+  return 0;
+#endif
+}
+
 int gammahip_set_option(gammahip_ctx* c, int key, long long value) {
   if (!c) return -1;
   drop_graph(c);
@@ -1566,6 +1725,9 @@ int gammahip_set_option(gammahip_ctx* c, int key, long long value) {
       c->block = (int)value; return 0;
     case GAMMAHIP_OPT_REMAP: c->remap = value ? 1 : 0; return 0;
     case GAMMAHIP_OPT_GRAPH: c->use_graph = value ? 1 : 0; return 0;
+    case GAMMAHIP_OPT_MLP_THR:
+      if (value < 0 || value > 16) return fail(c, "network threshold exponent must be in 0..16");
+      c->d.mlp_thr = pow(10., -(double)value); return 0;
     default: return fail(c, "unknown option");
   }
 }
@@ -1839,6 +2001,35 @@ int gammahip_mlp_infer(const float* params, const double* states, long long n, d
   hipLaunchKernelGGL(k_mlp_infer, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, 0, w, w2b, st, n, out, mfma);
   CK(hipGetLastError());
   CK(hipMemcpy(p_out, out, sizeof(double) * n, hipMemcpyDeviceToHost));
+  return 0;
+#endif
+}
+
+int gammahip_c2p_batch(const double* states, const double* guess, const double* guess2, long long n,
+                       double gamma, int solver, int maxit, double* p_out, int* iters, int* fail,
+                       int* used) {
+#ifdef GH_EMU
+  (void)states; (void)guess; (void)guess2; (void)n; (void)gamma; (void)solver; (void)maxit;
+  (void)p_out; (void)iters; (void)fail; (void)used;
+  return -1;
+#else
+  if (n < 1 || !(gamma > 1.) || (solver != gh::C2P_BRENT && solver != gh::C2P_NEWTON) || maxit < 1)
+    return plain_fail("bad c2p_batch arguments");
+  DevBuf B;
+  double *st = B.get<double>(3 * n), *g = B.get<double>(n), *g2 = guess2 ? B.get<double>(n) : nullptr;
+  double* p = B.get<double>(n);
+  int *it = B.get<int>(n), *fl = B.get<int>(n), *us = B.get<int>(n);
+  if (!st || !g || (guess2 && !g2) || !p || !it || !fl || !us) return plain_fail("allocation failed");
+  CK(hipMemcpy(st, states, sizeof(double) * 3 * n, hipMemcpyHostToDevice));
+  CK(hipMemcpy(g, guess, sizeof(double) * n, hipMemcpyHostToDevice));
+  if (guess2) CK(hipMemcpy(g2, guess2, sizeof(double) * n, hipMemcpyHostToDevice));
+  hipLaunchKernelGGL(k_c2p_batch, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, 0, st, g, g2, n, gamma,
+                     solver, maxit, p, it, fl, us);
+  CK(hipGetLastError());
+  CK(hipMemcpy(p_out, p, sizeof(double) * n, hipMemcpyDeviceToHost));
+  CK(hipMemcpy(iters, it, sizeof(int) * n, hipMemcpyDeviceToHost));
+  CK(hipMemcpy(fail, fl, sizeof(int) * n, hipMemcpyDeviceToHost));
+  if (used) CK(hipMemcpy(used, us, sizeof(int) * n, hipMemcpyDeviceToHost));
   return 0;
 #endif
 }

@@ -126,7 +126,10 @@ GH_HD double c2p_df(double p, double D, double S, double E, double G, int geom, 
 
 enum { C2P_BRENT = 0, C2P_NEWTON = 1 };
 
-struct C2PStat { int iters; int fail; };   // fail: 1 no bracket, 2 max iter, 4 p_hi cap
+// iters: root-finder iterations; fail: 1 no bracket, 2 max iter, 4 p_hi cap;
+// evals: residual evaluations outside the root finder (bracketing, judging
+// a second start); used: calls that started from the second start
+struct C2PStat { int iters; int fail; int evals; int used; };
 
 // Brent's method (Brent 1973, ch. 4) with the GSL fsolver contract used by
 // rel_sph.cpp:273-285: root = best iterate b, bracket = [min(b,c), max(b,c)],
@@ -213,9 +216,24 @@ GH_HD double newton(double D, double S, double E, double G, int geom, double lo,
   return x;
 }
 
+// (D, |S|, tau) of a conserved state as cons2prim below forms them (the
+// spherical 2D angular momentum is divided by r); inputs of the learned guess
+template <int ND, int NQ>
+GH_HD void c2p_inputs(const double* cons, double r, int geom, double* D, double* S, double* tau) {
+  double ss[2] = {cons[SS1], ND == 2 ? cons[SS2] : 0.};
+  if (ND == 2 && geom == SPH) ss[1] /= r;
+  double S2 = 0.;
+  for (int d = 0; d < ND; ++d) S2 += ss[d] * ss[d];
+  *D = cons[DEN];
+  *S = sqrt(S2);
+  *tau = cons[TAU];
+}
+
 // FluidState::cons2prim  rel_sph.cpp:213-320 / rel_cart.cpp:125-229.
-// prim[PPP] on entry is the previous pressure (the reference's guess);
-// guess_override > 0 replaces it as the starting point (learned guess).
+// prim[PPP] on entry is the previous pressure (the reference's guess).
+// guess_override > 0 (Newton only) is a second start, e.g. the learned guess:
+// the solve starts from whichever of the two has the smaller residual |f|, so
+// the previous pressure stays both the fallback and the judge.
 // cons on exit = prim2cons(prim) (rel_sph.cpp:318).
 template <int ND, int NQ>
 GH_HD void cons2prim(double* prim, double* cons, double r, int geom, double G, int solver,
@@ -228,18 +246,27 @@ GH_HD void cons2prim(double* prim, double* cons, double r, int geom, double G, i
   double S = sqrt(S2);
   double pprev = geom == SPH ? fmax(1.e-13, prim[PPP]) : prim[PPP];
   double pCand = pprev;
-  double start = guess_override > 0. ? guess_override : pCand;
+  double start = pCand;
+  if (guess_override > 0. && solver == C2P_NEWTON) {
+    st->evals += 2;
+    if (fabs(c2p_f(guess_override, D, S, E, G, geom)) < fabs(c2p_f(pCand, D, S, E, G, geom))) {
+      start = guess_override;
+      st->used++;
+    }
+  }
   double f_init = c2p_f(solver == C2P_BRENT ? pCand : start, D, S, E, G, geom);
   if (solver == C2P_NEWTON) pCand = start;
   if (fabs(f_init) > 0.) {
     double p_lo = fmax(0., (1. + 1e-13) * S - E), p_hi;
     bool find_p = true;
+    st->evals += 2;
     if (c2p_f(p_lo, D, S, E, G, geom) < 0) find_p = false;
     if (f_init < 0) p_hi = 1. * pCand;
     else {
       p_hi = pCand;
       int guard = 0;
       while (c2p_f(p_hi, D, S, E, G, geom) > 0) {
+        st->evals++;
         p_hi *= 10;
         if (++guard > 400) { find_p = false; st->fail |= 4; break; }
       }

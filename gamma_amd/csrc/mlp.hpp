@@ -1,7 +1,8 @@
 // mlp.hpp -- learned initial pressure for cons2prim: a 3 -> 64 -> 64 -> 1 MLP.
 //
 // Inputs (log D, |S|/D, tau/D) of a conserved state enter through fixed maps
-// (log D, log1p(|S|/D), log(tau/D)) and a per-feature affine normalisation;
+// (log D, log1p(|S|/D), log of the non-kinetic part of tau/D; see features())
+// and a per-feature affine normalisation;
 // the output is the normalised log(p/D), so log p = log D + y/out_scale +
 // out_mean (the map p/D <-> (|S|/D, tau/D) is scale free).  Hidden layers use
 // ReLU.  Parameters are one float array (layout below); inference runs the
@@ -24,11 +25,19 @@ constexpr int ONORM = OB3 + 1;         // in_mean[3], in_scale[3], out_mean, out
 constexpr int NPARAM = ONORM + 8;
 constexpr int NTRAIN = ONORM;          // trained entries (the normalisation is fixed)
 
-// features of a conserved state (D, |S|, tau), normalised
+// features of a conserved state (D, |S|, tau), normalised: log D, log1p(q)
+// and log e with q = |S|/D, t = tau/D and e = t + 1 - sqrt(1 + q^2), the part
+// of tau/D that is not bulk kinetic energy.  For a cold gas e ~ (k - 1)(p/rho)/W
+// with k = G/(G-1) and W the Lorentz factor, so log e carries log p where t and
+// q alone hold it only in digits an FP32 input cannot keep.  e is formed in
+// FP64 and floored at the rounding level of t.
 GH_HD void features(const float* w, double D, double S, double tau, float* x) {
   double Dp = D > 1e-300 ? D : 1e-300;
-  double t = tau / Dp;
-  double f0 = log(Dp), f1 = log1p(S / Dp), f2 = log(t > 1e-300 ? t : 1e-300);
+  double q = S / Dp, t = tau / Dp;
+  double e = t + 1. - sqrt(1. + q * q);
+  double fl = 1e-16 * (1. + fabs(t));
+  if (!(e > fl)) e = fl;
+  double f0 = log(Dp), f1 = log1p(q), f2 = log(e);
   x[0] = (float)((f0 - w[ONORM + 0]) * w[ONORM + 3]);
   x[1] = (float)((f1 - w[ONORM + 1]) * w[ONORM + 4]);
   x[2] = (float)((f2 - w[ONORM + 2]) * w[ONORM + 5]);

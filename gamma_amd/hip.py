@@ -24,7 +24,7 @@ F_GRADMV0, F_GRADF10 = 300, 400
 FACE_X, FACE_V, FACE_LL, FACE_LR, FACE_FLUX0 = 1000, 1001, 1002, 1003, 1010
 RESET_SHOCK, DETECT_SHOCK, INJECT, INIT_GAMMAS, ADVANCE_TIME = range(5)
 C2P_BRENT, C2P_NEWTON = 0, 1
-OPT_BLOCK, OPT_REMAP, OPT_GRAPH = 0, 1, 2
+OPT_BLOCK, OPT_REMAP, OPT_GRAPH, OPT_MLP_THR = 0, 1, 2, 3
 
 
 class Config(C.Structure):
@@ -43,7 +43,7 @@ class Config(C.Structure):
 class TrackHooks(C.Structure):
     _fields_ = [
         ("geom", C.c_int), ("kin_mode", C.c_int), ("v_in", C.c_double), ("v_out", C.c_double),
-        ("bnd_mode", C.c_int), ("bm", C.c_double * 4),
+        ("bnd_mode", C.c_int), ("bm", C.c_double * 4), ("rg_mode", C.c_int), ("ncell", C.c_int),
     ]
 
 
@@ -84,6 +84,8 @@ def load_library(path: str = LIB_PATH):
         "gammahip_mlp_grad": (I, [P, P, I, P, P]),
         "gammahip_mlp_infer": (I, [P, P, C.c_longlong, P, I]),
         "gammahip_mfma_probe": (I, [P, P, P]),
+        "gammahip_c2p_batch": (I, [P, P, P, C.c_longlong, D, I, I, P, P, P, P]),
+        "gammahip_mlp_load": (I, [P, P]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -118,13 +120,26 @@ def make_config(rc: S.RunConfig, ntracks_active: int, solver=C2P_BRENT, guess=0)
     return c
 
 
-def make_hooks(rc: S.RunConfig) -> TrackHooks:
+def make_hooks(rc: S.RunConfig, ncell: int = 0, over: dict | None = None) -> TrackHooks:
+    """Hooks of one track: the run config's, with per-track overrides."""
+    o = dict(geom=rc.geom, kin_mode=rc.kin_mode, v_in=rc.v_in, v_out=rc.v_out,
+             bnd_mode=rc.bnd_mode, bm=rc.bm, rg_mode=-1)
+    o.update(over or {})
     h = TrackHooks()
-    h.geom, h.kin_mode, h.v_in, h.v_out = rc.geom, rc.kin_mode, rc.v_in, rc.v_out
-    h.bnd_mode = rc.bnd_mode
+    h.geom, h.kin_mode, h.v_in, h.v_out = o["geom"], o["kin_mode"], o["v_in"], o["v_out"]
+    h.bnd_mode, h.rg_mode, h.ncell = o["bnd_mode"], o["rg_mode"], int(ncell)
     for k in range(4):
-        h.bm[k] = rc.bm[k]
+        h.bm[k] = o["bm"][k]
     return h
+
+
+def setup_hooks(setup: S.Setup) -> list:
+    """One hooks entry per active track of a setup."""
+    rc = setup.cfg
+    na = rc.ncell_f1 if rc.nd == 2 else setup.x.shape[0]
+    nc = setup.ncells if setup.ncells is not None else [0] * na
+    ov = setup.hooks if setup.hooks is not None else [None] * na
+    return [make_hooks(rc, nc[a], ov[a]) for a in range(na)]
 
 
 class Grid:
@@ -149,7 +164,7 @@ class Grid:
             self.h = None
             raise RuntimeError("gammahip_create: " + msg)
         if hooks_per_track is None:
-            hooks_per_track = [make_hooks(rc)] * (na if rc.nd == 1 else 1)
+            hooks_per_track = setup_hooks(setup)
         arr = (TrackHooks * len(hooks_per_track))(*hooks_per_track)
         self._keep = [np.ascontiguousarray(a, dtype=np.float64)
                       for a in (setup.x, setup.dx, setup.y, setup.dy, setup.prim)]
@@ -198,11 +213,21 @@ class Grid:
     def sync(self):
         self._check(self.L.gammahip_sync(self.h))
 
-    def set_options(self, block=None, remap=None, graph=None):
-        """Launch options (threads per workgroup, XCD-aware tile order, hipGraph replay)."""
-        for key, val in ((OPT_BLOCK, block), (OPT_REMAP, remap), (OPT_GRAPH, graph)):
+    def set_options(self, block=None, remap=None, graph=None, mlp_thr=None):
+        """Launch options (threads per workgroup, XCD-aware tile order, hipGraph
+        replay, trigger exponent n of the learned guess: threshold 10^-n)."""
+        for key, val in ((OPT_BLOCK, block), (OPT_REMAP, remap), (OPT_GRAPH, graph),
+                         (OPT_MLP_THR, mlp_thr)):
             if val is not None:
                 self._check(self.L.gammahip_set_option(self.h, key, int(val)))
+
+    def load_surrogate(self, params):
+        """Seed the Newton cons2prim with the learned guess (None switches it off)."""
+        if params is None:
+            self._check(self.L.gammahip_mlp_load(self.h, None))
+            return
+        self._mlp = np.ascontiguousarray(params, dtype=np.float32)
+        self._check(self.L.gammahip_mlp_load(self.h, _p(self._mlp)))
 
     # ---- state access ----
     def cells(self, field: int) -> np.ndarray:

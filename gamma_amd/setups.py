@@ -97,6 +97,12 @@ class Setup:
     dy: np.ndarray
     prim: np.ndarray
     meta: dict = field(default_factory=dict)
+    # initial active cells per track (None: ncell_mv for every track); the
+    # arrays above stay [track][ncell_mv], read up to each count
+    ncells: np.ndarray | None = None
+    # per-track hook overrides for ensembles of 1D runs: dicts with keys of
+    # RunConfig (geom, kin_mode, v_in, v_out, bnd_mode, bm, rg_mode)
+    hooks: list | None = None
 
 
 # ---------------------------------------------------------------------------
@@ -169,14 +175,17 @@ def bm_profile(r, t, lfac_shock2_0, t0, rho_a, eta, gamma):
 
 
 def blandford_mckee_1d(n: int = 4000, lfac_shock=100., eta=1.e-10, gamma=4. / 3.,
-                       nmax=None, sync=1, shock=1, dth=5.e-4) -> Setup:
+                       nmax=None, sync=1, shock=1, dth=5.e-4, t0=None) -> Setup:
     """Config: 1D spherical BM blast wave, shock Lorentz factor 100, ambient
     rho = 1, p = eta rho c^2, cells geometrically spaced on [0.5, 1.5] R_sh
-    with r_{i+1/2} = 0.5 * 3^(i/n).  Boundary faces as BM1D.cpp:239-244
+    with r_{i+1/2} = 0.5 R_sh 3^(i/n).  Boundary faces as BM1D.cpp:239-244
     (v_in = 0, v_out = 1.5), inner ghosts from the analytic solution
-    (BM1D.cpp:254-274), AMR rule of BM1D.cpp:277-319."""
+    (BM1D.cpp:254-274), AMR rule of BM1D.cpp:277-319.  The start time puts
+    the shock at R_sh = 1 (BM1D.cpp:42) unless t0 is given."""
     L0 = lfac_shock * lfac_shock
-    t0 = 1. / (1. - 1. / (1. + 8. * L0))          # R_shock(t0) = 1 (BM1D.cpp:42)
+    if t0 is None:
+        t0 = 1. / (1. - 1. / (1. + 8. * L0))      # R_shock(t0) = 1 (BM1D.cpp:42)
+    rshock = t0 * (1. - 1. / (1. + 8. * L0))
     rho_norm, v_norm, l_norm = MP, C_LIGHT, C_LIGHT  # BM1D.cpp:20-23
     na, nmp, nme = normalized_constants(rho_norm, v_norm, l_norm)
     cfg = RunConfig(nd=1, geom=GEOM_SPH, shock=shock, sync=sync, nmax=nmax or int(n * 1.25),
@@ -185,11 +194,10 @@ def blandford_mckee_1d(n: int = 4000, lfac_shock=100., eta=1.e-10, gamma=4. / 3.
                     bnd_mode=BND_BM, bm=(L0, t0, 1., eta, 0., 0.),
                     rg_mode=RG_BM1D, rg=(3., 0.1, dth, eta, 1., 0.))
     i = np.arange(n + 1)
-    edges = 0.5 * np.power(3., i / n)
+    edges = rshock * 0.5 * np.power(3., i / n)
     x = 0.5 * (edges[1:] + edges[:-1])
     dx = edges[1:] - edges[:-1]
     prim = np.zeros((cfg.nq, 1, n))
-    rshock = 1.
     inside = x <= rshock
     rho_b, v_b, p_b = bm_profile(np.minimum(x, rshock * 0.999999), t0, L0, t0, 1., eta, gamma)
     prim[RHO, 0] = np.where(inside, rho_b, 1.)
@@ -198,6 +206,131 @@ def blandford_mckee_1d(n: int = 4000, lfac_shock=100., eta=1.e-10, gamma=4. / 3.
     prim[TR1, 0] = np.where(inside, 2., 1.)
     return Setup("blandford_mckee_1d", cfg, t0, x[None, :], dx[None, :],
                  np.zeros(1), np.ones(1), prim, meta=dict(L0=L0, eta=eta))
+
+
+def jet_lfac(theta):
+    """Angular Lorentz-factor profile of the 2D blast wave: 100 exp(-(theta/0.1)^2) + 2."""
+    return 100. * np.exp(-(np.asarray(theta) / 0.1) ** 2) + 2.
+
+
+def bm_jet_2d(ntheta: int = 1024, theta_max: float = 0.5, n_lo: int = 500, n_hi: int = 4000,
+              eta=1.e-10, gamma=4. / 3., nmax=None, sync=1, shock=1, t0=1.) -> Setup:
+    """Config: 2D spherical (r, theta) relativistic blast wave whose shock
+    Lorentz factor follows jet_lfac(theta), on ntheta theta tracks over
+    [0, theta_max].  Each track starts as the BM solution of its own Lorentz
+    factor at the common time t0 (BM1D.cpp:62-86 per track), on cells spaced
+    geometrically over [0.5, 1.5] R_shock(theta) as in the 1D case; its length
+    is seeded between n_lo (Lorentz factor 2) and n_hi (102) in proportion to
+    the Lorentz factor, so tracks are uneven from the start.  Inner boundary:
+    each track's own BM inflow (BM1D.cpp:254-274); boundary faces v_in = 0,
+    v_out = 1.5; the per-track AMR rule of BM1D.cpp:277-319 with dtheta the
+    track width; shock detection (threshold 0.01) and local synchrotron
+    particles (gamma_max 1e8) on."""
+    dth = theta_max / ntheta
+    th = (np.arange(ntheta) + 0.5) * dth
+    lf = jet_lfac(th)
+    n = np.rint(n_lo + (n_hi - n_lo) * (lf - 2.) / 100.).astype(np.int64)
+    nmv = int(n.max())
+    rho_norm, v_norm, l_norm = MP, C_LIGHT, C_LIGHT
+    na, nmp, nme = normalized_constants(rho_norm, v_norm, l_norm)
+    cfg = RunConfig(nd=2, geom=GEOM_SPH, shock=shock, sync=sync, nmax=nmax or int(nmv * 1.25),
+                    ncell_mv=nmv, ncell_f1=ntheta, gamma=gamma, n_alpha=na, n_mp=nmp, n_me=nme,
+                    kin_mode=KIN_FIXED, v_in=0., v_out=1.5, bnd_mode=BND_BM,
+                    bm=(lf[0] ** 2, t0, 1., eta, 0., 0.),
+                    rg_mode=RG_BM1D, rg=(3., 0.1, dth, eta, 1., 0.))
+    x = np.zeros((ntheta, nmv))
+    dx = np.ones((ntheta, nmv))
+    prim = np.zeros((cfg.nq, ntheta, nmv))
+    prim[RHO] = 1.
+    prim[PPP] = eta
+    prim[TR1] = 1.
+    hooks = []
+    for a in range(ntheta):
+        L0 = lf[a] ** 2
+        rsh = t0 * (1. - 1. / (1. + 8. * L0))
+        m = int(n[a])
+        edges = rsh * 0.5 * np.power(3., np.arange(m + 1) / m)
+        xa = 0.5 * (edges[1:] + edges[:-1])
+        x[a, :m] = xa
+        dx[a, :m] = edges[1:] - edges[:-1]
+        x[a, m:] = xa[-1]
+        inside = xa <= rsh
+        rb, vb, pb = bm_profile(np.minimum(xa, rsh * 0.999999), t0, L0, t0, 1., eta, gamma)
+        prim[RHO, a, :m] = np.where(inside, rb, 1.)
+        prim[UU1, a, :m] = np.where(inside, vb, 0.)
+        prim[PPP, a, :m] = np.where(inside, pb, eta)
+        prim[TR1, a, :m] = np.where(inside, 2., 1.)
+        hooks.append(dict(bm=(L0, t0, 1., eta)))
+    return Setup("bm_jet_2d", cfg, t0, x, dx, th, np.full(ntheta, dth), prim,
+                 meta=dict(theta_max=theta_max, lengths="seeded"), ncells=n, hooks=hooks)
+
+
+def lorentz_tube_1d(n: int, lfac: float, gamma=4. / 3., nmax=None, shock=1, rho=1., p=1.e-4,
+                    t0=1.) -> Setup:
+    """Planar relativistic shock tube: gas with Lorentz factor lfac flowing
+    right (x < 0.5) into gas at rest (x > 0.5), both rho = 1, p = 1e-4, on
+    [0, 1] with n cells; no kinematics, boundary or regrid hooks (1DST.cpp)."""
+    cfg = RunConfig(nd=1, geom=GEOM_CART, shock=shock, nmax=nmax or int(n * 1.25), ncell_mv=n,
+                    gamma=gamma)
+    dxc = 1. / n
+    x = (np.arange(n) + 0.5) * dxc
+    prim = np.zeros((cfg.nq, 1, n))
+    left = x <= 0.5
+    prim[RHO, 0] = rho
+    prim[PPP, 0] = p
+    prim[UU1, 0] = np.where(left, math.sqrt(1. - 1. / (lfac * lfac)), 0.)
+    prim[TR1, 0] = np.where(left, 1., 2.)
+    return Setup("lorentz_tube_1d", cfg, t0, x[None, :], np.full((1, n), dxc), np.zeros(1),
+                 np.ones(1), prim, meta=dict(lfac=lfac))
+
+
+def ensemble_1d(n_runs: int = 256, sizes=(512, 16384), lfacs=(2., 300.), seed=DEFAULT_SEED,
+                eta=1.e-10, dth=5.e-4):
+    """Config: n_runs independent 1D relativistic runs advanced together, one
+    track each: even members are spherical Blandford-McKee blast waves, odd
+    members planar Lorentz shock tubes.  Sizes and Lorentz factors are drawn
+    log-uniform from the given ranges with SplitMix64 (counters 2m and 2m+1
+    of member m).  All members share gamma = 4/3, CFL 0.2, shock detection
+    and the start time t0 = 1, and no synchrotron; each keeps its own
+    geometry, kinematics, boundary, regrid rule and time step.
+
+    Returns (ensemble, members): the ensemble Setup (one track per member,
+    per-track hooks and lengths) and the member Setups, each runnable alone
+    with the same capacity (nmax), so that member m of the ensemble must
+    match member m run by itself."""
+    u = splitmix_uniform(seed, np.arange(2 * n_runs, dtype=np.uint64))
+    u = 0.5 * (u + 1.)                                        # [0, 1)
+    lo, hi = np.log(sizes[0]), np.log(sizes[1])
+    ns = np.rint(np.exp(lo + (hi - lo) * u[0::2])).astype(np.int64)
+    lf = np.exp(np.log(lfacs[0]) + (np.log(lfacs[1]) - np.log(lfacs[0])) * u[1::2])
+    nmax = int(ns.max() * 1.25)
+    members = []
+    for m in range(n_runs):
+        if m % 2 == 0:
+            members.append(blandford_mckee_1d(int(ns[m]), lfac_shock=float(lf[m]), eta=eta, nmax=nmax,
+                                              sync=0, shock=1, dth=dth, t0=1.))
+        else:
+            members.append(lorentz_tube_1d(int(ns[m]), float(lf[m]), nmax=nmax, shock=1))
+    nmv = int(ns.max())
+    bm_cfg = members[0].cfg
+    cfg = RunConfig(nd=1, geom=-1, shock=1, sync=0, nmax=nmax, ncell_mv=nmv, gamma=4. / 3.,
+                    n_alpha=bm_cfg.n_alpha, n_mp=bm_cfg.n_mp, n_me=bm_cfg.n_me,
+                    rg_mode=RG_NONE, rg=bm_cfg.rg)
+    x = np.zeros((n_runs, nmv))
+    dx = np.ones((n_runs, nmv))
+    prim = np.zeros((cfg.nq, n_runs, nmv))
+    hooks = []
+    for m, ms in enumerate(members):
+        k = int(ns[m])
+        x[m, :k], dx[m, :k] = ms.x[0], ms.dx[0]
+        x[m, k:] = ms.x[0, -1]
+        prim[:, m, :k] = ms.prim[:, 0, :]
+        c = ms.cfg
+        hooks.append(dict(geom=c.geom, kin_mode=c.kin_mode, v_in=c.v_in, v_out=c.v_out,
+                          bnd_mode=c.bnd_mode, bm=c.bm, rg_mode=c.rg_mode))
+    ens = Setup("ensemble_1d", cfg, 1., x, dx, np.zeros(n_runs), np.ones(n_runs), prim,
+                meta=dict(seed=seed, sizes=ns.tolist(), lfacs=lf.tolist()), ncells=ns, hooks=hooks)
+    return ens, members
 
 
 def riemann_2d(nx: int = 2048, ny: int = 2048, rho_l=10., p_l=13.33, rho_r=1.,

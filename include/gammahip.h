@@ -43,20 +43,26 @@ typedef struct gammahip_config {
   double gamma, cfl, vi, shock_thr, gamma_max_init;
   double Nalpha, Nmp, Nme;               /* normalizeConstants (constants.cpp:12-20)  */
   int    rg_mode;     /* checkCellForRegrid: 0 none, 1 dx/dy aspect ratio (2DST.cpp:  */
-                      /* 304-322), 2 Blandford-McKee rule (BM1D.cpp:277-319)          */
+                      /* 304-322), 2 Blandford-McKee rule (BM1D.cpp:277-319); the     */
+                      /* rule of tracks whose hooks.rg_mode is -1                     */
   double rg[6];       /* rule parameters {split, merge, dth, eta, target, -}          */
   int    c2p_solver;  /* 0 Brent (reference algorithm), 1 safeguarded Newton          */
   int    c2p_guess;   /* 0 previous pressure (reference), 1 learned MLP guess         */
   int    c2p_maxiter; /* 1000 in the reference                                        */
 } gammahip_config;
 
-/* Per-track hook parameters (userKinematics / userBoundaries of the setup). */
+/* Per-track hook parameters (userKinematics / userBoundaries /
+ * checkCellForRegrid of the setup).  Per-track rules and initial lengths let
+ * one context advance an ensemble of independent 1D runs of different setups
+ * and sizes, or start a 2D grid from uneven tracks. */
 typedef struct gammahip_track_hooks {
   int    geom;        /* used when config.geom == -1                                  */
   int    kin_mode;    /* 0 none, 1 fixed boundary-face speeds (BM1D.cpp:239-244)      */
   double v_in, v_out;
   int    bnd_mode;    /* 0 outflow only, 1 Blandford-McKee inflow (BM1D.cpp:254-274)  */
   double bm[4];       /* {lfacShock2(t0), t0, rho_ambient, eta}                       */
+  int    rg_mode;     /* this track's checkCellForRegrid rule; -1: config.rg_mode    */
+  int    ncell;       /* initial active cells of this track (2..ncell_mv); 0: ncell_mv */
 } gammahip_track_hooks;
 
 typedef struct gammahip_stats {
@@ -76,8 +82,9 @@ const char* gammahip_last_error(gammahip_ctx* ctx);
  * (grid.cpp:20-26) + apply(radiation_initGammas) (simu.cpp:34-36) + the first
  * Grid::updateGhosts of Simu::run (simu.cpp:43).  x,dx: [ntracks_active][ncell_mv];
  * y,dy: [ntracks_active] (F1 centre and width, 2D); prim: [nq][ntracks_active]
- * [ncell_mv] with 3-velocities in the UU slots; hooks: [ntracks_active];
- * t0: start time.  Synchronous. */
+ * [ncell_mv] with 3-velocities in the UU slots (track a read up to its
+ * hooks[a].ncell); hooks: [ntracks_active] in 1D and 2D; t0: start time.
+ * Synchronous. */
 int  gammahip_load(gammahip_ctx* ctx, const double* x, const double* dx, const double* y,
                    const double* dy, const double* prim, const gammahip_track_hooks* hooks,
                    double t0);
@@ -117,9 +124,13 @@ int  gammahip_sync(gammahip_ctx* ctx);
 
 /* Launch options (no counterpart in the reference; chosen by the autotuner in
  * gamma_amd/autotune.py): threads per workgroup of the per-cell kernels (64,
- * 128, 256), XCD-aware ordering of the (track, chunk) tiles, and replay of
- * two-step hipGraphs by gammahip_step. */
-enum { GAMMAHIP_OPT_BLOCK = 0, GAMMAHIP_OPT_REMAP = 1, GAMMAHIP_OPT_GRAPH = 2 };
+ * 128, 256), XCD-aware ordering of the (track, chunk) tiles, replay of
+ * two-step hipGraphs by gammahip_step, and n for the learned guess's trigger:
+ * a wavefront runs the network when one of its cells' previous pressures
+ * leaves a residual |f| above 10^-n times the magnitude of f's terms
+ * (default 3). */
+enum { GAMMAHIP_OPT_BLOCK = 0, GAMMAHIP_OPT_REMAP = 1, GAMMAHIP_OPT_GRAPH = 2,
+       GAMMAHIP_OPT_MLP_THR = 3 };
 int  gammahip_set_option(gammahip_ctx* ctx, int key, long long value);
 
 /* ---- learned cons2prim guess (no counterpart in the reference) ----
@@ -137,6 +148,21 @@ int  gammahip_mlp_grad(const float* params, const double* states, int n, float* 
 int  gammahip_mlp_infer(const float* params, const double* states, long long n, double* p_out,
                         int mfma);
 int  gammahip_mfma_probe(const float* A /* 32x16 */, const float* B /* 16x32 */, float* D /* 32x32 */);
+/* Cartesian cons2prim of raw states [n][3] = (D, |S|, tau) started from
+ * guess[n] (FluidState::cons2prim, src/Hydro/rel_cart.cpp:125-229, with the
+ * previous pressure replaced by the given start).  guess2 (nullable): a second
+ * start; each state then starts from the one with the smaller |f| and
+ * used[s] = 1 when guess[s] won.  iters[s] counts residual evaluations
+ * (root-finder iterations plus bracketing), fail[s] the failure bits
+ * (1 no bracket, 2 iteration cap, 4 bracket-growth cap). */
+/* Seed the context's Newton cons2prim with the network (params = null
+ * switches it off).  The previous pressure stays the judge: each solve starts
+ * from whichever start has the smaller residual; stats.mlp_used counts the
+ * solves that started from the network. */
+int  gammahip_mlp_load(gammahip_ctx* ctx, const float* params);
+int  gammahip_c2p_batch(const double* states, const double* guess, const double* guess2, long long n,
+                        double gamma, int solver, int maxit, double* p_out, int* iters, int* fail,
+                        int* used);
 
 #ifdef __cplusplus
 }

@@ -72,6 +72,7 @@ struct og_grid {
   long it;
   long cnt[8];
   long *nsplit, *nmerge;
+  double *tbm;   /* [nj][4] BM boundary parameters of each track (config bm by default) */
   int err;
 };
 
@@ -669,8 +670,9 @@ static void user_kinematics(og_grid *g) {
 
 /* BM analytic profile in code units (c = 1, k = 0): BM1D.cpp:62-86.
  * bm[0] = lfacShock2 at t0, bm[1] = t0, bm[2] = rho_ambient, bm[3] = eta (p/rho). */
-static void bm_profile(const og_config *c, double r, double t, double *rho, double *u, double *p) {
-  double L0 = c->bm[0], t0 = c->bm[1], rhoa = c->bm[2], eta = c->bm[3];
+static void bm_profile(const og_config *c, const double *bm, double r, double t, double *rho, double *u,
+                       double *p) {
+  double L0 = bm[0], t0 = bm[1], rhoa = bm[2], eta = bm[3];
   double G = c->gamma;
   double tr = t0 / t;
   double lfacShock2 = L0 * tr * tr * tr;
@@ -694,7 +696,7 @@ static void user_boundaries(og_grid *g) {
     for (int i = 0; i <= g->iLbnd[j]; ++i) {
       Cell *cl = CELL(g, j, i);
       double rho, u, p;
-      bm_profile(&g->c, cl->G.x[0], g->t, &rho, &u, &p);
+      bm_profile(&g->c, g->tbm + 4 * j, cl->G.x[0], g->t, &rho, &u, &p);
       cl->S.prim[OG_RHO] = rho;
       cl->S.prim[OG_UU1] = u;
       cl->S.prim[OG_UU2] = 0;
@@ -1258,7 +1260,9 @@ og_grid *og_create(const og_config *cfg) {
   g->iRbnd = (int *)calloc(g->nj, sizeof(int));
   g->nsplit = (long *)calloc(g->nj, sizeof(long));
   g->nmerge = (long *)calloc(g->nj, sizeof(long));
+  g->tbm = (double *)calloc((size_t)4 * g->nj, sizeof(double));
   for (int j = 0; j < g->nj; ++j) {
+    for (int q = 0; q < 4; ++q) g->tbm[4 * j + q] = cfg->bm[q];
     g->nact[j] = cfg->ncell_mv;
     g->ntrack[j] = cfg->ncell_mv + 2 * ngst;
     g->iLbnd[j] = ngst - 1;
@@ -1276,7 +1280,45 @@ og_grid *og_create(const og_config *cfg) {
 void og_destroy(og_grid *g) {
   if (!g) return;
   free(g->C); free(g->I); free(g->nact); free(g->ntrack); free(g->iLbnd); free(g->iRbnd);
-  free(g->nsplit); free(g->nmerge); free(g);
+  free(g->nsplit); free(g->nmerge); free(g->tbm); free(g);
+}
+
+/* Initial active cells per track (Grid::prepForRun gives every track
+ * ncell[MV], 1d.cpp / 2d.cpp; uneven starts are what the reference's own
+ * regrid produces from even ones).  Call between og_create and og_load; the
+ * arrays given to og_load stay [track][ncell_mv], read up to each count. */
+int og_set_track_cells(og_grid *g, const int *ncells) {
+  const og_config *c = &g->c;
+  int ngst = c->ngst, nf = c->nd == 2 ? c->ncell_f1 : 1;
+  for (int jj = 0; jj < nf; ++jj)
+    if (ncells[jj] < 2 || ncells[jj] > c->ncell_mv || ncells[jj] > c->nmax) return -1;
+  for (int jj = 0; jj < nf; ++jj) {
+    int j = c->nd == 2 ? jj + ngst : 0;
+    g->nact[j] = ncells[jj];
+    g->ntrack[j] = ncells[jj] + 2 * ngst;
+    g->iRbnd[j] = ncells[jj] + ngst;
+  }
+  if (c->nd == 2)
+    for (int j = 0; j < g->nj; ++j) {
+      int js = j < ngst ? ngst : (j >= g->nj - ngst ? g->nj - ngst - 1 : j);
+      g->nact[j] = g->nact[js]; g->ntrack[j] = g->ntrack[js]; g->iRbnd[j] = g->iRbnd[js];
+    }
+  return 0;
+}
+
+/* Per-track parameters of the BM inner boundary (the setup's userBoundaries
+ * with a different blast wave on each track): bm [ntracks_active][4]; ghost
+ * tracks take their edge track's. */
+int og_set_track_bm(og_grid *g, const double *bm) {
+  const og_config *c = &g->c;
+  int ngst = c->ngst, nf = c->nd == 2 ? c->ncell_f1 : 1;
+  for (int j = 0; j < g->nj; ++j) {
+    int jj = c->nd == 2 ? j - ngst : 0;
+    if (jj < 0) jj = 0;
+    if (jj > nf - 1) jj = nf - 1;
+    for (int q = 0; q < 4; ++q) g->tbm[4 * j + q] = bm[4 * jj + q];
+  }
+  return 0;
 }
 
 int og_load(og_grid *g, const double *x, const double *dx, const double *y,
@@ -1285,7 +1327,7 @@ int og_load(og_grid *g, const double *x, const double *dx, const double *y,
   int ngst = c->ngst, nm = c->ncell_mv, nf = c->nd == 2 ? c->ncell_f1 : 1;
   for (int jj = 0; jj < nf; ++jj) {
     int j = c->nd == 2 ? jj + ngst : 0;
-    for (int ii = 0; ii < nm; ++ii) {
+    for (int ii = 0; ii < g->nact[j]; ++ii) {
       Cell *cl = CELL(g, j, ii + ngst);
       size_t k = (size_t)jj * nm + ii;
       cl->G.x[MV] = x[k]; cl->G.dx[MV] = dx[k];

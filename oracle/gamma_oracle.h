@@ -64,6 +64,8 @@ void     og_destroy(og_grid *g);
 /* Load the initial active-cell data (what Grid::initialGeometry/initialValues
  * write into Cinit): x,dx [ncell_f1][ncell_mv] in MV, y,dy [ncell_f1] in F1,
  * prim [nq][ncell_f1][ncell_mv] with 3-velocities in the UU slots. */
+int      og_set_track_cells(og_grid *g, const int *ncells);
+int      og_set_track_bm(og_grid *g, const double *bm);
 int      og_load(og_grid *g, const double *x, const double *dx,
                  const double *y, const double *dy, const double *prim);
 /* prepForRun + radiation_initGammas + the first updateGhosts of Simu::run */

@@ -64,6 +64,7 @@ def lib():
         sig = {
             "og_create": (P, [cfgp]), "og_destroy": (V, [P]),
             "og_load": (I, [P, P, P, P, P, P]), "og_init": (I, [P, D]),
+            "og_set_track_cells": (I, [P, P]), "og_set_track_bm": (I, [P, P]),
             "og_run": (I, [P, I]), "og_ntracks": (I, [P]), "og_nax": (I, [P]),
             "og_track_info": (V, [P, P, P, P, P]), "og_time": (D, [P]),
             "og_iter": (C.c_long, [P]), "og_last_dt": (D, [P]),
@@ -109,6 +110,13 @@ class OracleGrid:
         y = np.ascontiguousarray(setup.y, dtype=np.float64)
         dy = np.ascontiguousarray(setup.dy, dtype=np.float64)
         prim = np.ascontiguousarray(setup.prim, dtype=np.float64)
+        if setup.ncells is not None:
+            nc = np.ascontiguousarray(setup.ncells, dtype=np.int32)
+            if lib().og_set_track_cells(self.h, _p(nc)) != 0:
+                raise ValueError("bad per-track cell counts")
+        if setup.hooks is not None:
+            bm = np.array([list(h.get("bm", self.rc.bm))[:4] for h in setup.hooks], dtype=np.float64)
+            lib().og_set_track_bm(self.h, _p(bm))
         lib().og_load(self.h, _p(x), _p(dx), _p(y), _p(dy), _p(prim))
         lib().og_init(self.h, setup.t0)
         self.nj = lib().og_ntracks(self.h)

@@ -26,6 +26,9 @@ CASES = {
     "riemann_2d": (lambda: setups.riemann_2d(64, 16), 60),
     # edge cells grow, split next to the ghosts, which must follow their faces
     "expanding_tube": (lambda: setups.expanding_tube_1d(200), 300),
+    # 2D spherical blast wave with an angular Lorentz-factor profile: uneven
+    # seeded track lengths, per-track BM inflow, shock detection, synchrotron
+    "bm_jet_2d": (lambda: setups.bm_jet_2d(12, 0.3, 40, 160), 40),
 }
 
 
@@ -130,3 +133,65 @@ def test_gpu_parity_newton(case):
     st = g.stats()
     print(f"[gpu parity newton] {case}: {res} iters/call="
           f"{st['c2p_iters'] / max(st['c2p_calls'], 1):.3f}", flush=True)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", ["shock_tube_cart", "blandford_mckee", "riemann_2d"])
+def test_gpu_parity_newton_surrogate(case):
+    """Newton seeded by the learned guess (network forced on in every
+    wavefront) converges to the same bar; the previous pressure still judges."""
+    from gamma_amd import surrogate as SG
+    mk, n = CASES[case]
+    s = mk()
+    print(f"[gpu parity surrogate] case={case}", flush=True)
+    w, _ = SG.train(SG.synthetic_states(1 << 16, s.cfg.gamma, seed=4), steps=600, batch=2048, seed=4)
+    o = oc.OracleGrid(s)
+    o.run(n)
+    g = hip.Grid(s, solver=hip.C2P_NEWTON)
+    g.load_surrogate(w)
+    g.set_options(mlp_thr=16)
+    g.step(n)
+    res = check_newton(o, g, reference_spread(s, n))
+    st = g.stats()
+    assert st["mlp_used"] > 0, st
+    print(f"[gpu parity surrogate] {case}: {res} iters/call="
+          f"{st['c2p_iters'] / max(st['c2p_calls'], 1):.3f} network starts={st['mlp_used']}"
+          f" of {st['c2p_calls']}", flush=True)
+
+
+def _ensemble_check(lib_path, n_runs=6, sizes=(64, 256), nsteps=40):
+    """Every member of a 1D ensemble advanced in one context (per-member
+    geometry, hooks, regrid rule, length and time step) matches the same
+    member run alone through the oracle."""
+    ens, members = setups.ensemble_1d(n_runs, sizes=sizes)
+    g = hip.Grid(ens, lib_path=lib_path)
+    g.step(nsteps)
+    tg, _ = g.time()
+    act = [g.active(f) for f in (0, 1, 2)]
+    shk = g.active(hip.F_SHOCKED)
+    sg, mg = g.regrid_counts()
+    out = []
+    for m, s in enumerate(members):
+        o = oc.OracleGrid(s)
+        o.run(nsteps)
+        assert g.tracks()["ntrack"][m] == o.tracks()["ntrack"][0], f"member {m}: length differs"
+        so, mo = o.regrid_counts()
+        assert (sg[m], mg[m]) == (so[0], mo[0]), f"member {m}: regrid counts differ"
+        errs = [l1rel([act[f][m]], [o.active(f)[0]]) for f in (0, 1, 2)]
+        assert max(errs) <= TOL, (m, s.name, errs)
+        assert np.array_equal(shk[m], o.active(oc.F_SHOCKED)[0]), f"member {m}: shock flags differ"
+        assert abs(tg[m] - o.t) <= 1e-12 * abs(o.t), (m, tg[m], o.t)
+        out.append((s.name, errs, all(np.array_equal(act[f][m], o.active(f)[0]) for f in (0, 1, 2))))
+    return out
+
+
+def test_emu_parity_ensemble(emu_lib):
+    res = _ensemble_check(emu_lib)
+    assert all(bit for _, _, bit in res), res
+
+
+@pytest.mark.gpu
+def test_gpu_parity_ensemble():
+    print("[gpu parity] case=ensemble_1d", flush=True)
+    res = _ensemble_check(hip.LIB_PATH)
+    print(f"[gpu parity] ensemble_1d: {res}", flush=True)
