@@ -100,6 +100,58 @@ def run_form(setup, solver, options, steps, warmup, timing_steps, surrogate=None
             "stats": st, "ms": ms, "launches": nl, "cells": n1}
 
 
+def track_cells(g):
+    """Active cells summed over the active tracks (the metric's unit count)."""
+    tr = g.tracks()
+    rc = g.rc
+    nact = tr["nact"]
+    return int(nact[rc.ngst:-rc.ngst].sum()) if rc.nd == 2 else int(nact.sum())
+
+
+def run_other(setup, steps, warmup, options=None, solver=None):
+    """cell-updates/s of a secondary configuration (no per-kernel timing)."""
+    from gamma_amd import hip
+    g = hip.Grid(setup, solver=hip.C2P_BRENT if solver is None else solver)
+    if options:
+        g.set_options(**options)
+    g.step(warmup)
+    g.sync()
+    n0 = track_cells(g)
+    t0 = time.perf_counter()
+    g.step(steps)
+    g.sync()
+    el = time.perf_counter() - t0
+    n1 = track_cells(g)
+    st = g.stats()
+    lens = g.tracks()["ntrack"]
+    g.close()
+    return {"value": 0.5 * (n0 + n1) * steps / el, "ms_per_step": 1e3 * el / steps, "cells": n1,
+            "track_cells_min": int(lens.min()) - 4, "track_cells_max": int(lens.max()) - 4,
+            "c2p_iters_per_call": st["c2p_iters"] / max(st["c2p_calls"], 1),
+            "failures": {k: st[k] for k in ("c2p_nobracket", "c2p_maxiter", "nan_flux", "negative_dt")},
+            "splits": st["splits"], "merges": st["merges"]}
+
+
+def other_configs(steps, warmup):
+    """The uneven-track configurations: the 2D spherical jet blast wave (1024
+    theta tracks, seeded lengths 500-4000) and the ensemble of 256 1D runs
+    (512-16384 cells), each plain and batched (persistent packed launch,
+    XCD-aware tiles, hipGraph replay, Newton cons2prim)."""
+    from gamma_amd import hip, setups
+    out = {}
+    batched = dict(block=256, remap=1, graph=1, persist=8)
+    for name, mk in (("bm_jet_2d", lambda: setups.bm_jet_2d()),
+                     ("ensemble_1d", lambda: setups.ensemble_1d()[0])):
+        s = mk()
+        print(f"[bench] {name}: plain", file=sys.stderr, flush=True)
+        p = run_other(s, steps, warmup)
+        print(f"[bench] {name}: batched", file=sys.stderr, flush=True)
+        b = run_other(s, steps, warmup, batched, hip.C2P_NEWTON)
+        out[name] = {"plain": p, "batched": dict(b, options=batched),
+                     "speedup": b["value"] / p["value"]}
+    return out
+
+
 def kernel_table(res, nq):
     f1faces = 2.0 * res["cells"]
     kern = {}
@@ -122,6 +174,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--timing-steps", type=int, default=2)
+    ap.add_argument("--no-other", action="store_true",
+                    help="skip the uneven-track configurations")
+    ap.add_argument("--other-steps", type=int, default=100)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; multi-GPU sharding is a "
@@ -173,6 +228,7 @@ def main():
                                "final_loss": float(loss[-100:].mean()), "train_s": round(train_s, 2)},
                    "kernels": kernel_table(b, nq), "autotune": tuned["table"]}
 
+    other = None if args.no_other else other_configs(args.other_steps, 3)
     cpu = None if args.no_cpu_baseline else cpu_baseline()
     st = plain["stats"]
     out = {
@@ -190,6 +246,7 @@ def main():
                                         "f1_overflow")},
         "regrid": {"splits": st["splits"], "merges": st["merges"]},
         "batched": batched,
+        "other_configs": other,
     }
     print(json.dumps(out))
 

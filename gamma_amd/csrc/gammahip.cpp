@@ -71,6 +71,11 @@ struct Dev {
   const float* mw;
   const void* mw2b;
   double mlp_thr;          // relative residual of the previous pressure that asks for the network
+  // packed launch: (track, chunk) tiles covering only the live cells of every
+  // track, walked by a persistent grid (0: one workgroup per chunk of nax)
+  int packed;
+  int* tiles;              // [cap][2] = (track, chunk)
+  int* ntiles;             // [1]
 };
 
 GH_HD long long CI(const Dev& d, int j, int i) { return (long long)j * d.nax + i; }
@@ -899,23 +904,88 @@ __device__ __forceinline__ void tile_of(int remap, int& j, int& i) {
     tile_of(remap, j, i);                                                        \
     if (i < ni && j < d.nj) BODY(d, j, i, stage);                                \
   }
+// Packed form: a persistent grid walks the tile list built by k_tiles.  With
+// remap the list is cut into 8 contiguous bands, band x walked by the
+// workgroups that the dispatcher places on XCD x (linear id % 8), so adjacent
+// tracks -- F1 neighbours -- share an L2.  Grid sizes are multiples of 8.
+// The trip count depends on blockIdx only, so every lane of a wavefront runs
+// the same iterations.
+#define GH_PACKED_LOOP(...)                                                      \
+  if (d.packed) {                                                                \
+    int nt_ = *d.ntiles, G_ = gridDim.x, t_, end_, st_;                          \
+    if (remap) {                                                                 \
+      int x_ = blockIdx.x & 7;                                                   \
+      t_ = (int)((long long)nt_ * x_ / 8) + (int)(blockIdx.x >> 3);              \
+      end_ = (int)((long long)nt_ * (x_ + 1) / 8);                               \
+      st_ = G_ >> 3;                                                             \
+    } else {                                                                     \
+      t_ = blockIdx.x; end_ = nt_; st_ = G_;                                     \
+    }                                                                            \
+    for (; t_ < end_; t_ += st_) {                                               \
+      int j = d.tiles[2 * t_], i = d.tiles[2 * t_ + 1] * (int)blockDim.x + (int)threadIdx.x; \
+      __VA_ARGS__;                                                               \
+    }                                                                            \
+    return;                                                                      \
+  }
+#define GH_KERNEL2P(NAME, BODY)                                                 \
+  __global__ void __launch_bounds__(KB) NAME(Dev d, int ni, int remap) {        \
+    GH_PACKED_LOOP(if (i < ni) BODY(d, j, i))                                    \
+    int i, j;                                                                    \
+    tile_of(remap, j, i);                                                        \
+    if (i < ni && j < d.nj) BODY(d, j, i);                                       \
+  }
+#define GH_KERNEL2SP(NAME, BODY)                                                \
+  __global__ void __launch_bounds__(KB) NAME(Dev d, int ni, int remap, int stage) { \
+    GH_PACKED_LOOP(if (i < ni) BODY(d, j, i, stage))                             \
+    int i, j;                                                                    \
+    tile_of(remap, j, i);                                                        \
+    if (i < ni && j < d.nj) BODY(d, j, i, stage);                                \
+  }
 namespace {
 GH_KERNEL2(k_ghost_tracks, body_ghost_tracks)
-GH_KERNEL2(k_dt, body_dt)
-GH_KERNEL2(k_nb, body_nb)
-GH_KERNEL2(k_f1_shock, body_f1_shock)
+GH_KERNEL2P(k_dt, body_dt)
+GH_KERNEL2P(k_nb, body_nb)
+GH_KERNEL2P(k_f1_shock, body_f1_shock)
 GH_KERNEL2(k_shock_reset, body_shock_reset)
 GH_KERNEL2(k_shock_detect, body_shock_detect)
 GH_KERNEL2(k_rg_init, body_rg_init)
-GH_KERNEL2(k_rg_shockpos, body_rg_shockpos)
+GH_KERNEL2P(k_rg_shockpos, body_rg_shockpos)
 GH_KERNEL2(k_advance_time, body_advance_time)
 template <int ND, int NQ> GH_KERNEL2(k_ghost_mv, (body_ghost_mv<ND, NQ>))
-template <int ND, int NQ> GH_KERNEL2(k_faces, (body_faces<ND, NQ>))
-template <int NQ> GH_KERNEL2(k_f1faces, body_f1faces<NQ>)
-template <int ND, int NQ> GH_KERNEL2S(k_update, (body_update<ND, NQ>))
-template <int ND, int NQ> GH_KERNEL2(k_inject, (body_inject<ND, NQ>))
+template <int ND, int NQ> GH_KERNEL2P(k_faces, (body_faces<ND, NQ>))
+template <int NQ> GH_KERNEL2P(k_f1faces, body_f1faces<NQ>)
+template <int ND, int NQ> GH_KERNEL2SP(k_update, (body_update<ND, NQ>))
+template <int ND, int NQ> GH_KERNEL2P(k_inject, (body_inject<ND, NQ>))
 template <int ND, int NQ> GH_KERNEL2(k_init_gammas, (body_init_gammas<ND, NQ>))
-template <int ND, int NQ> GH_KERNEL2(k_rg_screen, (body_rg_screen<ND, NQ>))
+template <int ND, int NQ> GH_KERNEL2P(k_rg_screen, (body_rg_screen<ND, NQ>))
+
+// Tile list of the packed launch: ceil(ntrack / bs) chunks of every track, in
+// track order (one workgroup; block-wide scan of the per-thread counts).
+__global__ void __launch_bounds__(1024) k_tiles(Dev d, int bs) {
+  __shared__ int part[1024];
+  int tid = threadIdx.x;
+  int per = (d.nj + 1023) / 1024;
+  int j0 = tid * per, j1 = min(j0 + per, d.nj);
+  int sum = 0;
+  for (int j = j0; j < j1; ++j) sum += (d.ntrack[j] + bs - 1) / bs;
+  part[tid] = sum;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {
+    int v = tid >= off ? part[tid - off] : 0;
+    __syncthreads();
+    part[tid] += v;
+    __syncthreads();
+  }
+  int t = part[tid] - sum;
+  for (int j = j0; j < j1; ++j) {
+    int n = (d.ntrack[j] + bs - 1) / bs;
+    for (int c = 0; c < n; ++c, ++t) {
+      d.tiles[2 * t] = j;
+      d.tiles[2 * t + 1] = c;
+    }
+  }
+  if (tid == 1023) d.ntiles[0] = part[1023];
+}
 
 // per-track minimum of dt_loc with reset (1d.cpp:779-795, 2d.cpp:954-971)
 __global__ void __launch_bounds__(KB) k_dt_track_min(Dev d) {
@@ -1175,10 +1245,11 @@ template <int ND, int NQ>
 __global__ void __launch_bounds__(KB) k_update_mlp(Dev d, int ni, int remap, int stage) {
   __shared__ __bf16 lds[KB / 64][64 * H];
   __shared__ float rows[KB / 64][64];
-  int i, j;
-  tile_of(remap, j, i);
   int wv = threadIdx.x / 64;
   MlpGuess g{d.mw, (const bf16x8*)d.mw2b, lds[wv], rows[wv], d.mlp_thr, d.P.gamma};
+  GH_PACKED_LOOP(body_update_g<ND, NQ>(d, j, i, stage, i < ni, g))
+  int i, j;
+  tile_of(remap, j, i);
   body_update_g<ND, NQ>(d, j, i, stage, i < ni, g);
 }
 
@@ -1218,6 +1289,8 @@ struct gammahip_ctx {
   int block = 256;         // threads per workgroup of the per-cell kernels (64..256)
   int remap = 0;           // XCD-aware tile order
   int use_graph = 0;       // replay steps from a captured hipGraph
+  int persist = 0;         // packed launch: persistent workgroups per CU (0: off)
+  int ncu = 256;           // compute units of the device
 #ifndef GH_EMU
   hipStream_t stream = nullptr;
   hipEvent_t ev0[16], ev1[16];
@@ -1285,6 +1358,8 @@ int d2h(gammahip_ctx* c, void* dst, const void* src, size_t bytes) {
 #define L2S(c, body, ntr, ni, st)                              \
   for (int j_ = 0; j_ < (ntr); ++j_)                           \
     for (int i_ = 0; i_ < (ni); ++i_) body((c)->d, j_, i_, st)
+#define L2P L2
+#define L2SP L2S
 #else
 #define L2(c, kern, ntr, ni)                                                         \
   do {                                                                               \
@@ -1298,6 +1373,19 @@ int d2h(gammahip_ctx* c, void* dst, const void* src, size_t bytes) {
     dim3 g_(((ni) + bs_ - 1) / bs_, (ntr));                                          \
     hipLaunchKernelGGL(kern, g_, dim3(bs_), 0, (c)->stream, (c)->d, (int)(ni), (c)->remap, (int)(st)); \
   } while (0)
+// kernels with a packed form: a persistent grid of c->persist workgroups per CU
+#define L2P(c, kern, ntr, ni)                                                        \
+  do {                                                                               \
+    int bs_ = (c)->block;                                                            \
+    dim3 g_ = (c)->d.packed ? dim3((c)->persist * (c)->ncu) : dim3(((ni) + bs_ - 1) / bs_, (ntr)); \
+    hipLaunchKernelGGL(kern, g_, dim3(bs_), 0, (c)->stream, (c)->d, (int)(ni), (c)->remap); \
+  } while (0)
+#define L2SP(c, kern, ntr, ni, st)                                                   \
+  do {                                                                               \
+    int bs_ = (c)->block;                                                            \
+    dim3 g_ = (c)->d.packed ? dim3((c)->persist * (c)->ncu) : dim3(((ni) + bs_ - 1) / bs_, (ntr)); \
+    hipLaunchKernelGGL(kern, g_, dim3(bs_), 0, (c)->stream, (c)->d, (int)(ni), (c)->remap, (int)(st)); \
+  } while (0)
 #endif
 
 #ifdef GH_EMU
@@ -1305,6 +1393,15 @@ int d2h(gammahip_ctx* c, void* dst, const void* src, size_t bytes) {
 #else
 #define KSEL(emu, gpu) gpu
 #endif
+
+// tile list of the packed launch (after any change of track lengths or block size)
+void build_tiles(gammahip_ctx* c) {
+#ifndef GH_EMU
+  if (c->d.packed) hipLaunchKernelGGL(k_tiles, dim3(1), dim3(1024), 0, c->stream, c->d, c->block);
+#else
+  (void)c;
+#endif
+}
 
 void tbeg(gammahip_ctx* c, int id) {
 #ifndef GH_EMU
@@ -1330,19 +1427,20 @@ void tend(gammahip_ctx* c, int id) {
 
 template <int ND, int NQ>
 struct Pipe {
-  static void ghosts(gammahip_ctx* c) {
+  static void ghosts(gammahip_ctx* c, bool retile = false) {
     Dev& d = c->d;
     if (ND == 2) L2(c, KSEL(body_ghost_tracks, k_ghost_tracks), 2 * d.ngst, d.nax);
     L2(c, KSEL((body_ghost_mv<ND, NQ>), (k_ghost_mv<ND, NQ>)), d.nj, 2 * d.ngst);
+    if (retile) build_tiles(c);
   }
   static void prep(gammahip_ctx* c) {
     Dev& d = c->d;
     tbeg(c, 1);
-    L2(c, KSEL((body_faces<ND, NQ>), (k_faces<ND, NQ>)), d.nj, d.nax);
+    L2P(c, KSEL((body_faces<ND, NQ>), (k_faces<ND, NQ>)), d.nj, d.nax);
     tend(c, 1);
-    L2(c, KSEL(body_dt, k_dt), d.nj, d.nax);
+    L2P(c, KSEL(body_dt, k_dt), d.nj, d.nax);
     if (ND == 2) {
-      L2(c, KSEL(body_nb, k_nb), d.nj, d.nax);
+      L2P(c, KSEL(body_nb, k_nb), d.nj, d.nax);
 #ifdef GH_EMU
       for (int j = 0; j < d.nj; ++j) {
         int base = 0, nt = d.ntrack[j];
@@ -1358,9 +1456,9 @@ struct Pipe {
       hipLaunchKernelGGL(k_f1_scan, dim3(d.nj), dim3(KB), 0, c->stream, d);
 #endif
       tbeg(c, 2);
-      L2(c, KSEL(body_f1faces<NQ>, k_f1faces<NQ>), d.nj, d.nax);
+      L2P(c, KSEL(body_f1faces<NQ>, k_f1faces<NQ>), d.nj, d.nax);
       tend(c, 2);
-      if (d.shock) L2(c, KSEL(body_f1_shock, k_f1_shock), d.nj, d.nax);
+      if (d.shock) L2P(c, KSEL(body_f1_shock, k_f1_shock), d.nj, d.nax);
       std::swap(d.gf1, d.gf1n);
     }
   }
@@ -1390,8 +1488,8 @@ struct Pipe {
 #ifdef GH_EMU
     L2S(c, (body_update<ND, NQ>), d.nj, d.nax, stage);
 #else
-    if (d.mw) L2S(c, (k_update_mlp<ND, NQ>), d.nj, d.nax, stage);
-    else L2S(c, (k_update<ND, NQ>), d.nj, d.nax, stage);
+    if (d.mw) L2SP(c, (k_update_mlp<ND, NQ>), d.nj, d.nax, stage);
+    else L2SP(c, (k_update<ND, NQ>), d.nj, d.nax, stage);
 #endif
     tend(c, 0);
     std::swap(d.fx, d.fxn);
@@ -1400,8 +1498,8 @@ struct Pipe {
     Dev& d = c->d;
     if (!d.rg_mode) return;
     L2(c, KSEL(body_rg_init, k_rg_init), d.nj, 1);
-    if (d.rg_mode & (1 << RG_BM1D)) L2(c, KSEL(body_rg_shockpos, k_rg_shockpos), d.nj, d.nax);
-    L2(c, KSEL((body_rg_screen<ND, NQ>), (k_rg_screen<ND, NQ>)), d.nj, d.nax);
+    if (d.rg_mode & (1 << RG_BM1D)) L2P(c, KSEL(body_rg_shockpos, k_rg_shockpos), d.nj, d.nax);
+    L2P(c, KSEL((body_rg_screen<ND, NQ>), (k_rg_screen<ND, NQ>)), d.nj, d.nax);
 #ifdef GH_EMU
     for (int j = 0; j < d.nj; ++j) body_rg_apply<ND, NQ>(d, j, 0);
 #else
@@ -1413,7 +1511,7 @@ struct Pipe {
     switch (what) {
       case GAMMAHIP_RESET_SHOCK: if (d.shock) L2(c, KSEL(body_shock_reset, k_shock_reset), d.nj, d.nax); break;
       case GAMMAHIP_DETECT_SHOCK: if (d.shock) L2(c, KSEL(body_shock_detect, k_shock_detect), d.nj, d.nax); break;
-      case GAMMAHIP_INJECT: if (NQ == 10 && d.sync) L2(c, KSEL((body_inject<ND, NQ>), (k_inject<ND, NQ>)), d.nj, d.nax); break;
+      case GAMMAHIP_INJECT: if (NQ == 10 && d.sync) L2P(c, KSEL((body_inject<ND, NQ>), (k_inject<ND, NQ>)), d.nj, d.nax); break;
       case GAMMAHIP_INIT_GAMMAS: if (NQ == 10 && d.sync) L2(c, KSEL((body_init_gammas<ND, NQ>), (k_init_gammas<ND, NQ>)), d.nj, d.nax); break;
       case GAMMAHIP_ADVANCE_TIME: L2(c, KSEL(body_advance_time, k_advance_time), d.nj, 1); break;
       default: break;
@@ -1429,7 +1527,7 @@ struct Pipe {
   static void step(gammahip_ctx* c) {
     regrid(c);
     apply(c, GAMMAHIP_RESET_SHOCK);
-    ghosts(c);
+    ghosts(c, true);          // track lengths settle here: rebuild the tile list
     prep(c);
     apply(c, GAMMAHIP_DETECT_SHOCK);
     collect_dt(c);
@@ -1510,6 +1608,10 @@ int gammahip_create(const gammahip_config* cfg, gammahip_ctx** out) {
     hipError_t e = hipGetDeviceCount(&ndev);
     if (e != hipSuccess || ndev < 1) return fail(c, "no HIP device");
     if (hipStreamCreate(&c->stream) != hipSuccess) return fail(c, "stream creation failed");
+    int dev = 0, ncu = 0;
+    if (hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && ncu > 0)
+      c->ncu = (ncu + 7) / 8 * 8;     // persistent grids are multiples of 8 (one band per XCD)
     for (int k = 0; k < 16; ++k) {
       hipEventCreate(&c->ev0[k]);
       hipEventCreate(&c->ev1[k]);
@@ -1538,6 +1640,7 @@ int gammahip_create(const gammahip_config* cfg, gammahip_ctx** out) {
   c->ntr = d.nj;
   size_t nj = d.nj, cs = d.cs, nqs = d.nqs;
   int e = 0;
+  e |= dalloc(c, &d.tiles, (size_t)2 * nj * ((d.nax + 63) / 64)); e |= dalloc(c, &d.ntiles, 1);
   e |= dalloc(c, &d.ntrack, nj); e |= dalloc(c, &d.trg, nj); e |= dalloc(c, &d.tgeom, nj); e |= dalloc(c, &d.tkin, nj);
   e |= dalloc(c, &d.tbnd, nj); e |= dalloc(c, &d.tfirst, nj); e |= dalloc(c, &d.tiS, nj);
   e |= dalloc(c, &d.tvin, nj); e |= dalloc(c, &d.tvout, nj); e |= dalloc(c, &d.tbm, 4 * nj);
@@ -1681,7 +1784,7 @@ int gammahip_load(gammahip_ctx* c, const double* x, const double* dx, const doub
   // radiation_initGammas (simu.cpp:34-36), then the first updateGhosts (simu.cpp:43)
   int r = dispatch(c, [&](auto P) {
     P.apply(c, GAMMAHIP_INIT_GAMMAS);
-    P.ghosts(c);
+    P.ghosts(c, true);
   });
   if (r) return r;
   return post(c);
@@ -1722,7 +1825,15 @@ int gammahip_set_option(gammahip_ctx* c, int key, long long value) {
   switch (key) {
     case GAMMAHIP_OPT_BLOCK:
       if (value != 64 && value != 128 && value != 256) return fail(c, "block must be 64, 128 or 256");
-      c->block = (int)value; return 0;
+      c->block = (int)value;
+      build_tiles(c);
+      return post(c);
+    case GAMMAHIP_OPT_PERSIST:
+      if (value < 0 || value > 32) return fail(c, "persistent workgroups per CU must be in 0..32");
+      c->persist = (int)value;
+      c->d.packed = value > 0;
+      build_tiles(c);
+      return post(c);
     case GAMMAHIP_OPT_REMAP: c->remap = value ? 1 : 0; return 0;
     case GAMMAHIP_OPT_GRAPH: c->use_graph = value ? 1 : 0; return 0;
     case GAMMAHIP_OPT_MLP_THR:
@@ -1763,7 +1874,7 @@ int gammahip_step(gammahip_ctx* c, int nsteps) {
 }
 
 int gammahip_regrid(gammahip_ctx* c) { drop_graph(c); return dispatch(c, [&](auto P) { P.regrid(c); }) || post(c); }
-int gammahip_update_ghosts(gammahip_ctx* c) { drop_graph(c); return dispatch(c, [&](auto P) { P.ghosts(c); }) || post(c); }
+int gammahip_update_ghosts(gammahip_ctx* c) { drop_graph(c); return dispatch(c, [&](auto P) { P.ghosts(c, true); }) || post(c); }
 int gammahip_prep_for_update(gammahip_ctx* c) { drop_graph(c); return dispatch(c, [&](auto P) { P.prep(c); }) || post(c); }
 int gammahip_evolve(gammahip_ctx* c) { drop_graph(c); return dispatch(c, [&](auto P) { P.evolve(c); }) || post(c); }
 int gammahip_apply(gammahip_ctx* c, int what) { drop_graph(c); return dispatch(c, [&](auto P) { P.apply(c, what); }) || post(c); }

@@ -128,9 +128,12 @@ int  gammahip_sync(gammahip_ctx* ctx);
  * two-step hipGraphs by gammahip_step, and n for the learned guess's trigger:
  * a wavefront runs the network when one of its cells' previous pressures
  * leaves a residual |f| above 10^-n times the magnitude of f's terms
- * (default 3). */
+ * (default 3), and the packed launch: n > 0 runs the per-cell kernels as
+ * persistent grids of n workgroups per CU walking a list of (track, chunk)
+ * tiles that covers only each track's live cells (uneven tracks, ensembles),
+ * 0 launches one workgroup per chunk of the nmax-wide track slots. */
 enum { GAMMAHIP_OPT_BLOCK = 0, GAMMAHIP_OPT_REMAP = 1, GAMMAHIP_OPT_GRAPH = 2,
-       GAMMAHIP_OPT_MLP_THR = 3 };
+       GAMMAHIP_OPT_MLP_THR = 3, GAMMAHIP_OPT_PERSIST = 4 };
 int  gammahip_set_option(gammahip_ctx* ctx, int key, long long value);
 
 /* ---- learned cons2prim guess (no counterpart in the reference) ----

@@ -195,3 +195,25 @@ def test_gpu_parity_ensemble():
     print("[gpu parity] case=ensemble_1d", flush=True)
     res = _ensemble_check(hip.LIB_PATH)
     print(f"[gpu parity] ensemble_1d: {res}", flush=True)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", ["shock_tube_cart", "riemann_2d", "bm_jet_2d"])
+@pytest.mark.parametrize("opts", [
+    dict(block=128, remap=1, graph=1, persist=0),
+    dict(block=256, remap=0, graph=0, persist=4),
+    dict(block=64, remap=1, graph=1, persist=8),
+], ids=["remap-graph", "packed", "packed-remap-graph"])
+def test_gpu_parity_launch_options(case, opts):
+    """Launch options change where work runs, never the arithmetic: the
+    trajectory under every option matches the oracle as the default launch does."""
+    mk, n = CASES[case]
+    s = mk()
+    print(f"[gpu parity options] case={case} opts={opts}", flush=True)
+    o = oc.OracleGrid(s)
+    o.run(n)
+    g = hip.Grid(s)
+    g.set_options(**opts)
+    g.step(n)
+    res = check_pair(o, g, shock=bool(s.cfg.shock))
+    print(f"[gpu parity options] {case} {opts}: {res}", flush=True)
