@@ -522,7 +522,7 @@ struct NoGuess {
   GH_HD double operator()(bool, double, double, double, double, int) const { return -1.; }
 };
 
-template <int ND, int NQ, class GuessT>
+template <int ND, int NQ, int SV, class GuessT>
 GH_HD void body_update_g(const Dev& d, int j, int i, int stage, bool valid, const GuessT& guess) {
   bool run = valid && active_track(d, j);
   int nt = run ? d.ntrack[j] : 0;
@@ -605,7 +605,7 @@ GH_HD void body_update_g(const Dev& d, int j, int i, int stage, bool valid, cons
     double D = 1., S = 0., tau = 1.;
     if (cell) c2p_inputs<ND, NQ>(U, r1, geom, &D, &S, &tau);
     double gs = guess(cell, D, S, tau, p[PPP], geom);
-    if (cell) cons2prim<ND, NQ>(p, U, r1, geom, G, d.solver, gs, d.maxit, &st);
+    if (cell) cons2prim<ND, NQ>(p, U, r1, geom, G, SV, gs, d.maxit, &st);
   }
   int ncalls = 1;
   if (stage > 1) {
@@ -624,7 +624,7 @@ GH_HD void body_update_g(const Dev& d, int j, int i, int stage, bool valid, cons
     double D = 1., S = 0., tau = 1.;
     if (cell) c2p_inputs<ND, NQ>(U, cxn, geom, &D, &S, &tau);
     double gs = guess(cell, D, S, tau, p[PPP], geom);
-    if (cell) cons2prim<ND, NQ>(p, U, cxn, geom, G, d.solver, gs, d.maxit, &st);
+    if (cell) cons2prim<ND, NQ>(p, U, cxn, geom, G, SV, gs, d.maxit, &st);
     ncalls = 2;
   }
   if (!cell) return;
@@ -641,9 +641,16 @@ GH_HD void body_update_g(const Dev& d, int j, int i, int stage, bool valid, cons
 
 // Grid::update + Cell::update + RK3 averages (1d.cpp:798-841, 2d.cpp:974-1036,
 // cell.cpp:102-127, rk3.cpp:12-195).  One thread owns face i and cell i.
+// The solver is a template argument so that each kernel holds one root
+// finder only (smaller code, fewer instruction-cache misses).
+template <int ND, int NQ, int SV>
+GH_HD void body_update_s(const Dev& d, int j, int i, int stage) {
+  body_update_g<ND, NQ, SV>(d, j, i, stage, true, NoGuess{});
+}
 template <int ND, int NQ>
 GH_HD void body_update(const Dev& d, int j, int i, int stage) {
-  body_update_g<ND, NQ>(d, j, i, stage, true, NoGuess{});
+  if (d.solver == C2P_NEWTON) body_update_s<ND, NQ, C2P_NEWTON>(d, j, i, stage);
+  else body_update_s<ND, NQ, C2P_BRENT>(d, j, i, stage);
 }
 
 // ---------------------------------------------------------------------------
@@ -910,36 +917,45 @@ __device__ __forceinline__ void tile_of(int remap, int& j, int& i) {
 // tracks -- F1 neighbours -- share an L2.  Grid sizes are multiples of 8.
 // The trip count depends on blockIdx only, so every lane of a wavefront runs
 // the same iterations.
-#define GH_PACKED_LOOP(...)                                                      \
-  if (d.packed) {                                                                \
-    int nt_ = *d.ntiles, G_ = gridDim.x, t_, end_, st_;                          \
-    if (remap) {                                                                 \
-      int x_ = blockIdx.x & 7;                                                   \
-      t_ = (int)((long long)nt_ * x_ / 8) + (int)(blockIdx.x >> 3);              \
-      end_ = (int)((long long)nt_ * (x_ + 1) / 8);                               \
-      st_ = G_ >> 3;                                                             \
-    } else {                                                                     \
-      t_ = blockIdx.x; end_ = nt_; st_ = G_;                                     \
+// Packed form: a persistent grid walks the tile list built by k_tiles.  With
+// remap the list is cut into 8 contiguous bands, band x walked by the
+// workgroups that the dispatcher places on XCD x (linear id % 8), so adjacent
+// tracks -- F1 neighbours -- share an L2.  Grid sizes are multiples of 8.
+// The plain form is the same loop with one trip, so the body is emitted once.
+// The trip count depends on blockIdx only, so every lane of a wavefront runs
+// the same iterations.
+#define GH_TILE_LOOP(...)                                                        \
+  {                                                                              \
+    int t_ = 0, end_ = 1, st_ = 1;                                               \
+    if (d.packed) {                                                              \
+      int nt_ = *d.ntiles, G_ = gridDim.x;                                       \
+      if (remap) {                                                               \
+        int x_ = blockIdx.x & 7;                                                 \
+        t_ = (int)((long long)nt_ * x_ / 8) + (int)(blockIdx.x >> 3);            \
+        end_ = (int)((long long)nt_ * (x_ + 1) / 8);                             \
+        st_ = G_ >> 3;                                                           \
+      } else {                                                                   \
+        t_ = blockIdx.x; end_ = nt_; st_ = G_;                                   \
+      }                                                                          \
     }                                                                            \
     for (; t_ < end_; t_ += st_) {                                               \
-      int j = d.tiles[2 * t_], i = d.tiles[2 * t_ + 1] * (int)blockDim.x + (int)threadIdx.x; \
+      int i, j;                                                                  \
+      if (d.packed) {                                                            \
+        j = d.tiles[2 * t_];                                                     \
+        i = d.tiles[2 * t_ + 1] * (int)blockDim.x + (int)threadIdx.x;            \
+      } else {                                                                   \
+        tile_of(remap, j, i);                                                    \
+      }                                                                          \
       __VA_ARGS__;                                                               \
     }                                                                            \
-    return;                                                                      \
   }
 #define GH_KERNEL2P(NAME, BODY)                                                 \
   __global__ void __launch_bounds__(KB) NAME(Dev d, int ni, int remap) {        \
-    GH_PACKED_LOOP(if (i < ni) BODY(d, j, i))                                    \
-    int i, j;                                                                    \
-    tile_of(remap, j, i);                                                        \
-    if (i < ni && j < d.nj) BODY(d, j, i);                                       \
+    GH_TILE_LOOP(if (i < ni && j < d.nj) BODY(d, j, i))                          \
   }
 #define GH_KERNEL2SP(NAME, BODY)                                                \
   __global__ void __launch_bounds__(KB) NAME(Dev d, int ni, int remap, int stage) { \
-    GH_PACKED_LOOP(if (i < ni) BODY(d, j, i, stage))                             \
-    int i, j;                                                                    \
-    tile_of(remap, j, i);                                                        \
-    if (i < ni && j < d.nj) BODY(d, j, i, stage);                                \
+    GH_TILE_LOOP(if (i < ni && j < d.nj) BODY(d, j, i, stage))                   \
   }
 namespace {
 GH_KERNEL2(k_ghost_tracks, body_ghost_tracks)
@@ -954,7 +970,7 @@ GH_KERNEL2(k_advance_time, body_advance_time)
 template <int ND, int NQ> GH_KERNEL2(k_ghost_mv, (body_ghost_mv<ND, NQ>))
 template <int ND, int NQ> GH_KERNEL2P(k_faces, (body_faces<ND, NQ>))
 template <int NQ> GH_KERNEL2P(k_f1faces, body_f1faces<NQ>)
-template <int ND, int NQ> GH_KERNEL2SP(k_update, (body_update<ND, NQ>))
+template <int ND, int NQ, int SV> GH_KERNEL2SP(k_update, (body_update_s<ND, NQ, SV>))
 template <int ND, int NQ> GH_KERNEL2P(k_inject, (body_inject<ND, NQ>))
 template <int ND, int NQ> GH_KERNEL2(k_init_gammas, (body_init_gammas<ND, NQ>))
 template <int ND, int NQ> GH_KERNEL2P(k_rg_screen, (body_rg_screen<ND, NQ>))
@@ -1247,10 +1263,7 @@ __global__ void __launch_bounds__(KB) k_update_mlp(Dev d, int ni, int remap, int
   __shared__ float rows[KB / 64][64];
   int wv = threadIdx.x / 64;
   MlpGuess g{d.mw, (const bf16x8*)d.mw2b, lds[wv], rows[wv], d.mlp_thr, d.P.gamma};
-  GH_PACKED_LOOP(body_update_g<ND, NQ>(d, j, i, stage, i < ni, g))
-  int i, j;
-  tile_of(remap, j, i);
-  body_update_g<ND, NQ>(d, j, i, stage, i < ni, g);
+  GH_TILE_LOOP(body_update_g<ND, NQ, C2P_NEWTON>(d, j, i, stage, i < ni, g))
 }
 
 // lane-layout probe of v_mfma_f32_32x32x16_bf16: D = A B for row-major A (32x16), B (16x32)
@@ -1489,7 +1502,8 @@ struct Pipe {
     L2S(c, (body_update<ND, NQ>), d.nj, d.nax, stage);
 #else
     if (d.mw) L2SP(c, (k_update_mlp<ND, NQ>), d.nj, d.nax, stage);
-    else L2SP(c, (k_update<ND, NQ>), d.nj, d.nax, stage);
+    else if (d.solver == C2P_NEWTON) L2SP(c, (k_update<ND, NQ, C2P_NEWTON>), d.nj, d.nax, stage);
+    else L2SP(c, (k_update<ND, NQ, C2P_BRENT>), d.nj, d.nax, stage);
 #endif
     tend(c, 0);
     std::swap(d.fx, d.fxn);
