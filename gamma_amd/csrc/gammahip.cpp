@@ -76,6 +76,7 @@ struct Dev {
   int packed;
   int* tiles;              // [cap][2] = (track, chunk)
   int* ntiles;             // [1]
+  double* ttrig;           // [nj][7] Trig of each track (2D spherical geometry)
 };
 
 GH_HD long long CI(const Dev& d, int j, int i) { return (long long)j * d.nax + i; }
@@ -93,9 +94,19 @@ GH_HD void store_v(const Dev& d, double* a, long long k, const double* v) {
     if (!(ND == 1 && q == 3)) a[sq<ND>(q) * d.cs + k] = v[q];
 }
 
+// the track's Trig (zeros where the geometry does not use it)
+template <int ND>
+GH_HD Trig trig_at(const Dev& d, int j) {
+  Trig t = {0., 0., 0., 0., 0., 0., 0.};
+  if (ND == 2 && d.tgeom[j] == SPH) {
+    const double* a = d.ttrig + 7 * (long long)j;
+    t.s_hdy = a[0]; t.s_y = a[1]; t.c_y = a[2]; t.s_1 = a[3]; t.s_2 = a[4]; t.c_1 = a[5]; t.c_2 = a[6];
+  }
+  return t;
+}
 template <int ND>
 GH_HD Geo geo_of(const Dev& d, int j, long long k) {
-  return cell_geo<ND>(d.cx[k], d.cdx[k], d.ty[j], d.tdy[j], d.tgeom[j]);
+  return cell_geo<ND>(d.cx[k], d.cdx[k], d.ty[j], d.tdy[j], d.tgeom[j], trig_at<ND>(d, j));
 }
 // r argument of the grid sweeps: centroid in 1D (1d.cpp:915), centre in 2D (2d.cpp:1127)
 template <int ND>
@@ -230,7 +241,8 @@ GH_HD void body_faces(const Dev& d, int j, int i) {
       if (i == nt - 2 - n) v = d.tvout[j];
     }
   }
-  double dA = face_area<ND>(0, xf, d.ty[j], d.tdy[j], geom);
+  Trig T = trig_at<ND>(d, j);
+  double dA = face_area<ND>(0, xf, d.tdy[j], geom, T.s_y, T.s_hdy);
   double fl[NQ];
   hllc_flux<ND, NQ>(pL, cL, fL, pR, cR, fR, w, v, dA, 0, xf, geom, fl);
   if (fl[0] != fl[0]) GH_ATOMIC_ADD(&d.cnt[CNT_NANFLUX], 1ull);
@@ -336,6 +348,7 @@ template <int NQ>
 GH_HD void trans_grad(const Dev& d, int j, long long k, const double* p0, const double* m0,
                       const Geo& g0, double* gf) {
   int geom = d.tgeom[j];
+  Trig Tj = trig_at<2>(d, j);
   double x0 = d.cx[k];
   double xL0 = x0 - d.cdx[k] / 2., xR0 = x0 + d.cdx[k] / 2.;
   double gavg[NQ];
@@ -350,11 +363,11 @@ GH_HD void trans_grad(const Dev& d, int j, long long k, const double* p0, const 
         long long kn = CI(d, tj, lo + n);
         double xn = d.cx[kn];
         double xLn = xn - d.cdx[kn] / 2., xRn = xn + d.cdx[kn] / 2.;
-        double xf1I = d.ty[j];
-        if (side == 0) xf1I -= d.tdy[j] / 2.;
+        // face at y_j - dy_j/2 on the lower side; the upper side keeps y_j (the
+        // second branch of 2d.cpp:386-391 tests d == 0 again)
         double xm = (fmax(xL0, xLn) + fmin(xR0, xRn)) / 2.;
         double ov = fmax(0, fmin(xR0, xRn) - fmax(xL0, xLn));
-        double dA = face_area<2>(1, xm, xf1I, ov, geom);
+        double dA = face_area<2>(1, xm, ov, geom, side == 0 ? Tj.s_1 : Tj.s_y, 0.);
         Geo gn = geo_of<2>(d, tj, kn);
         double dxmL = xm - g0.cen0, dxmR = xm - gn.cen0;
         double pn[NQ], mn[NQ];
@@ -387,6 +400,7 @@ GH_HD void body_f1faces(const Dev& d, int j, int i) {
   int nt = d.ntrack[j];
   if (i >= nt) return;
   int geom = d.tgeom[j];
+  Trig Tj = trig_at<2>(d, j);
   double G = d.P.gamma;
   long long k = CI(d, j, i);
   Geo g0 = geo_of<2>(d, j, k);
@@ -409,7 +423,7 @@ GH_HD void body_f1faces(const Dev& d, int j, int i) {
     double xLn = xn - d.cdx[kn] / 2., xRn = xn + d.cdx[kn] / 2.;
     double xm = (fmax(xL0, xLn) + fmin(xR0, xRn)) / 2.;
     double ov = fmax(0, fmin(xR0, xRn) - fmax(xL0, xLn));
-    double dA = face_area<2>(1, xm, jpos, ov, geom);
+    double dA = face_area<2>(1, xm, ov, geom, Tj.s_2, 0.);
     double pL[NQ], cL[NQ], fL[NQ], pR[NQ], cR[NQ], fR[NQ];
     if (pc0 || d.nbucnt[kn] == 0) {
       cell_state<2, NQ>(d, j, i, 1, pL, cL, fL);
@@ -592,11 +606,11 @@ GH_HD void body_update_g(const Dev& d, int j, int i, int stage, bool valid, cons
       });
       for (int q = 0; q < NQ; ++q) U[q] += (dn[q] - up[q]) * dt;
     }
-    sources<ND, NQ>(p, U, d.cx[k], d.cdx[k], d.ty[j], g.dV, dt, geom, G);
+    sources<ND, NQ>(p, U, d.cx[k], d.cdx[k], trig_at<ND>(d, j), g.dV, dt, geom, G);
     if (NQ == 10 && d.sync) radiative(p, U, g.dV, dt, d.P);
     cxn = (xnR + xnL) / 2.;
     cdxn = (xnR - xnL);
-    g2 = cell_geo<ND>(cxn, cdxn, d.ty[j], d.tdy[j], geom);
+    g2 = cell_geo<ND>(cxn, cdxn, d.ty[j], d.tdy[j], geom, trig_at<ND>(d, j));
     for (int q = 0; q < NQ; ++q) U[q] /= g2.dV;
     r1 = g2.cen0;
   }
@@ -618,7 +632,7 @@ GH_HD void body_update_g(const Dev& d, int j, int i, int stage, bool valid, cons
       double xaR = w0 * d.fx0[k] + w1 * xnR;
       cxn = (xaR + xaL) / 2.;
       cdxn = (xaR - xaL);
-      Geo g3 = cell_geo<ND>(cxn, cdxn, d.ty[j], d.tdy[j], geom);
+      Geo g3 = cell_geo<ND>(cxn, cdxn, d.ty[j], d.tdy[j], geom, trig_at<ND>(d, j));
       for (int q = 0; q < NQ; ++q) U[q] /= g3.dV;
     }
     double D = 1., S = 0., tau = 1.;
@@ -744,6 +758,10 @@ GH_HD void rg_move_cell(const Dev& d, long long kd, long long ks) {
   }
 }
 
+// Grid::split (1d.cpp:560-662): cells i+1.. and faces i.. move up one
+// slot, then the two halves of cell i are filled (split_fill).
+template <int ND, int NQ>
+GH_HD void split_fill(const Dev& d, int j, int i);
 template <int ND, int NQ>
 GH_HD void rg_split(const Dev& d, int j, int i) {
   int nt = d.ntrack[j];
@@ -751,6 +769,11 @@ GH_HD void rg_split(const Dev& d, int j, int i) {
   for (int m = nt - 1; m >= i + 1; --m) rg_move_cell<ND>(d, b + m + 1, b + m);
   for (int m = nt - 2; m >= i; --m) d.fx[b + m + 1] = d.fx[b + m];
   d.ntrack[j] = nt + 1;
+  split_fill<ND, NQ>(d, j, i);
+}
+template <int ND, int NQ>
+GH_HD void split_fill(const Dev& d, int j, int i) {
+  long long b = CI(d, j, 0);
   int geom = d.tgeom[j];
   double pLc[NQ], pc[NQ], pRo[NQ], gC[NQ];
   load_v<ND, NQ>(d, d.prim, b + i - 1, pLc);
@@ -773,8 +796,9 @@ GH_HD void rg_split(const Dev& d, int j, int i) {
   d.cx[b + i + 1] = x_old + dl_old / 4.;
   d.cdx[b + i] = dl_old / 2.;
   d.cdx[b + i + 1] = dl_old / 2.;
-  Geo gc = cell_geo<ND>(d.cx[b + i], d.cdx[b + i], d.ty[j], d.tdy[j], geom);
-  Geo gr = cell_geo<ND>(d.cx[b + i + 1], d.cdx[b + i + 1], d.ty[j], d.tdy[j], geom);
+  Trig T = trig_at<ND>(d, j);
+  Geo gc = cell_geo<ND>(d.cx[b + i], d.cdx[b + i], d.ty[j], d.tdy[j], geom, T);
+  Geo gr = cell_geo<ND>(d.cx[b + i + 1], d.cdx[b + i + 1], d.ty[j], d.tdy[j], geom, T);
   double xL = gc.cen0, xR = gr.cen0;
   double xI = d.cx[b + i] + d.cdx[b + i] / 2.;
   double pn[NQ], pr[NQ];
@@ -788,8 +812,27 @@ GH_HD void rg_split(const Dev& d, int j, int i) {
   d.fx[b + i] = x_old;
 }
 
+// Grid::merge (1d.cpp:665-728): cell i absorbs its smaller neighbour
+// v (merge_fill, returns whether v = i-1), then the cells after the pair move
+// down one slot.
+template <int ND, int NQ>
+GH_HD bool merge_fill(const Dev& d, int j, int i);
 template <int ND, int NQ>
 GH_HD void rg_merge(const Dev& d, int j, int i) {
+  int nt = d.ntrack[j];
+  long long b = CI(d, j, 0);
+  bool left = merge_fill<ND, NQ>(d, j, i);
+  if (left) {
+    for (int m = i; m < nt; ++m) rg_move_cell<ND>(d, b + m - 1, b + m);
+    for (int m = i; m < nt - 1; ++m) d.fx[b + m - 1] = d.fx[b + m];
+  } else {
+    for (int m = i + 2; m < nt; ++m) rg_move_cell<ND>(d, b + m - 1, b + m);
+    for (int m = i + 1; m < nt - 1; ++m) d.fx[b + m - 1] = d.fx[b + m];
+  }
+  d.ntrack[j] = nt - 1;
+}
+template <int ND, int NQ>
+GH_HD bool merge_fill(const Dev& d, int j, int i) {
   int nt = d.ntrack[j];
   int nact = nt - 2 * d.ngst;
   long long b = CI(d, j, 0);
@@ -810,18 +853,11 @@ GH_HD void rg_merge(const Dev& d, int j, int i) {
   double dxnew = dxc + dxv;
   d.cx[b + i] = xnew;
   d.cdx[b + i] = dxnew;
-  Geo g2 = cell_geo<ND>(xnew, dxnew, d.ty[j], d.tdy[j], geom);
+  Geo g2 = cell_geo<ND>(xnew, dxnew, d.ty[j], d.tdy[j], geom, trig_at<ND>(d, j));
   C2PStat st = {0, 0, 0, 0};
   cons2prim<ND, NQ>(pc, cc, ND == 1 ? g2.cen0 : xnew, geom, G, d.solver, -1., d.maxit, &st);
   store_v<ND, NQ>(d, d.prim, b + i, pc);
-  if (left) {
-    for (int m = i; m < nt; ++m) rg_move_cell<ND>(d, b + m - 1, b + m);
-    for (int m = i; m < nt - 1; ++m) d.fx[b + m - 1] = d.fx[b + m];
-  } else {
-    for (int m = i + 2; m < nt; ++m) rg_move_cell<ND>(d, b + m - 1, b + m);
-    for (int m = i + 1; m < nt - 1; ++m) d.fx[b + m - 1] = d.fx[b + m];
-  }
-  d.ntrack[j] = nt - 1;
+  return left;
 }
 
 // parallel screen: first active cell of each track whose rule asks for an
@@ -1063,10 +1099,126 @@ __global__ void __launch_bounds__(KB) k_f1_scan(Dev d) {
     if (base > d.fcap) atomicAdd(&d.cnt[CNT_F1OVF], 1ull);
   }
 }
+// The applyRegrid loop of one track on one workgroup.  The decisions and
+// the arithmetic of each split/merge stay on thread 0, in the sequential
+// order; the O(ntrack) work per action -- moving the cells after it one slot
+// and locating the shock -- is spread over the workgroup.  Moves go in chunks
+// of RGB cells, each chunk read into registers before any of it is written,
+// chunks taken from the far end towards the action, which is the order in
+// which the sequential loops copy.
+constexpr int RGB = 256;
 template <int ND, int NQ>
-__global__ void k_rg_apply(Dev d) {
-  int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j < d.nj) body_rg_apply<ND, NQ>(d, j, 0);
+struct CellRec {
+  static constexpr int NS = ND == 1 ? NQ - 1 : NQ;
+  double p[NS], gm[ND == 2 ? NS : 1], gf[ND == 2 ? NS : 1];
+  double x, dx, dt;
+  int nb[6];
+};
+template <int ND, int NQ>
+__device__ __forceinline__ void rec_load(const Dev& d, long long k, CellRec<ND, NQ>& r) {
+  constexpr int NS = CellRec<ND, NQ>::NS;
+  for (int s = 0; s < NS; ++s) r.p[s] = d.prim[s * d.cs + k];
+  r.x = d.cx[k]; r.dx = d.cdx[k]; r.dt = d.dtloc[k];
+  if (ND == 2) {
+    for (int s = 0; s < NS; ++s) { r.gm[s] = d.gmv[s * d.cs + k]; r.gf[s] = d.gf1[s * d.cs + k]; }
+    r.nb[0] = d.nbdj[k]; r.nb[1] = d.nbdlo[k]; r.nb[2] = d.nbdcnt[k];
+    r.nb[3] = d.nbuj[k]; r.nb[4] = d.nbulo[k]; r.nb[5] = d.nbucnt[k];
+  }
+}
+template <int ND, int NQ>
+__device__ __forceinline__ void rec_store(const Dev& d, long long k, const CellRec<ND, NQ>& r) {
+  constexpr int NS = CellRec<ND, NQ>::NS;
+  for (int s = 0; s < NS; ++s) d.prim[s * d.cs + k] = r.p[s];
+  d.cx[k] = r.x; d.cdx[k] = r.dx; d.dtloc[k] = r.dt;
+  if (ND == 2) {
+    for (int s = 0; s < NS; ++s) { d.gmv[s * d.cs + k] = r.gm[s]; d.gf1[s * d.cs + k] = r.gf[s]; }
+    d.nbdj[k] = r.nb[0]; d.nbdlo[k] = r.nb[1]; d.nbdcnt[k] = r.nb[2];
+    d.nbuj[k] = r.nb[3]; d.nbulo[k] = r.nb[4]; d.nbucnt[k] = r.nb[5];
+  }
+}
+// cells [lo, hi) of the track at b move by dir (+1 or -1), faces [flo, fhi) too
+template <int ND, int NQ>
+__device__ void wg_shift(const Dev& d, long long b, int lo, int hi, int flo, int fhi, int dir) {
+  int t = threadIdx.x;
+  int n = hi - lo, nf = fhi - flo;
+  int nmax = n > nf ? n : nf;
+  for (int c0 = 0; c0 < nmax; c0 += RGB) {
+    // up: serve the top first; down: serve the bottom first
+    int o = c0 + t;
+    int m = dir > 0 ? hi - 1 - o : lo + o;
+    int mf = dir > 0 ? fhi - 1 - o : flo + o;
+    bool act = o < n, actf = o < nf;
+    CellRec<ND, NQ> r;
+    double fx = 0.;
+    if (act) rec_load<ND, NQ>(d, b + m, r);
+    if (actf) fx = d.fx[b + mf];
+    __syncthreads();
+    if (act) rec_store<ND, NQ>(d, b + m + dir, r);
+    if (actf) d.fx[b + mf + dir] = fx;
+    __syncthreads();
+  }
+}
+// rg_shock_index over the workgroup: the last active cell with p > 1.5 eta
+__device__ double wg_shock_index(const Dev& d, int j, int* red) {
+  int nt = d.ntrack[j], iL = d.ngst - 1, iR = nt - d.ngst;
+  double thr = 1.5 * d.rg[3];
+  int best = iL + 1;
+  for (int ii = iL + 1 + (int)threadIdx.x; ii <= iR - 1; ii += RGB)
+    if (d.prim[PPP * d.cs + CI(d, j, ii)] > thr && ii > best) best = ii;
+  red[threadIdx.x] = best;
+  __syncthreads();
+  for (int w = RGB / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w && red[threadIdx.x + w] > red[threadIdx.x]) red[threadIdx.x] = red[threadIdx.x + w];
+    __syncthreads();
+  }
+  double r = red[0];
+  __syncthreads();
+  return r;
+}
+template <int ND, int NQ>
+__global__ void __launch_bounds__(RGB) k_rg_apply(Dev d) {
+  __shared__ int red[RGB];
+  __shared__ int s_act;
+  int j = blockIdx.x;
+  if (!active_track(d, j)) return;
+  int i = d.tfirst[j];
+  if (i == INT_MAX) return;
+  bool bm = d.trg[j] == RG_BM1D;
+  long long b = CI(d, j, 0);
+  double iS = bm ? wg_shock_index(d, j, red) : 0.;
+  for (;; ++i) {
+    int nt = d.ntrack[j];
+    if (i > nt - d.ngst - 1) break;
+    if (threadIdx.x == 0) {
+      int a = rg_check<ND, NQ>(d, j, i, iS);
+      s_act = (a == RG_SPLIT && nt < d.nax - 1) ? 1 : (a == RG_MERGE && nt > 2 * d.ngst + 2) ? 2 : 0;
+    }
+    __syncthreads();
+    int act = s_act;
+    __syncthreads();
+    if (act == 1) {
+      wg_shift<ND, NQ>(d, b, i + 1, nt, i, nt - 1, +1);
+      if (threadIdx.x == 0) {
+        d.ntrack[j] = nt + 1;
+        split_fill<ND, NQ>(d, j, i);
+        d.nsplit[j]++;
+      }
+      __syncthreads();
+      if (bm) iS = wg_shock_index(d, j, red);
+    } else if (act == 2) {
+      if (threadIdx.x == 0) s_act = merge_fill<ND, NQ>(d, j, i) ? 1 : 0;
+      __syncthreads();
+      bool left = s_act == 1;
+      if (left) wg_shift<ND, NQ>(d, b, i, nt, i, nt - 1, -1);
+      else wg_shift<ND, NQ>(d, b, i + 2, nt, i + 1, nt - 1, -1);
+      if (threadIdx.x == 0) {
+        d.ntrack[j] = nt - 1;
+        d.nmerge[j]++;
+      }
+      __syncthreads();
+      if (bm) iS = wg_shock_index(d, j, red);
+    }
+  }
 }
 }  // namespace
 #endif
@@ -1517,7 +1669,7 @@ struct Pipe {
 #ifdef GH_EMU
     for (int j = 0; j < d.nj; ++j) body_rg_apply<ND, NQ>(d, j, 0);
 #else
-    hipLaunchKernelGGL((k_rg_apply<ND, NQ>), dim3((d.nj + 63) / 64), dim3(64), 0, c->stream, d);
+    hipLaunchKernelGGL((k_rg_apply<ND, NQ>), dim3(d.nj), dim3(RGB), 0, c->stream, d);
 #endif
   }
   static void apply(gammahip_ctx* c, int what) {
@@ -1655,6 +1807,7 @@ int gammahip_create(const gammahip_config* cfg, gammahip_ctx** out) {
   size_t nj = d.nj, cs = d.cs, nqs = d.nqs;
   int e = 0;
   e |= dalloc(c, &d.tiles, (size_t)2 * nj * ((d.nax + 63) / 64)); e |= dalloc(c, &d.ntiles, 1);
+  e |= dalloc(c, &d.ttrig, (size_t)7 * nj);
   e |= dalloc(c, &d.ntrack, nj); e |= dalloc(c, &d.trg, nj); e |= dalloc(c, &d.tgeom, nj); e |= dalloc(c, &d.tkin, nj);
   e |= dalloc(c, &d.tbnd, nj); e |= dalloc(c, &d.tfirst, nj); e |= dalloc(c, &d.tiS, nj);
   e |= dalloc(c, &d.tvin, nj); e |= dalloc(c, &d.tvout, nj); e |= dalloc(c, &d.tbm, 4 * nj);
@@ -1782,6 +1935,15 @@ int gammahip_load(gammahip_ctx* c, const double* x, const double* dx, const doub
   e |= h2d(c, d.tvout, tvout.data(), sizeof(double) * nj);
   e |= h2d(c, d.tbm, tbm.data(), sizeof(double) * 4 * nj);
   e |= h2d(c, d.ty, ty.data(), sizeof(double) * nj);
+  {
+    std::vector<double> tt7((size_t)7 * nj, 0.);
+    for (int j = 0; j < nj; ++j) {
+      Trig t = trig_of(ty[j], tdy[j]);
+      double* a = &tt7[(size_t)7 * j];
+      a[0] = t.s_hdy; a[1] = t.s_y; a[2] = t.c_y; a[3] = t.s_1; a[4] = t.s_2; a[5] = t.c_1; a[6] = t.c_2;
+    }
+    e |= h2d(c, d.ttrig, tt7.data(), sizeof(double) * 7 * nj);
+  }
   e |= h2d(c, d.tdy, tdy.data(), sizeof(double) * nj);
   e |= h2d(c, d.tt, tt.data(), sizeof(double) * nj);
   e |= h2d(c, d.prim, hp.data(), sizeof(double) * hp.size());
@@ -1958,7 +2120,7 @@ int gammahip_download(gammahip_ctx* c, int field, double* out) {
         double pv[10], cv[10];
         for (int qq = 0; qq < nq; ++qq) pv[qq] = (nd == 1 && qq == 3) ? 0. : p[(size_t)sidx(qq) * cs + k];
         if (nd == 1) {
-          Geo g = cell_geo<1>(x[k], dx[k], ty[j], tdy[j], tg[j]);
+          Geo g = cell_geo<1>(x[k], dx[k], ty[j], tdy[j], tg[j], Trig{});
           if (nq == 6) prim2cons<1, 6>(pv, cv, g.cen0, tg[j], d.P.gamma);
           else prim2cons<1, 10>(pv, cv, g.cen0, tg[j], d.P.gamma);
         } else {
@@ -2004,7 +2166,8 @@ int gammahip_download(gammahip_ctx* c, int field, double* out) {
     for (int j = 0; j < d.nj; ++j)
       for (int i = 0; i < d.nax; ++i) {
         size_t k = (size_t)j * d.nax + i;
-        Geo g = nd == 1 ? cell_geo<1>(x[k], dx[k], ty[j], tdy[j], tg[j]) : cell_geo<2>(x[k], dx[k], ty[j], tdy[j], tg[j]);
+        Geo g = nd == 1 ? cell_geo<1>(x[k], dx[k], ty[j], tdy[j], tg[j], Trig{})
+                        : cell_geo<2>(x[k], dx[k], ty[j], tdy[j], tg[j], trig_of(ty[j], tdy[j]));
         out[k] = field == 204 ? g.dV : g.cen0;
       }
     return 0;

@@ -107,17 +107,23 @@ GH_HD double c2p_f(double p, double D, double S, double E, double G, int geom) {
   return (E + p - D * lf - (G * p * lf * lf) / (G - 1.));
 }
 
-// d f / d p for the Newton solver (analytic derivative of c2p_f); *fscale is
-// the sum of the magnitudes of the terms of f, i.e. the scale of its rounding error
+// d f / d p for the Newton solver (analytic derivative of c2p_f).  *fscale
+// is the sum of the magnitudes of the terms of f; *fnoise bounds its rounding
+// error more honestly, weighting the terms that hold the Lorentz factor by
+// 1 + W^2: W = 1/sqrt(w) with w = 1 - S^2/(E+p)^2 formed by cancellation
+// carries a relative error of about W^2 eps (W^2 ~ 1e4-1e6 in the
+// ultra-relativistic shells).
 GH_HD double c2p_df(double p, double D, double S, double E, double G, int geom, double* fval,
-                    double* fscale) {
+                    double* fscale, double* fnoise) {
   double Ep = E + p;
   double w = 1. - (S * S) / (Ep * Ep);
   double aw = geom == SPH ? fabs(w) : w;
   double lf = 1. / sqrt(aw);
   double gg = G / (G - 1.);
   *fval = (E + p - D * lf - (G * p * lf * lf) / (G - 1.));
-  *fscale = fabs(E) + fabs(p) + fabs(D * lf) + fabs((G * p * lf * lf) / (G - 1.));
+  double tw = fabs(D * lf) + fabs((G * p * lf * lf) / (G - 1.));
+  *fscale = fabs(E) + fabs(p) + tw;
+  *fnoise = *fscale + lf * lf * tw;
   // d lf / dp = -lf^3 S^2 / (E+p)^3 (sign of w folded in for |w|)
   double dw = 2. * (S * S) / (Ep * Ep * Ep);
   double dlf = -0.5 * lf * lf * lf * (geom == SPH && w < 0 ? -dw : dw);
@@ -194,20 +200,28 @@ GH_HD double brent(double D, double S, double E, double G, int geom, double lo, 
 // bisection step whenever Newton leaves the bracket.  Converged when |f| is at
 // the rounding level of its own terms (x is a root to working precision), or
 // after a step below 1e-15 relative (quadratic convergence puts the new
-// iterate well inside the reference's 1e-13 tolerance).
+// iterate well inside the reference's 1e-13 tolerance), or once the bracket is
+// that narrow.  Near v = 1 the rounding of f is W^2 larger than its terms
+// suggest and Newton ends up bouncing across the root on noise: once |f| is
+// within that larger bound and a step no longer halves the one before, x is
+// as good as f can tell, and the solve stops there.
 GH_HD double newton(double D, double S, double E, double G, int geom, double lo, double hi,
                     double x0, int maxit, C2PStat* st) {
   double x = (x0 >= lo && x0 <= hi) ? x0 : 0.5 * (lo + hi);
+  double dxold = hi - lo;
   int it = 0;
   for (;;) {
     it++;
-    double f, fs;
-    double df = c2p_df(x, D, S, E, G, geom, &f, &fs);
-    if (fabs(f) <= 4. * 2.2204460492503131e-16 * fs) break;
+    double f, fs, fn;
+    double df = c2p_df(x, D, S, E, G, geom, &f, &fs, &fn);
+    const double eps = 2.2204460492503131e-16;
+    if (fabs(f) <= 4. * eps * fs) break;
     if (f > 0) lo = x; else hi = x;      // f decreases through the root
     double xn = x - f / df;
     if (!(xn > lo && xn < hi)) xn = 0.5 * (lo + hi);
     double dx = fabs(xn - x);
+    if (it > 1 && fabs(f) <= 4. * eps * fn && dx > 0.5 * dxold) break;
+    dxold = dx;
     x = xn;
     if (dx <= 1.e-15 * fabs(x) || hi - lo <= 1.e-15 * fabs(lo)) break;
     if (it >= maxit) { st->fail |= 2; break; }
@@ -458,8 +472,22 @@ GH_HD double shock_sd(const double* S1, const double* S2, int dim, bool reverse,
 // ---- geometry (Geometry/*.cpp) ----
 struct Geo { double dV, dl0, dl1, cen0, cen1; };
 
+// Sines and cosines of a track's angular position y and width dy (2D
+// spherical geometry): the only transcendental functions of the geometry, and
+// functions of the track alone.  They are evaluated once per track on the
+// host (the C library's sin/cos, as the reference uses) and tabulated, so the
+// device never depends on its own math library for them.
+struct Trig { double s_hdy, s_y, c_y, s_1, s_2, c_1, c_2; };
+inline Trig trig_of(double y, double dy) {
+  Trig t;
+  double th1 = y - dy / 2., th2 = y + dy / 2.;
+  t.s_hdy = sin(.5 * dy); t.s_y = sin(y); t.c_y = cos(y);
+  t.s_1 = sin(th1); t.s_2 = sin(th2); t.c_1 = cos(th1); t.c_2 = cos(th2);
+  return t;
+}
+
 template <int ND>
-GH_HD Geo cell_geo(double x, double dx, double y, double dy, int geom) {
+GH_HD Geo cell_geo(double x, double dx, double y, double dy, int geom, const Trig& T) {
   Geo g;
   if (geom == CART) {
     g.dl0 = dx; g.dl1 = dy;
@@ -479,25 +507,27 @@ GH_HD Geo cell_geo(double x, double dx, double y, double dy, int geom) {
     g.dl0 = dr; g.dl1 = r * dth;
     double r1 = r - dr / 2., r2 = r + dr / 2.;
     double rsq = r1 * r1 + r1 * r2 + r2 * r2;
-    g.dV = 4. / 3. * kPi * rsq * dr * fabs(sin(.5 * dth) * sin(th));
+    g.dV = 4. / 3. * kPi * rsq * dr * fabs(T.s_hdy * T.s_y);
     double rr = r * r, dr2 = dr * dr;
     double th1 = th - dth / 2., th2 = th + dth / 2.;
     g.cen0 = 3. * r * (rr + dr2) / (3 * rr + dr2);
-    g.cen1 = (th1 * cos(th1) - th2 * cos(th2) + fabs(sin(th2)) - fabs(sin(th1))) / (cos(th1) - cos(th2));
+    g.cen1 = (th1 * T.c_1 - th2 * T.c_2 + fabs(T.s_2) - fabs(T.s_1)) / (T.c_1 - T.c_2);
   }
   return g;
 }
 
 // Interface::computedA for a face normal to dim at (x, y) with transverse
-// extent dxf (cartesian.cpp:135-143, spherical1D.cpp:60-76, spherical.cpp:81-112)
+// extent dxf (cartesian.cpp:135-143, spherical1D.cpp:60-76, spherical.cpp:81-112).
+// 2D spherical: sy = sin(y) and shd = sin(dxf / 2) (dim 0 only), from the
+// track's Trig table.
 template <int ND>
-GH_HD double face_area(int dim, double x, double y, double dxf, int geom) {
+GH_HD double face_area(int dim, double x, double dxf, int geom, double sy, double shd) {
   if (geom == CART) return ND == 1 ? 1. : dxf;
   if (ND == 1) return 4. * kPi * x * x;
-  if (dim == 0) return 4. * kPi * x * x * fabs(sin(.5 * dxf) * sin(y));
+  if (dim == 0) return 4. * kPi * x * x * fabs(shd * sy);
   double dr = dxf, r1 = x - dr / 2., r2 = x + dr / 2.;
   double rsq = (r1 + r2) * dr;
-  return kPi * fabs(sin(y)) * rsq;
+  return kPi * fabs(sy) * rsq;
 }
 
 GH_HD double minmod(double a, double b) {      // 1d.cpp:250-260
@@ -508,7 +538,7 @@ GH_HD double minmod(double a, double b) {      // 1d.cpp:250-260
 
 // Cell::sourceTerms  rel_sph.cpp:452-496 (cartesian: none)
 template <int ND, int NQ>
-GH_HD void sources(const double* prim, double* U, double x, double dx, double y, double dV,
+GH_HD void sources(const double* prim, double* U, double x, double dx, const Trig& T, double dV,
                    double dt, int geom, double G) {
   if (geom != SPH) return;
   double r = x, dr = dx;
@@ -520,7 +550,7 @@ GH_HD void sources(const double* prim, double* U, double x, double dx, double y,
     double rho = prim[RHO], ut = prim[UU2], p = prim[PPP];
     double h = 1. + p * G / (G - 1.) / rho;
     U[SS1] += (rho * h * ut * ut + 2. * p) * (r / rsq) * dV * dt;
-    U[SS2] += p * cos(y) / fabs(sin(y)) * dV * dt;
+    U[SS2] += p * T.c_y / fabs(T.s_y) * dV * dt;
   }
 }
 
