@@ -26,7 +26,7 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0      # MI355X HBM3E, GB/s
 
 # HIP-event timed kernels (ids of gammahip_kernel_times)
-KNAMES = {0: "k_update", 1: "k_faces", 2: "k_f1faces"}
+KNAMES = {0: "k_update", 1: "k_faces", 2: "k_f1faces", 3: "k_f1grad"}
 
 
 def algorithmic_bytes(kid: int, cells: float, f1faces: float, nq: int, stage_mix: float) -> float:
@@ -43,11 +43,13 @@ def algorithmic_bytes(kid: int, cells: float, f1faces: float, nq: int, stage_mix
         return cells * per
     if kid == 1:   # k_faces: prim, centroid inputs (x, dx), face x in; v, lL, lR, flux, grad_mv out
         return cells * ((nq + 2 + 1) * d + (3 + nq + nq) * d)
-    if kid == 2:   # k_f1faces: per cell prim, grad_mv, grad_f1 (new) + geometry + slot ints;
-        # per face the neighbour's prim, grad_mv, old grad_f1, geometry; flux out
-        return cells * ((3 * nq + 2) * d + 5 * i4) + f1faces * ((3 * nq + 2) * d + nq * d)
-    if kid == 3:   # k_gradf1: prim, grad_mv, geometry of the cell and of its neighbours; grad_f1 out
-        return cells * ((2 * nq + 2) * d + 6 * i4 + nq * d) + 2 * f1faces * (2 * nq + 2) * d
+    if kid == 2:   # k_f1faces (one thread per F1 face): owner cell prim, grad_mv, new grad_f1,
+        # x, dx once per cell; per face the upper cell's prim, grad_mv, old grad_f1,
+        # x, dx, six slot/list ints, and the flux out
+        return cells * (3 * nq + 2) * d + f1faces * ((3 * nq + 2) * d + nq * d + 6 * i4)
+    if kid == 3:   # k_f1grad: prim, grad_mv, x, dx and the neighbour lists of each cell
+        # (neighbours' values shared between cells), grad_f1 out
+        return cells * ((3 * nq + 2) * d + 6 * i4)
     return 0.0
 
 
@@ -91,13 +93,14 @@ def run_form(setup, solver, options, steps, warmup, timing_steps, surrogate=None
     el = time.perf_counter() - t0
     n1 = int(g.tracks()["nact"][2:-2].sum())
     st = g.stats()
+    nf1 = float(sum(a.sum() for a in g.active(hip.F_NF1))) if setup.cfg.nd == 2 else 0.
     g.kernel_times()                      # switches event timing on and clears
     g.step(timing_steps)
     g.sync()
     ms, nl = g.kernel_times()
     g.close()
     return {"value": 0.5 * (n0 + n1) * steps / el, "ms_per_step": 1e3 * el / steps,
-            "stats": st, "ms": ms, "launches": nl, "cells": n1}
+            "stats": st, "ms": ms, "launches": nl, "cells": n1, "f1faces": nf1}
 
 
 def track_cells(g):
@@ -153,7 +156,7 @@ def other_configs(steps, warmup):
 
 
 def kernel_table(res, nq):
-    f1faces = 2.0 * res["cells"]
+    f1faces = res["f1faces"]
     kern = {}
     for kid, name in KNAMES.items():
         if res["launches"][kid]:

@@ -60,6 +60,7 @@ struct Dev {
   // cells
   double *prim, *cx, *cdx, *dtloc, *U0, *dV0, *gmv, *gf1, *gf1n, *sd, *pspec;
   int *shk, *nbdj, *nbdlo, *nbdcnt, *nbuj, *nbulo, *nbucnt, *f1off, *f1tot;
+  int *f1own;              // [nj][fcap] cell owning each F1 face slot
   int *incf;               // first fresh lower-track cell whose F1 list holds this cell
   // faces
   double *fx, *fxn, *fx0, *fv, *flL, *flR, *fflux;
@@ -388,20 +389,14 @@ GH_HD void trans_grad(const Dev& d, int j, long long k, const double* p0, const 
   }
 }
 
-// One cell's pass of the F1 loop of Grid::computeFluxes (2d.cpp:901-950):
-// its transverse gradient (computeTransGradients, stored as the new value),
-// then its faces with the upper neighbours.  The upper track's gradients are
-// the previous stage's (2d.cpp:901 loops j upward, so track j+1 has not been
-// recomputed yet); stale lists of the top ghost track pointing into their own
-// track read the previous values too (their faces only feed ghost cells).
+// The F1 loop of Grid::computeFluxes (2d.cpp:901-950) in two passes.
+// Cell pass: each cell's transverse gradient (computeTransGradients), stored
+// as the new value.
 template <int NQ>
-GH_HD void body_f1faces(const Dev& d, int j, int i) {
+GH_HD void body_f1grad(const Dev& d, int j, int i) {
   if (j > d.nj - 2) return;
   int nt = d.ntrack[j];
   if (i >= nt) return;
-  int geom = d.tgeom[j];
-  Trig Tj = trig_at<2>(d, j);
-  double G = d.P.gamma;
   long long k = CI(d, j, i);
   Geo g0 = geo_of<2>(d, j, k);
   double pc[NQ], mc[NQ], fc[NQ];
@@ -409,57 +404,73 @@ GH_HD void body_f1faces(const Dev& d, int j, int i) {
   load_v<2, NQ>(d, d.gmv, k, mc);
   trans_grad<NQ>(d, j, k, pc, mc, g0, fc);
   store_v<2, NQ>(d, d.gf1n, k, fc);
-  int tj = d.nbuj[k], lo = d.nbulo[k], cnt = d.nbucnt[k], off = d.f1off[k];
-  if (cnt <= 0) return;
+}
+
+// Face pass: one F1 face slot s of track j, i.e. the face between its owner
+// cell (j, i) and upper neighbour n of that cell.  The owner's gradient is the
+// one just computed; the upper track's is the previous stage's (2d.cpp:901
+// loops j upward, so track j+1 has not been recomputed yet); stale lists of
+// the top ghost track pointing into their own track read the previous values
+// too (their faces only feed ghost cells).
+template <int NQ>
+GH_HD void body_f1face(const Dev& d, int j, int s) {
+  if (j > d.nj - 2) return;
+  int tot = d.f1tot[j];
+  if (s >= tot || s >= d.fcap) return;
+  int i = d.f1own[(long long)j * d.fcap + s];
+  int geom = d.tgeom[j];
+  Trig Tj = trig_at<2>(d, j);
+  double G = d.P.gamma;
+  long long k = CI(d, j, i);
+  int tj = d.nbuj[k], lo = d.nbulo[k], n = s - d.f1off[k];
+  Geo g0 = geo_of<2>(d, j, k);
   double x0 = d.cx[k];
   double xL0 = x0 - d.cdx[k] / 2., xR0 = x0 + d.cdx[k] / 2.;
   double jpos = d.ty[j] + d.tdy[j] / 2.;
   bool pc0 = (j == 0 || j == d.nj - 2 || d.nbdcnt[k] == 0);
-  for (int n = 0; n < cnt; ++n) {
-    long long slot = off + n;
-    if (slot >= d.fcap) { GH_ATOMIC_ADD(&d.cnt[CNT_F1OVF], 1ull); break; }
-    long long kn = CI(d, tj, lo + n);
-    double xn = d.cx[kn];
-    double xLn = xn - d.cdx[kn] / 2., xRn = xn + d.cdx[kn] / 2.;
-    double xm = (fmax(xL0, xLn) + fmin(xR0, xRn)) / 2.;
-    double ov = fmax(0, fmin(xR0, xRn) - fmax(xL0, xLn));
-    double dA = face_area<2>(1, xm, ov, geom, Tj.s_2, 0.);
-    double pL[NQ], cL[NQ], fL[NQ], pR[NQ], cR[NQ], fR[NQ];
-    if (pc0 || d.nbucnt[kn] == 0) {
-      cell_state<2, NQ>(d, j, i, 1, pL, cL, fL);
-      cell_state<2, NQ>(d, tj, lo + n, 1, pR, cR, fR);
-    } else {
-      Geo gn = geo_of<2>(d, tj, kn);
-      double dxmL = xm - g0.cen0, dxmR = xm - gn.cen0;
-      double dxfL = jpos - g0.cen1, dxfR = jpos - gn.cen1;
-      double pn[NQ], mn[NQ], fn[NQ];
-      load_v<2, NQ>(d, d.prim, kn, pn);
-      load_v<2, NQ>(d, d.gmv, kn, mn);
-      // the upper track has not recomputed its transverse gradients yet (2d.cpp:901 loops j upward)
-      load_v<2, NQ>(d, d.gf1, kn, fn);
-      for (int q = 0; q < NQ; ++q) {
-        double qL = pc[q] + mc[q] * dxmL;
-        double qR = pn[q] + mn[q] * dxmR;
-        pL[q] = qL + fc[q] * dxfL;
-        pR[q] = qR + fn[q] * dxfR;
-      }
-      prim2cons<2, NQ>(pL, cL, xm, geom, G);
-      prim2cons<2, NQ>(pR, cR, xm, geom, G);
-      state2flux<2, NQ>(pL, cL, fL, 1, xm, geom);
-      state2flux<2, NQ>(pR, cR, fR, 1, xm, geom);
+  long long kn = CI(d, tj, lo + n);
+  double xn = d.cx[kn];
+  double xLn = xn - d.cdx[kn] / 2., xRn = xn + d.cdx[kn] / 2.;
+  double xm = (fmax(xL0, xLn) + fmin(xR0, xRn)) / 2.;
+  double ov = fmax(0, fmin(xR0, xRn) - fmax(xL0, xLn));
+  double dA = face_area<2>(1, xm, ov, geom, Tj.s_2, 0.);
+  double pL[NQ], cL[NQ], fL[NQ], pR[NQ], cR[NQ], fR[NQ];
+  if (pc0 || d.nbucnt[kn] == 0) {
+    cell_state<2, NQ>(d, j, i, 1, pL, cL, fL);
+    cell_state<2, NQ>(d, tj, lo + n, 1, pR, cR, fR);
+  } else {
+    Geo gn = geo_of<2>(d, tj, kn);
+    double dxmL = xm - g0.cen0, dxmR = xm - gn.cen0;
+    double dxfL = jpos - g0.cen1, dxfR = jpos - gn.cen1;
+    double pc[NQ], mc[NQ], fc[NQ], pn[NQ], mn[NQ], fn[NQ];
+    load_v<2, NQ>(d, d.prim, k, pc);
+    load_v<2, NQ>(d, d.gmv, k, mc);
+    load_v<2, NQ>(d, d.gf1n, k, fc);
+    load_v<2, NQ>(d, d.prim, kn, pn);
+    load_v<2, NQ>(d, d.gmv, kn, mn);
+    load_v<2, NQ>(d, d.gf1, kn, fn);
+    for (int q = 0; q < NQ; ++q) {
+      double qL = pc[q] + mc[q] * dxmL;
+      double qR = pn[q] + mn[q] * dxmR;
+      pL[q] = qL + fc[q] * dxfL;
+      pR[q] = qR + fn[q] * dxfR;
     }
-    Waves w = lambdas<2, NQ>(pL, cL, pR, cR, 1, xm, geom, G);
-    double fl[NQ];
-    hllc_flux<2, NQ>(pL, cL, fL, pR, cR, fR, w, 0., dA, 1, xm, geom, fl);
-    long long fk = (long long)j * d.fcap + slot;
-    for (int q = 0; q < NQ; ++q) d.ff1[q * d.fs1 + fk] = fl[q];
-    if (d.shock) {
-      double psf = -1.e300, psr = -1.e300;
-      double sdf = shock_sd<2, NQ>(pL, pR, 1, false, G, &psf);
-      double sdr = shock_sd<2, NQ>(pR, pL, 1, true, G, &psr);
-      d.ff1sd[0 * d.fs1 + fk] = sdf; d.ff1sd[1 * d.fs1 + fk] = psf;
-      d.ff1sd[2 * d.fs1 + fk] = sdr; d.ff1sd[3 * d.fs1 + fk] = psr;
-    }
+    prim2cons<2, NQ>(pL, cL, xm, geom, G);
+    prim2cons<2, NQ>(pR, cR, xm, geom, G);
+    state2flux<2, NQ>(pL, cL, fL, 1, xm, geom);
+    state2flux<2, NQ>(pR, cR, fR, 1, xm, geom);
+  }
+  Waves w = lambdas<2, NQ>(pL, cL, pR, cR, 1, xm, geom, G);
+  double fl[NQ];
+  hllc_flux<2, NQ>(pL, cL, fL, pR, cR, fR, w, 0., dA, 1, xm, geom, fl);
+  long long fk = (long long)j * d.fcap + s;
+  for (int q = 0; q < NQ; ++q) d.ff1[q * d.fs1 + fk] = fl[q];
+  if (d.shock) {
+    double psf = -1.e300, psr = -1.e300;
+    double sdf = shock_sd<2, NQ>(pL, pR, 1, false, G, &psf);
+    double sdr = shock_sd<2, NQ>(pR, pL, 1, true, G, &psr);
+    d.ff1sd[0 * d.fs1 + fk] = sdf; d.ff1sd[1 * d.fs1 + fk] = psf;
+    d.ff1sd[2 * d.fs1 + fk] = sdr; d.ff1sd[3 * d.fs1 + fk] = psr;
   }
 }
 
@@ -1005,7 +1016,19 @@ GH_KERNEL2P(k_rg_shockpos, body_rg_shockpos)
 GH_KERNEL2(k_advance_time, body_advance_time)
 template <int ND, int NQ> GH_KERNEL2(k_ghost_mv, (body_ghost_mv<ND, NQ>))
 template <int ND, int NQ> GH_KERNEL2P(k_faces, (body_faces<ND, NQ>))
-template <int NQ> GH_KERNEL2P(k_f1faces, body_f1faces<NQ>)
+template <int NQ> GH_KERNEL2P(k_f1grad, body_f1grad<NQ>)
+// face pass over the F1 slots of each track: thread (j, c*bs + t) of a plain
+// (chunk, track) grid walks slots c*bs + t + m*(chunks*bs)
+// W = minimum wavefronts per SIMD the compiler must allow (registers capped
+// accordingly, spilling if it must): 1 leaves the choice to the compiler, 4
+// trades spills for latency hiding.  The launch option picks one.
+template <int NQ, int W>
+__global__ void __launch_bounds__(KB, W) k_f1faces(Dev d) {
+  int j = blockIdx.y;
+  int stride = gridDim.x * blockDim.x;
+  int tot = min(d.f1tot[j], d.fcap);
+  for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < tot; s += stride) body_f1face<NQ>(d, j, s);
+}
 template <int ND, int NQ, int SV> GH_KERNEL2SP(k_update, (body_update_s<ND, NQ, SV>))
 template <int ND, int NQ> GH_KERNEL2P(k_inject, (body_inject<ND, NQ>))
 template <int ND, int NQ> GH_KERNEL2(k_init_gammas, (body_init_gammas<ND, NQ>))
@@ -1089,7 +1112,11 @@ __global__ void __launch_bounds__(KB) k_f1_scan(Dev d) {
       sm[threadIdx.x] += t;
       __syncthreads();
     }
-    if (i < nt) d.f1off[CI(d, j, i)] = base + sm[threadIdx.x] - v;
+    if (i < nt) {
+      int off = base + sm[threadIdx.x] - v;
+      d.f1off[CI(d, j, i)] = off;
+      for (int n = 0; n < v && off + n < d.fcap; ++n) d.f1own[(long long)j * d.fcap + off + n] = i;
+    }
     int tot = sm[KB - 1];
     __syncthreads();
     base += tot;
@@ -1455,6 +1482,7 @@ struct gammahip_ctx {
   int remap = 0;           // XCD-aware tile order
   int use_graph = 0;       // replay steps from a captured hipGraph
   int persist = 0;         // packed launch: persistent workgroups per CU (0: off)
+  int occ4 = 0;            // F1 face kernel built for 4 wavefronts per SIMD
   int ncu = 256;           // compute units of the device
 #ifndef GH_EMU
   hipStream_t stream = nullptr;
@@ -1612,7 +1640,9 @@ struct Pipe {
         for (int i = 0; i < nt; ++i) {
           long long k = CI(d, j, i);
           d.f1off[k] = base;
-          base += d.nbucnt[k] > 0 ? d.nbucnt[k] : 0;
+          int v = d.nbucnt[k] > 0 ? d.nbucnt[k] : 0;
+          for (int n = 0; n < v && base + n < d.fcap; ++n) d.f1own[(long long)j * d.fcap + base + n] = i;
+          base += v;
         }
         d.f1tot[j] = base;
         if (base > d.fcap) d.cnt[CNT_F1OVF]++;
@@ -1620,8 +1650,21 @@ struct Pipe {
 #else
       hipLaunchKernelGGL(k_f1_scan, dim3(d.nj), dim3(KB), 0, c->stream, d);
 #endif
+      tbeg(c, 3);
+      L2P(c, KSEL(body_f1grad<NQ>, k_f1grad<NQ>), d.nj, d.nax);
+      tend(c, 3);
       tbeg(c, 2);
-      L2P(c, KSEL(body_f1faces<NQ>, k_f1faces<NQ>), d.nj, d.nax);
+#ifdef GH_EMU
+      for (int j = 0; j < d.nj; ++j)
+        for (int sl = 0; sl < d.fcap; ++sl) body_f1face<NQ>(d, j, sl);
+#else
+      {
+        int bs = c->block;
+        dim3 g((d.nax + bs - 1) / bs, d.nj);
+        if (c->occ4) hipLaunchKernelGGL((k_f1faces<NQ, 4>), g, dim3(bs), 0, c->stream, d);
+        else hipLaunchKernelGGL((k_f1faces<NQ, 1>), g, dim3(bs), 0, c->stream, d);
+      }
+#endif
       tend(c, 2);
       if (d.shock) L2P(c, KSEL(body_f1_shock, k_f1_shock), d.nj, d.nax);
       std::swap(d.gf1, d.gf1n);
@@ -1824,6 +1867,7 @@ int gammahip_create(const gammahip_config* cfg, gammahip_ctx** out) {
     e |= dalloc(c, &d.nbdj, cs); e |= dalloc(c, &d.nbdlo, cs); e |= dalloc(c, &d.nbdcnt, cs);
     e |= dalloc(c, &d.nbuj, cs); e |= dalloc(c, &d.nbulo, cs); e |= dalloc(c, &d.nbucnt, cs);
     e |= dalloc(c, &d.f1off, cs); e |= dalloc(c, &d.f1tot, nj); e |= dalloc(c, &d.incf, cs);
+    e |= dalloc(c, &d.f1own, (size_t)nj * d.fcap);
     e |= dalloc(c, &d.ff1, nqs * (size_t)d.fs1);
     if (d.shock) e |= dalloc(c, &d.ff1sd, 4 * (size_t)d.fs1);
   }
@@ -2004,6 +2048,7 @@ int gammahip_set_option(gammahip_ctx* c, int key, long long value) {
       c->block = (int)value;
       build_tiles(c);
       return post(c);
+    case GAMMAHIP_OPT_OCC: c->occ4 = value ? 1 : 0; return 0;
     case GAMMAHIP_OPT_PERSIST:
       if (value < 0 || value > 32) return fail(c, "persistent workgroups per CU must be in 0..32");
       c->persist = (int)value;
@@ -2142,6 +2187,12 @@ int gammahip_download(gammahip_ctx* c, int field, double* out) {
     case 1002: return plane(d.flL);
     case 1003: return plane(d.flR);
     default: break;
+  }
+  if (field == 210 && nd == 2) {       // F1 faces above each cell (its upper-neighbour count)
+    std::vector<int> s(cs);
+    if (d2h(c, s.data(), d.nbucnt, sizeof(int) * cs)) return -1;
+    for (size_t k = 0; k < cs; ++k) out[k] = s[k] > 0 ? s[k] : 0;
+    return 0;
   }
   if (field == 208 && d.shock) {
     std::vector<int> s(cs);

@@ -203,8 +203,10 @@ GH_HD double brent(double D, double S, double E, double G, int geom, double lo, 
 // iterate well inside the reference's 1e-13 tolerance), or once the bracket is
 // that narrow.  Near v = 1 the rounding of f is W^2 larger than its terms
 // suggest and Newton ends up bouncing across the root on noise: once |f| is
-// within that larger bound and a step no longer halves the one before, x is
-// as good as f can tell, and the solve stops there.
+// within that larger bound and a step no longer halves the one before, Newton
+// has nothing more to say.  If the bracket is already tight (1e-10 relative)
+// it is bisected down to 1e-15, so that the answer is the sign change of f
+// whatever the start; otherwise the solve stops at x.
 GH_HD double newton(double D, double S, double E, double G, int geom, double lo, double hi,
                     double x0, int maxit, C2PStat* st) {
   double x = (x0 >= lo && x0 <= hi) ? x0 : 0.5 * (lo + hi);
@@ -220,7 +222,16 @@ GH_HD double newton(double D, double S, double E, double G, int geom, double lo,
     double xn = x - f / df;
     if (!(xn > lo && xn < hi)) xn = 0.5 * (lo + hi);
     double dx = fabs(xn - x);
-    if (it > 1 && fabs(f) <= 4. * eps * fn && dx > 0.5 * dxold) break;
+    if (it > 1 && fabs(f) <= 4. * eps * fn && dx > 0.5 * dxold) {
+      if (hi - lo > 1e-10 * fabs(x)) break;
+      while (hi - lo > 1e-15 * fabs(x) && it < maxit) {
+        it++;
+        double xm = 0.5 * (lo + hi);
+        if (c2p_f(xm, D, S, E, G, geom) > 0) lo = xm; else hi = xm;
+        x = 0.5 * (lo + hi);
+      }
+      break;
+    }
     dxold = dx;
     x = xn;
     if (dx <= 1.e-15 * fabs(x) || hi - lo <= 1.e-15 * fabs(lo)) break;

@@ -19,12 +19,12 @@ from . import setups as S
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libgammahip.so")
 
-F_X, F_DX, F_Y, F_DY, F_DV, F_CEN, F_DTLOC, F_SD, F_SHOCKED, F_PSPEC = range(200, 210)
+F_X, F_DX, F_Y, F_DY, F_DV, F_CEN, F_DTLOC, F_SD, F_SHOCKED, F_PSPEC, F_NF1 = range(200, 211)
 F_GRADMV0, F_GRADF10 = 300, 400
 FACE_X, FACE_V, FACE_LL, FACE_LR, FACE_FLUX0 = 1000, 1001, 1002, 1003, 1010
 RESET_SHOCK, DETECT_SHOCK, INJECT, INIT_GAMMAS, ADVANCE_TIME = range(5)
 C2P_BRENT, C2P_NEWTON = 0, 1
-OPT_BLOCK, OPT_REMAP, OPT_GRAPH, OPT_MLP_THR, OPT_PERSIST = 0, 1, 2, 3, 4
+OPT_BLOCK, OPT_REMAP, OPT_GRAPH, OPT_MLP_THR, OPT_PERSIST, OPT_OCC = 0, 1, 2, 3, 4, 5
 
 
 class Config(C.Structure):
@@ -213,12 +213,13 @@ class Grid:
     def sync(self):
         self._check(self.L.gammahip_sync(self.h))
 
-    def set_options(self, block=None, remap=None, graph=None, mlp_thr=None, persist=None):
+    def set_options(self, block=None, remap=None, graph=None, mlp_thr=None, persist=None, occ=None):
         """Launch options (threads per workgroup, XCD-aware tile order, hipGraph
         replay, trigger exponent n of the learned guess: threshold 10^-n,
-        persistent packed launch with n workgroups per CU or 0)."""
+        persistent packed launch with n workgroups per CU or 0, F1 face kernel
+        capped at 4 wavefronts per SIMD)."""
         for key, val in ((OPT_BLOCK, block), (OPT_REMAP, remap), (OPT_GRAPH, graph),
-                         (OPT_MLP_THR, mlp_thr), (OPT_PERSIST, persist)):
+                         (OPT_MLP_THR, mlp_thr), (OPT_PERSIST, persist), (OPT_OCC, occ)):
             if val is not None:
                 self._check(self.L.gammahip_set_option(self.h, key, int(val)))
 

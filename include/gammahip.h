@@ -108,7 +108,8 @@ int  gammahip_apply(gammahip_ctx* ctx, int what);              /* Grid::apply(&C
 /* Synchronous downloads (Grid::printCols reads the same state).  field:
  * 0..nq-1 primitive q, 100+q conserved q (prim2cons of the stored state),
  * 200 x, 201 dx, 202 y, 203 dy, 204 dV, 205 centroid, 206 dt_loc, 207 Sd,
- * 208 isShocked, 209 pspec, 300+q grad_mv, 400+q grad_f1;
+ * 208 isShocked, 209 pspec, 210 F1 faces above the cell (2D), 300+q grad_mv,
+ * 400+q grad_f1;
  * face fields: 1000 x, 1001 v, 1002 lL, 1003 lR, 1010+q flux. */
 int  gammahip_download(gammahip_ctx* ctx, int field, double* out /* [ntracks][nax] */);
 int  gammahip_track_info(gammahip_ctx* ctx, int* ntrack, int* nact, int* iLbnd, int* iRbnd);
@@ -131,9 +132,11 @@ int  gammahip_sync(gammahip_ctx* ctx);
  * (default 3), and the packed launch: n > 0 runs the per-cell kernels as
  * persistent grids of n workgroups per CU walking a list of (track, chunk)
  * tiles that covers only each track's live cells (uneven tracks, ensembles),
- * 0 launches one workgroup per chunk of the nmax-wide track slots. */
+ * 0 launches one workgroup per chunk of the nmax-wide track slots; OCC 1
+ * runs the F1 face kernel in a build capped at 128 registers (4 wavefronts
+ * per SIMD, spilling what does not fit). */
 enum { GAMMAHIP_OPT_BLOCK = 0, GAMMAHIP_OPT_REMAP = 1, GAMMAHIP_OPT_GRAPH = 2,
-       GAMMAHIP_OPT_MLP_THR = 3, GAMMAHIP_OPT_PERSIST = 4 };
+       GAMMAHIP_OPT_MLP_THR = 3, GAMMAHIP_OPT_PERSIST = 4, GAMMAHIP_OPT_OCC = 5 };
 int  gammahip_set_option(gammahip_ctx* ctx, int key, long long value);
 
 /* ---- learned cons2prim guess (no counterpart in the reference) ----

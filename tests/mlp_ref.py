@@ -90,13 +90,16 @@ def y_of_pressure(w, states, p):
 
 def c2p_residual(p, D, S, tau, G):
     """Cartesian cons2prim residual f(p) (src/Hydro/rel_cart.cpp:108-123), its
-    derivative and the magnitude sum of its terms (its rounding scale)."""
+    derivative and its rounding scale: the magnitudes of its terms, those
+    holding the Lorentz factor W weighted by 1 + W^2 (W is formed from
+    1 - S^2/(E+p)^2, a cancellation that costs a factor W^2 in precision)."""
     E = D + tau
     Ep = E + p
     lf = 1.0 / np.sqrt(1.0 - S * S / (Ep * Ep))
     gg = G / (G - 1.0)
     f = E + p - D * lf - gg * p * lf * lf
-    scale = np.abs(E) + np.abs(p) + np.abs(D * lf) + np.abs(gg * p * lf * lf)
+    tw = np.abs(D * lf) + np.abs(gg * p * lf * lf)
+    scale = np.abs(E) + np.abs(p) + (1.0 + lf * lf) * tw
     dlf = -lf ** 3 * S * S / Ep ** 3
     df = 1.0 - D * dlf - gg * (lf * lf + 2.0 * p * lf * dlf)
     return f, df, scale

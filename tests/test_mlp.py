@@ -147,7 +147,7 @@ def test_training_converges_and_gains_over_cold_start():
     eps = np.finfo(np.float64).eps
     for pp in (pm, pc):
         f, _, fs = R.c2p_residual(pp, D, S, tau, G53)
-        assert np.all(np.abs(f) <= 64 * eps * fs)          # a root to working precision
+        assert np.all(np.abs(f) <= 16 * eps * fs)          # a root to working precision
     # both starts reach the same root, to the conditioning of the state
     _, df, fs = R.c2p_residual(pc, D, S, tau, G53)
     cond = fs / np.abs(pc * df)

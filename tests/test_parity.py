@@ -40,7 +40,8 @@ def run_pair(setup, nsteps, lib_path, solver=hip.C2P_BRENT):
     return o, g
 
 
-def check_pair(o, g, shock=False, exact_expected=False):
+def check_pair(o, g, shock=False, exact_expected=False, bars=None):
+    """bars: per-field L1 bars replacing TOL (see gpu_bars)."""
     out = {}
     to, tg = o.tracks(), g.tracks()
     assert np.array_equal(to["ntrack"], tg["ntrack"]), "track lengths differ"
@@ -51,7 +52,8 @@ def check_pair(o, g, shock=False, exact_expected=False):
         a, b = g.active(f), o.active(f)
         err = l1rel(a, b)
         out[name] = err
-        assert err <= TOL, f"{name}: L1-relative error {err:.3e} > {TOL}"
+        bar = TOL if bars is None else bars[name]
+        assert err <= bar, f"{name}: L1-relative error {err:.3e} > {bar:.3e}"
         out[name + "_bitwise"] = all(np.array_equal(x, y) for x, y in zip(a, b))
     if shock:
         a, b = g.active(hip.F_SHOCKED), o.active(oc.F_SHOCKED)
@@ -74,6 +76,25 @@ def test_emu_parity(case, emu_lib):
     check_pair(o, g, shock=bool(s.cfg.shock), exact_expected=True)
 
 
+# Cases whose device arithmetic is +, -, *, /, sqrt only (cartesian, no
+# synchrotron, no analytic boundary): IEEE-exact on both sides, so the GPU must
+# reproduce the oracle bit for bit.  The others also call pow (synchrotron,
+# shock detector, BM inflow, BM regrid rule), where the device math library and
+# the C library may round differently in the last place; the first such
+# difference then grows through the Brent iterations like any perturbation of
+# the reference's own 1e-13 root tolerance (reference_spread below), so their
+# bar is max(1e-10, twice that spread), with identical lengths, regrid counts
+# and shock flags still required.
+GPU_EXACT = {"shock_tube_cart", "riemann_2d", "expanding_tube"}
+
+
+def gpu_bars(case, setup, nsteps):
+    if case in GPU_EXACT:
+        return None
+    sp = reference_spread(setup, nsteps)
+    return {k: max(TOL, 2. * v) for k, v in sp.items()}
+
+
 @pytest.mark.gpu
 @pytest.mark.parametrize("case", list(CASES))
 def test_gpu_parity(case):
@@ -81,7 +102,8 @@ def test_gpu_parity(case):
     s = mk()
     print(f"[gpu parity] case={case} steps={n} nd={s.cfg.nd} cells={s.x.size}", flush=True)
     o, g = run_pair(s, n, hip.LIB_PATH)
-    res = check_pair(o, g, shock=bool(s.cfg.shock))
+    res = check_pair(o, g, shock=bool(s.cfg.shock), exact_expected=case in GPU_EXACT,
+                     bars=gpu_bars(case, s, n))
     print(f"[gpu parity] {case}: {res}", flush=True)
 
 
@@ -202,8 +224,8 @@ def test_gpu_parity_ensemble():
 @pytest.mark.parametrize("opts", [
     dict(block=128, remap=1, graph=1, persist=0),
     dict(block=256, remap=0, graph=0, persist=4),
-    dict(block=64, remap=1, graph=1, persist=8),
-], ids=["remap-graph", "packed", "packed-remap-graph"])
+    dict(block=64, remap=1, graph=1, persist=8, occ=1),
+], ids=["remap-graph", "packed", "packed-remap-graph-occ4"])
 def test_gpu_parity_launch_options(case, opts):
     """Launch options change where work runs, never the arithmetic: the
     trajectory under every option matches the oracle as the default launch does."""
@@ -215,5 +237,6 @@ def test_gpu_parity_launch_options(case, opts):
     g = hip.Grid(s)
     g.set_options(**opts)
     g.step(n)
-    res = check_pair(o, g, shock=bool(s.cfg.shock))
+    res = check_pair(o, g, shock=bool(s.cfg.shock), exact_expected=case in GPU_EXACT,
+                     bars=gpu_bars(case, s, n))
     print(f"[gpu parity options] {case} {opts}: {res}", flush=True)
