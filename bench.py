@@ -138,17 +138,22 @@ def run_other(setup, steps, warmup, options=None, solver=None):
 def other_configs(steps, warmup):
     """The uneven-track configurations: the 2D spherical jet blast wave (1024
     theta tracks, seeded lengths 500-4000) and the ensemble of 256 1D runs
-    (512-16384 cells), each plain and batched (persistent packed launch,
-    XCD-aware tiles, hipGraph replay, Newton cons2prim)."""
-    from gamma_amd import hip, setups
+    (512-16384 cells), each plain and batched (Newton cons2prim, launch
+    options -- packed persistent grids, XCD order, graphs, block size --
+    picked by the autotuner on a scratch copy)."""
+    from gamma_amd import autotune, hip, setups
     out = {}
-    batched = dict(block=256, remap=1, graph=1, persist=8)
     for name, mk in (("bm_jet_2d", lambda: setups.bm_jet_2d()),
                      ("ensemble_1d", lambda: setups.ensemble_1d()[0])):
         s = mk()
         print(f"[bench] {name}: plain", file=sys.stderr, flush=True)
         p = run_other(s, steps, warmup)
-        print(f"[bench] {name}: batched", file=sys.stderr, flush=True)
+        print(f"[bench] {name}: autotuning and batched", file=sys.stderr, flush=True)
+        scratch = hip.Grid(s, solver=hip.C2P_NEWTON)
+        scratch.step(warmup)
+        tuned = autotune.tune(scratch, steps=4, settle=2, sweeps=1)
+        scratch.close()
+        batched = {k: tuned["best"][k] for k in ("block", "remap", "graph", "persist", "occ")}
         b = run_other(s, steps, warmup, batched, hip.C2P_NEWTON)
         out[name] = {"plain": p, "batched": dict(b, options=batched),
                      "speedup": b["value"] / p["value"]}
@@ -214,7 +219,7 @@ def main():
         print("[bench] autotuning the batched form", file=sys.stderr, flush=True)
         tuned = autotune.tune(scratch, surrogate=w)
         scratch.close()
-        opts = {k: tuned["best"][k] for k in ("block", "remap", "graph", "mlp_thr")}
+        opts = {k: tuned["best"][k] for k in ("block", "remap", "graph", "persist", "occ", "mlp_thr")}
         b = run_form(s, hip.C2P_NEWTON, opts, args.steps, args.warmup, args.timing_steps,
                      surrogate=w if opts["mlp_thr"] is not None else None)
         print(f"[bench] batched {b['ms_per_step']:.3f} ms/step with {opts}", file=sys.stderr, flush=True)

@@ -1,28 +1,32 @@
 """Autotuner for the launch options of the HIP backend.
 
-Candidates: threads per workgroup of the per-cell kernels (64, 128, 256),
-XCD-aware tile order (off/on), two-step hipGraph replay (off/on) and, when a
-trained network is given, the learned cons2prim guess (off, or on with
-trigger exponents 2 and 4).  Each candidate is timed on the live grid (the
-production state, not a toy): a short untimed run to settle (and capture the
-graph), then `steps` timed steps between stream synchronisations.  The grid
-keeps evolving during tuning, so candidates are interleaved over two rounds
-and each scored by its best round.  The winner is applied to the grid and
-returned with the full table.
+Options: threads per workgroup of the per-cell kernels (64, 128, 256),
+XCD-aware tile order (off/on), two-step hipGraph replay (off/on), persistent
+packed launch (off, 4 or 8 workgroups per CU), the 4-wavefront build of the F1
+face kernel (off/on) and, when a trained network is given, the learned
+cons2prim start (off, or on with trigger exponents 2 and 4).
+
+Search: coordinate descent from the defaults -- each option in turn is set to
+each of its values with the others held, the fastest kept -- repeated for
+`sweeps` passes.  Each evaluation runs on the live grid (the production state,
+not a toy): a short untimed run to settle (and capture the graph), then
+`steps` timed steps between stream synchronisations.  The grid keeps evolving
+while tuning, so the incumbent is re-timed with every challenger and the
+comparison made within that pair.  The winner is applied to the grid and
+returned with every evaluation made.
 """
 from __future__ import annotations
 
-import itertools
 import time
 
-
-def candidates(graph_options=(0, 1), mlp_options=(None,)):
-    for block, remap, graph, mlp in itertools.product((64, 128, 256), (0, 1), graph_options, mlp_options):
-        yield {"block": block, "remap": remap, "graph": graph, "mlp_thr": mlp}
+SPACE = (("block", (256, 128, 64)), ("remap", (0, 1)), ("graph", (0, 1)),
+         ("persist", (0, 4, 8)), ("occ", (0, 1)))
+DEFAULTS = {"block": 256, "remap": 0, "graph": 0, "persist": 0, "occ": 0, "mlp_thr": None}
 
 
 def apply(grid, cand: dict, surrogate=None):
-    grid.set_options(block=cand["block"], remap=cand["remap"], graph=cand["graph"])
+    grid.set_options(block=cand["block"], remap=cand["remap"], graph=cand["graph"],
+                     persist=cand["persist"], occ=cand["occ"])
     if cand.get("mlp_thr") is None:
         if surrogate is not None:
             grid.load_surrogate(None)
@@ -31,21 +35,31 @@ def apply(grid, cand: dict, surrogate=None):
         grid.set_options(mlp_thr=cand["mlp_thr"])
 
 
-def tune(grid, steps: int = 6, settle: int = 2, rounds: int = 2, graph_options=(0, 1),
-         surrogate=None, mlp_options=(None, 2, 4)) -> dict:
-    cands = list(candidates(graph_options, mlp_options if surrogate is not None else (None,)))
-    best_ms = {i: float("inf") for i in range(len(cands))}
-    for _ in range(rounds):
-        for i, c in enumerate(cands):
-            apply(grid, c, surrogate)
-            grid.step(settle)
-            grid.sync()
-            t0 = time.perf_counter()
-            grid.step(steps)
-            grid.sync()
-            ms = 1e3 * (time.perf_counter() - t0) / steps
-            best_ms[i] = min(best_ms[i], ms)
-    win = min(best_ms, key=best_ms.get)
-    apply(grid, cands[win], surrogate)
-    table = [dict(cands[i], ms_per_step=round(best_ms[i], 4)) for i in range(len(cands))]
-    return {"best": dict(cands[win], ms_per_step=round(best_ms[win], 4)), "table": table}
+def _time(grid, cand, surrogate, steps, settle):
+    apply(grid, cand, surrogate)
+    grid.step(settle)
+    grid.sync()
+    t0 = time.perf_counter()
+    grid.step(steps)
+    grid.sync()
+    return 1e3 * (time.perf_counter() - t0) / steps
+
+
+def tune(grid, steps: int = 6, settle: int = 2, sweeps: int = 2, surrogate=None,
+         mlp_options=(None, 2, 4)) -> dict:
+    space = list(SPACE) + ([("mlp_thr", mlp_options)] if surrogate is not None else [])
+    best = dict(DEFAULTS)
+    table = []
+    for _ in range(sweeps):
+        for key, values in space:
+            for v in values:
+                if v == best[key]:
+                    continue
+                cand = dict(best, **{key: v})
+                t_inc = _time(grid, best, surrogate, steps, settle)
+                t_new = _time(grid, cand, surrogate, steps, settle)
+                table.append(dict(cand, ms_per_step=round(t_new, 4), incumbent_ms=round(t_inc, 4)))
+                if t_new < t_inc:
+                    best = cand
+    ms = _time(grid, best, surrogate, steps, settle)
+    return {"best": dict(best, ms_per_step=round(ms, 4)), "table": table}

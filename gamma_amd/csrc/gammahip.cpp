@@ -734,8 +734,8 @@ GH_HD int rg_check(const Dev& d, int j, int i, double iS) {
     double p[NQ];
     load_v<ND, NQ>(d, d.prim, k, p);
     double lf = lfac<ND, NQ>(p);
-    if (ar > d.rg[0] * target / pow(lf, 3. / 2.)) return RG_SPLIT;
-    if (ar < d.rg[1] * target / pow(lf, 3. / 2.)) return RG_MERGE;
+    if (ar > d.rg[0] * target / gh_pow(lf, 3. / 2.)) return RG_SPLIT;
+    if (ar < d.rg[1] * target / gh_pow(lf, 3. / 2.)) return RG_MERGE;
     return RG_SKIP;
   }
   return RG_SKIP;
@@ -1443,6 +1443,13 @@ __global__ void __launch_bounds__(KB) k_update_mlp(Dev d, int ni, int remap, int
   int wv = threadIdx.x / 64;
   MlpGuess g{d.mw, (const bf16x8*)d.mw2b, lds[wv], rows[wv], d.mlp_thr, d.P.gamma};
   GH_TILE_LOOP(body_update_g<ND, NQ, C2P_NEWTON>(d, j, i, stage, i < ni, g))
+}
+
+// x^y as the kernels evaluate it (gh_pow), for numerics checks
+__global__ void k_pow_batch(const double* __restrict__ x, const double* __restrict__ y,
+                            double* __restrict__ out, long long n) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = gh_pow(x[i], y[i]);
 }
 
 // lane-layout probe of v_mfma_f32_32x32x16_bf16: D = A B for row-major A (32x16), B (16x32)
@@ -2369,6 +2376,24 @@ int gammahip_c2p_batch(const double* states, const double* guess, const double* 
   CK(hipMemcpy(iters, it, sizeof(int) * n, hipMemcpyDeviceToHost));
   CK(hipMemcpy(fail, fl, sizeof(int) * n, hipMemcpyDeviceToHost));
   if (used) CK(hipMemcpy(used, us, sizeof(int) * n, hipMemcpyDeviceToHost));
+  return 0;
+#endif
+}
+
+int gammahip_pow_batch(const double* x, const double* y, double* out, long long n) {
+  if (n < 1) return -1;
+#ifdef GH_EMU
+  for (long long i = 0; i < n; ++i) out[i] = gh_pow(x[i], y[i]);
+  return 0;
+#else
+  DevBuf B;
+  double *dx = B.get<double>(n), *dy = B.get<double>(n), *dz = B.get<double>(n);
+  if (!dx || !dy || !dz) return plain_fail("allocation failed");
+  CK(hipMemcpy(dx, x, sizeof(double) * n, hipMemcpyHostToDevice));
+  CK(hipMemcpy(dy, y, sizeof(double) * n, hipMemcpyHostToDevice));
+  hipLaunchKernelGGL(k_pow_batch, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, 0, dx, dy, dz, n);
+  CK(hipGetLastError());
+  CK(hipMemcpy(out, dz, sizeof(double) * n, hipMemcpyDeviceToHost));
   return 0;
 #endif
 }

@@ -20,6 +20,7 @@
 #include <hip/hip_runtime.h>
 #endif
 #include <math.h>
+#include "ddpow.hpp"
 
 #define GH_HD __host__ __device__ __forceinline__
 
@@ -452,7 +453,7 @@ GH_HD double shock_sd(const double* S1, const double* S2, int dim, bool reverse,
   double I = 0;
   for (int ip = 0; ip < n_evals; ++ip) {
     double p = p1 + (ip + .5) * dp;
-    double rho = rho1 * pow(p / p1, 1. / G);
+    double rho = rho1 * gh_pow(p / p1, 1. / G);
     double h = 1 + p * G / (G - 1.) / rho;
     double cs = sqrt(G * p / (rho + p * G / (G - 1.)));
     double dI = sqrt(h * h + A1 * A1 * (1. - cs * cs)) / ((h * h + A1 * A1) * rho * cs) * dp;
@@ -572,7 +573,7 @@ GH_HD void radiative(const double* prim, double* U, double dV, double dt, const 
   double eps = rho * (h - 1.) / gma;
   double eB = kEpsB * eps;
   double B = sqrt(8. * kPi * eB);
-  double dgma = P.Nalpha * pow(rho, 4. / 3.) * B * B * dV * dt;
+  double dgma = P.Nalpha * gh_pow(rho, 4. / 3.) * B * B * dV * dt;
   U[GMX] += dgma;
   U[GMN] += dgma;
 }
@@ -581,7 +582,7 @@ GH_HD void radiative(const double* prim, double* U, double dV, double dt, const 
 template <int ND, int NQ>
 GH_HD void inject(double* prim, bool shocked, double pspec, const Phys& P) {
   double lf = lfac<ND, NQ>(prim), rho = prim[RHO];
-  double r43 = pow(rho, 4. / 3.);
+  double r43 = gh_pow(rho, 4. / 3.);
   double lim = lf * r43 / (lf * rho);
   if (shocked) {
     if (pspec > prim[PSN] || isnan(prim[PSN])) prim[PSN] = pspec;
@@ -604,7 +605,7 @@ GH_HD void inject(double* prim, bool shocked, double pspec, const Phys& P) {
 template <int ND, int NQ>
 GH_HD void init_gammas(double* prim) {
   double lf = lfac<ND, NQ>(prim), rho = prim[RHO];
-  double lim = lf * pow(rho, 4. / 3.);
+  double lim = lf * gh_pow(rho, 4. / 3.);
   if (prim[GMX] > lim / (lf * rho) || prim[GMX] <= 0.) prim[GMX] = lim / (lf * rho);
   if (prim[GMN] > lim / (lf * rho) || prim[GMN] <= 0.) prim[GMN] = lim / (lf * rho);
 }
@@ -622,9 +623,9 @@ GH_HD void bm_profile(const double* bm, double r, double t, double G, double* rh
   double lfacf = sqrt(fmax(1., .5 * L));
   double Df = 2. * L * rhoa;
   double chi = (1. + 8. * L) * (1. - r / t);
-  *p = pf * pow(chi, -17. / 12.);
+  *p = pf * gh_pow(chi, -17. / 12.);
   double lf = sqrt(lfacf * lfacf / chi + 1);
-  double Dn = Df * pow(chi, -7. / 4.);
+  double Dn = Df * gh_pow(chi, -7. / 4.);
   *rho = Dn / lf;
   *u = sqrt(1. - 1. / (lf * lf)) * lf;
 }

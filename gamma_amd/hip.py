@@ -86,6 +86,7 @@ def load_library(path: str = LIB_PATH):
         "gammahip_mfma_probe": (I, [P, P, P]),
         "gammahip_c2p_batch": (I, [P, P, P, C.c_longlong, D, I, I, P, P, P, P]),
         "gammahip_mlp_load": (I, [P, P]),
+        "gammahip_pow_batch": (I, [P, P, P, C.c_longlong]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)

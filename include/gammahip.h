@@ -161,6 +161,10 @@ int  gammahip_mfma_probe(const float* A /* 32x16 */, const float* B /* 16x32 */,
  * used[s] = 1 when guess[s] won.  iters[s] counts residual evaluations
  * (root-finder iterations plus bracketing), fail[s] the failure bits
  * (1 no bracket, 2 iteration cap, 4 bracket-growth cap). */
+/* x^y exactly as the kernels evaluate it (double-double exp(y log x) from
+ * IEEE-exact operations, identical bits on host and device; no counterpart
+ * in the reference, which calls the C library's pow), for numerics checks. */
+int  gammahip_pow_batch(const double* x, const double* y, double* out, long long n);
 /* Seed the context's Newton cons2prim with the network (params = null
  * switches it off).  The previous pressure stays the judge: each solve starts
  * from whichever start has the smaller residual; stats.mlp_used counts the

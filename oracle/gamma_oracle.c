@@ -22,6 +22,7 @@
  * reference's own tolerance.
  */
 #include "gamma_oracle.h"
+#include "og_pow.h"
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
@@ -487,7 +488,7 @@ static double compute_sd(const og_config *c, const FS *S1, const FS *S2, int dim
   double I = 0;
   for (int ip = 0; ip < n_evals; ++ip) {
     double p = p1 + (ip + .5) * dp;
-    double rho = rho1 * pow(p / p1, 1. / G);
+    double rho = rho1 * og_pow(p / p1, 1. / G);
     double h = 1 + p * G / (G - 1.) / rho;
     double cs = compute_cs(G, rho, p);
     double dI = sqrt(h * h + A1 * A1 * (1. - cs * cs)) / ((h * h + A1 * A1) * rho * cs) * dp;
@@ -535,7 +536,7 @@ static void face_measure_shock(const og_config *c, const Face *f, Cell *cL, Cell
 
 static double gammae2trac(const og_config *c, double gme, const FS *S) {   /* :196-215 */
   double lfac = fs_lfac(c, S), rho = S->prim[OG_RHO];
-  if (gme != 0.) return (lfac * pow(rho, 4. / 3.) / gme);
+  if (gme != 0.) return (lfac * og_pow(rho, 4. / 3.) / gme);
   return 0;
 }
 
@@ -553,7 +554,7 @@ static double gamma_min_init(const og_config *c, const FS *S) {           /* :21
 static void cell_init_gammas(const og_config *c, Cell *cl) {              /* :246-260 */
   FS *S = &cl->S;
   double lfac = fs_lfac(c, S), rho = S->prim[OG_RHO];
-  double lim = lfac * pow(rho, 4. / 3.);
+  double lim = lfac * og_pow(rho, 4. / 3.);
   if (S->prim[OG_GMX] > lim / (lfac * rho) || S->prim[OG_GMX] <= 0.) {
     S->prim[OG_GMX] = lim / (lfac * rho); S->cons[OG_GMX] = lim;
   }
@@ -565,7 +566,7 @@ static void cell_init_gammas(const og_config *c, Cell *cl) {              /* :24
 static void cell_inject(const og_config *c, Cell *cl) {                    /* :262-283 */
   FS *S = &cl->S;
   double lfac = fs_lfac(c, S), rho = S->prim[OG_RHO];
-  double lim = lfac * pow(rho, 4. / 3.) / (lfac * rho);
+  double lim = lfac * og_pow(rho, 4. / 3.) / (lfac * rho);
   if (cl->isShocked) {
     if (cl->pspec > S->prim[OG_PSN] || isnan(S->prim[OG_PSN])) S->prim[OG_PSN] = cl->pspec;
     S->prim[OG_GMX] = gammae2trac(c, c->gamma_max_init, S) / (lfac * rho);
@@ -582,7 +583,7 @@ static void cell_radiative(const og_config *c, Cell *cl, double dt) {      /* :2
   double eps = rho * (h - 1.) / gma;
   double eB = EPS_B * eps;
   double B = sqrt(8. * M_PI * eB);
-  double dgma = c->Nalpha * pow(rho, 4. / 3.) * B * B * cl->G.dV * dt;
+  double dgma = c->Nalpha * og_pow(rho, 4. / 3.) * B * B * cl->G.dV * dt;
   S->cons[OG_GMX] += dgma;
   S->cons[OG_GMN] += dgma;
 }
@@ -682,9 +683,9 @@ static void bm_profile(const og_config *c, const double *bm, double r, double t,
   double lfacf = sqrt(fmax(1., .5 * lfacShock2));
   double Df = 2. * lfacShock2 * rhoa;
   double chi = (1. + 8. * lfacShock2) * (1. - r / t);
-  *p = pf * pow(chi, -17. / 12.);
+  *p = pf * og_pow(chi, -17. / 12.);
   double lfac = sqrt(lfacf * lfacf / chi + 1);
-  double D = Df * pow(chi, -7. / 4.);
+  double D = Df * og_pow(chi, -7. / 4.);
   *rho = D / lfac;
   *u = sqrt(1. - 1. / (lfac * lfac)) * lfac;
 }
@@ -1203,8 +1204,8 @@ static int regrid_check(og_grid *g, int j, int i, double *iS_cache, int *iS_vali
     double dist = i - *iS_cache;
     if (0 < dist && dist < 10) ar *= 10;
     double lfac = fs_lfac(c, &cl->S);
-    if (ar > c->rg[0] * target / pow(lfac, 3. / 2.)) return SPLIT;
-    if (ar < c->rg[1] * target / pow(lfac, 3. / 2.)) return MERGE;
+    if (ar > c->rg[0] * target / og_pow(lfac, 3. / 2.)) return SPLIT;
+    if (ar < c->rg[1] * target / og_pow(lfac, 3. / 2.)) return MERGE;
     return SKIP;
   }
   return SKIP;
@@ -1523,4 +1524,9 @@ void og_inject(const og_config *c, double *prim, int shocked, double pspec) {
   cl.isShocked = shocked; cl.pspec = pspec;
   cell_inject(c, &cl);
   memcpy(prim, cl.S.prim, sizeof(double) * c->nq);
+}
+
+/* x^y as every kernel of the oracle evaluates it (og_pow.h), for tests */
+void og_pow_batch(const double *x, const double *y, double *out, long n) {
+  for (long i = 0; i < n; ++i) out[i] = og_pow(x[i], y[i]);
 }

@@ -66,6 +66,7 @@ void     og_destroy(og_grid *g);
  * prim [nq][ncell_f1][ncell_mv] with 3-velocities in the UU slots. */
 int      og_set_track_cells(og_grid *g, const int *ncells);
 int      og_set_track_bm(og_grid *g, const double *bm);
+void     og_pow_batch(const double *x, const double *y, double *out, long n);
 int      og_load(og_grid *g, const double *x, const double *dx,
                  const double *y, const double *dy, const double *prim);
 /* prepForRun + radiation_initGammas + the first updateGhosts of Simu::run */

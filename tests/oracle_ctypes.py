@@ -65,6 +65,7 @@ def lib():
             "og_create": (P, [cfgp]), "og_destroy": (V, [P]),
             "og_load": (I, [P, P, P, P, P, P]), "og_init": (I, [P, D]),
             "og_set_track_cells": (I, [P, P]), "og_set_track_bm": (I, [P, P]),
+            "og_pow_batch": (V, [P, P, P, C.c_long]),
             "og_run": (I, [P, I]), "og_ntracks": (I, [P]), "og_nax": (I, [P]),
             "og_track_info": (V, [P, P, P, P, P]), "og_time": (D, [P]),
             "og_iter": (C.c_long, [P]), "og_last_dt": (D, [P]),

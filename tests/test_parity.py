@@ -76,16 +76,15 @@ def test_emu_parity(case, emu_lib):
     check_pair(o, g, shock=bool(s.cfg.shock), exact_expected=True)
 
 
-# Cases whose device arithmetic is +, -, *, /, sqrt only (cartesian, no
-# synchrotron, no analytic boundary): IEEE-exact on both sides, so the GPU must
-# reproduce the oracle bit for bit.  The others also call pow (synchrotron,
-# shock detector, BM inflow, BM regrid rule), where the device math library and
-# the C library may round differently in the last place; the first such
-# difference then grows through the Brent iterations like any perturbation of
-# the reference's own 1e-13 root tolerance (reference_spread below), so their
-# bar is max(1e-10, twice that spread), with identical lengths, regrid counts
-# and shock flags still required.
-GPU_EXACT = {"shock_tube_cart", "riemann_2d", "expanding_tube"}
+# Every kernel evaluates +, -, *, /, sqrt and fma (IEEE-exact, identical on
+# both sides), powers through the double-double gh_pow that the oracle
+# restates operation for operation, and the sines and cosines of 2D spherical
+# tracks from a table computed on the host with the C library the oracle uses.
+# So the GPU must reproduce the oracle bit for bit on every case.  gpu_bars
+# keeps the looser bar -- max(1e-10, twice the reference's own root-tolerance
+# spread) -- for a case taken out of this set, should a device build ever
+# round differently.
+GPU_EXACT = set(CASES)
 
 
 def gpu_bars(case, setup, nsteps):
@@ -217,6 +216,26 @@ def test_gpu_parity_ensemble():
     print("[gpu parity] case=ensemble_1d", flush=True)
     res = _ensemble_check(hip.LIB_PATH)
     print(f"[gpu parity] ensemble_1d: {res}", flush=True)
+    assert all(bit for _, _, bit in res), res
+
+
+@pytest.mark.gpu
+def test_gpu_pow_equals_oracle_bitwise():
+    """The device's gh_pow returns the oracle's bits (the premise of bitwise
+    parity on the cases with synchrotron, shock detection and BM hooks)."""
+    import ctypes as C
+    rng = np.random.default_rng(3)
+    n = 1 << 20
+    x = np.exp(rng.uniform(np.log(1e-300), np.log(1e300), n))
+    y = rng.choice([4. / 3., -17. / 12., -7. / 4., 1.5, 0.75, 0.6, 1. / 3., -0.5], n)
+    x[:6] = [0., 0., 1., 1e-310, 5e-324, 2.5]
+    y[:6] = [1.5, -1.5, 0.7, 0.75, 0.75, 0.]
+    out, ref = np.zeros(n), np.zeros(n)
+    p = lambda a: a.ctypes.data_as(C.c_void_p)
+    assert hip.load_library().gammahip_pow_batch(p(x), p(y), p(out), n) == 0
+    oc.lib().og_pow_batch(p(x), p(y), p(ref), n)
+    same = (out == ref) | (np.isnan(out) & np.isnan(ref))
+    assert same.all(), (x[~same][:5], y[~same][:5], out[~same][:5], ref[~same][:5])
 
 
 @pytest.mark.gpu
