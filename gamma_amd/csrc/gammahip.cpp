@@ -62,9 +62,14 @@ struct Dev {
   int *shk, *nbdj, *nbdlo, *nbdcnt, *nbuj, *nbulo, *nbucnt, *f1off, *f1tot;
   int *f1own;              // [nj][fcap] cell owning each F1 face slot
   int *incf;               // first fresh lower-track cell whose F1 list holds this cell
+  int *dncnt, *dnoff;      // fresh lower-track sources of each cell, their offset in the track's down slots
   // faces
   double *fx, *fxn, *fx0, *fv, *flL, *flR, *fflux;
   double *ff1, *ff1sd;
+  // copies of the F1 face fluxes (and shock values) ordered by upper cell:
+  // the faces a cell receives from fresh lists of the track below sit in
+  // down slots dnoff.. of its own track, in the order of their source cells
+  double *ff1dn, *ff1sddn;
   unsigned long long* cnt;
   long long *nsplit, *nmerge;
   // learned cons2prim guess: FP32 parameters and the packed BF16 hidden-layer
@@ -198,9 +203,10 @@ GH_HD void body_faces(const Dev& d, int j, int i) {
   int geom = d.tgeom[j];
   double G = d.P.gamma;
   long long k = CI(d, j, i);
-  if (ND == 2) {               // reset before the neighbour pass fills it
+  if (ND == 2) {               // reset before the neighbour pass fills them
     d.incf[k] = INT_MAX;
-    if (i == nt - 2) d.incf[k + 1] = INT_MAX;
+    d.dncnt[k] = 0;
+    if (i == nt - 2) { d.incf[k + 1] = INT_MAX; d.dncnt[k + 1] = 0; }
   }
   double xf = d.fx[k];
   double pL[NQ], cL[NQ], fL[NQ], pR[NQ], cR[NQ], fR[NQ];
@@ -341,7 +347,10 @@ GH_HD void body_nb(const Dev& d, int j, int i) {
   lo = 0; cnt = 0;
   if (j != d.nj - 1) nb_side(d, j, i, j + 1, &lo, &cnt);
   d.nbuj[k] = j + 1; d.nbulo[k] = lo; d.nbucnt[k] = cnt;
-  for (int t = lo; t < lo + cnt; ++t) GH_ATOMIC_MIN(&d.incf[CI(d, j + 1, t)], i);
+  for (int t = lo; t < lo + cnt; ++t) {
+    GH_ATOMIC_MIN(&d.incf[CI(d, j + 1, t)], i);
+    GH_ATOMIC_ADD(&d.dncnt[CI(d, j + 1, t)], 1);
+  }
 }
 
 // Grid::computeTransGradients (2d.cpp:366-430) of cell (j,i) into gf
@@ -465,19 +474,33 @@ GH_HD void body_f1face(const Dev& d, int j, int s) {
   hllc_flux<2, NQ>(pL, cL, fL, pR, cR, fR, w, 0., dA, 1, xm, geom, fl);
   long long fk = (long long)j * d.fcap + s;
   for (int q = 0; q < NQ; ++q) d.ff1[q * d.fs1 + fk] = fl[q];
+  // fresh lists (cells 1..nt-2): also the upper cell's down slot
+  long long fd = -1;
+  if (i >= 1 && i <= d.ntrack[j] - 2) {
+    long long ku = CI(d, tj, lo + n);
+    long long slot = d.dnoff[ku] + (i - d.incf[ku]);
+    if (slot < d.fcap) fd = (long long)tj * d.fcap + slot;
+  }
+  if (fd >= 0)
+    for (int q = 0; q < NQ; ++q) d.ff1dn[q * d.fs1 + fd] = fl[q];
   if (d.shock) {
     double psf = -1.e300, psr = -1.e300;
     double sdf = shock_sd<2, NQ>(pL, pR, 1, false, G, &psf);
     double sdr = shock_sd<2, NQ>(pR, pL, 1, true, G, &psr);
     d.ff1sd[0 * d.fs1 + fk] = sdf; d.ff1sd[1 * d.fs1 + fk] = psf;
     d.ff1sd[2 * d.fs1 + fk] = sdr; d.ff1sd[3 * d.fs1 + fk] = psr;
+    if (fd >= 0) { d.ff1sddn[fd] = sdr; d.ff1sddn[d.fs1 + fd] = psr; }
   }
 }
 
 // Visit, in the reference's accumulation order (track j' ascending, cell i'
-// ascending, 2d.cpp:901-906), every F1 face whose upper cell is (J, i).
-// Fresh lists of track J-1 are non-decreasing ranges (binary search); the
-// stale lists that ghost copies carry on cells 0 and nt-1 are checked one by one.
+// ascending, 2d.cpp:901-906), every F1 face whose upper cell is (J, i):
+// fn(false, fk) for a face in the lower cell's slots of ff1 / ff1sd, fn(true,
+// fk) for a face in (J, i)'s own down slots of ff1dn / ff1sddn.  Fresh lists
+// of track J-1 are non-decreasing ranges, so the sources of (J, i) there are
+// consecutive cells and their faces sit in consecutive down slots (filled by
+// the face pass); the stale lists that ghost copies carry on cells 0 and nt-1
+// are checked one by one.
 template <class F>
 GH_HD void for_incoming_f1(const Dev& d, int J, int i, F&& fn) {
   auto stale = [&](int jp, int ip) {
@@ -485,7 +508,7 @@ GH_HD void for_incoming_f1(const Dev& d, int J, int i, F&& fn) {
     int c = d.nbucnt[kp], l = d.nbulo[kp];
     if (d.nbuj[kp] == J && c > 0 && l <= i && i < l + c) {
       long long slot = d.f1off[kp] + (i - l);
-      if (slot < d.fcap) fn((long long)jp * d.fcap + slot);
+      if (slot < d.fcap) fn(false, (long long)jp * d.fcap + slot);
     }
   };
   // ghost tracks below the domain carry stale lists copied from track jL+1,
@@ -500,18 +523,9 @@ GH_HD void for_incoming_f1(const Dev& d, int J, int i, F&& fn) {
     if (jp < 0 || jp > d.nj - 2) return;
     int ntp = d.ntrack[jp];
     stale(jp, 0);
-    // fresh lists of track J-1 are non-decreasing ranges: the sources of
-    // (J,i) are consecutive cells starting at the first one (filled by k_nb)
-    int a = d.incf[CI(d, J, i)];
-    for (int ip = a; ip <= ntp - 2; ++ip) {
-      long long kp = CI(d, jp, ip);
-      int l = d.nbulo[kp], c = d.nbucnt[kp];
-      if (l > i) break;
-      if (c > 0 && i < l + c) {
-        long long slot = d.f1off[kp] + (i - l);
-        if (slot < d.fcap) fn((long long)jp * d.fcap + slot);
-      }
-    }
+    long long k = CI(d, J, i);
+    int c = d.dncnt[k], off = d.dnoff[k];
+    for (int m = 0; m < c && off + m < d.fcap; ++m) fn(true, (long long)J * d.fcap + off + m);
     stale(jp, ntp - 1);
   }
 }
@@ -530,9 +544,11 @@ GH_HD void body_f1_shock(const Dev& d, int j, int i) {
       if (d.ff1sd[d.fs1 + fk] > ps) ps = d.ff1sd[d.fs1 + fk];
     }
   }
-  for_incoming_f1(d, j, i, [&](long long fk) {
-    Sd = fmax(Sd, d.ff1sd[2 * d.fs1 + fk]);
-    if (d.ff1sd[3 * d.fs1 + fk] > ps) ps = d.ff1sd[3 * d.fs1 + fk];
+  for_incoming_f1(d, j, i, [&](bool dn, long long fk) {
+    double sd = dn ? d.ff1sddn[fk] : d.ff1sd[2 * d.fs1 + fk];
+    double pv = dn ? d.ff1sddn[d.fs1 + fk] : d.ff1sd[3 * d.fs1 + fk];
+    Sd = fmax(Sd, sd);
+    if (pv > ps) ps = pv;
   });
   d.sd[k] = Sd;
   d.pspec[k] = ps;
@@ -612,8 +628,9 @@ GH_HD void body_update_g(const Dev& d, int j, int i, int stage, bool valid, cons
         long long fk = (long long)j * d.fcap + off + n;
         for (int q = 0; q < NQ; ++q) up[q] += d.ff1[q * d.fs1 + fk];
       }
-      for_incoming_f1(d, j, i, [&](long long fk) {
-        for (int q = 0; q < NQ; ++q) dn[q] += d.ff1[q * d.fs1 + fk];
+      for_incoming_f1(d, j, i, [&](bool down, long long fk) {
+        const double* src = down ? d.ff1dn : d.ff1;
+        for (int q = 0; q < NQ; ++q) dn[q] += src[q * d.fs1 + fk];
       });
       for (int q = 0; q < NQ; ++q) U[q] += (dn[q] - up[q]) * dt;
     }
@@ -1124,6 +1141,24 @@ __global__ void __launch_bounds__(KB) k_f1_scan(Dev d) {
   if (threadIdx.x == 0) {
     d.f1tot[j] = base;
     if (base > d.fcap) atomicAdd(&d.cnt[CNT_F1OVF], 1ull);
+  }
+  __syncthreads();
+  base = 0;                    // down slots: the same scan over the fresh-source counts
+  for (int i0 = 0; i0 < nt; i0 += KB) {
+    int i = i0 + threadIdx.x;
+    int v = i < nt ? d.dncnt[CI(d, j, i)] : 0;
+    sm[threadIdx.x] = v;
+    __syncthreads();
+    for (int s = 1; s < KB; s <<= 1) {
+      int t = (int)threadIdx.x >= s ? sm[threadIdx.x - s] : 0;
+      __syncthreads();
+      sm[threadIdx.x] += t;
+      __syncthreads();
+    }
+    if (i < nt) d.dnoff[CI(d, j, i)] = base + sm[threadIdx.x] - v;
+    int tot = sm[KB - 1];
+    __syncthreads();
+    base += tot;
   }
 }
 // The applyRegrid loop of one track on one workgroup.  The decisions and
@@ -1653,6 +1688,12 @@ struct Pipe {
         }
         d.f1tot[j] = base;
         if (base > d.fcap) d.cnt[CNT_F1OVF]++;
+        base = 0;
+        for (int i = 0; i < nt; ++i) {
+          long long k = CI(d, j, i);
+          d.dnoff[k] = base;
+          base += d.dncnt[k];
+        }
       }
 #else
       hipLaunchKernelGGL(k_f1_scan, dim3(d.nj), dim3(KB), 0, c->stream, d);
@@ -1877,6 +1918,9 @@ int gammahip_create(const gammahip_config* cfg, gammahip_ctx** out) {
     e |= dalloc(c, &d.f1own, (size_t)nj * d.fcap);
     e |= dalloc(c, &d.ff1, nqs * (size_t)d.fs1);
     if (d.shock) e |= dalloc(c, &d.ff1sd, 4 * (size_t)d.fs1);
+    e |= dalloc(c, &d.ff1dn, nqs * (size_t)d.fs1);
+    if (d.shock) e |= dalloc(c, &d.ff1sddn, 2 * (size_t)d.fs1);
+    e |= dalloc(c, &d.dncnt, cs); e |= dalloc(c, &d.dnoff, cs);
   }
   if (d.shock) { e |= dalloc(c, &d.sd, cs); e |= dalloc(c, &d.pspec, cs); e |= dalloc(c, &d.shk, cs); }
   if (e) return -1;
