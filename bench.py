@@ -111,10 +111,12 @@ def track_cells(g):
     return int(nact[rc.ngst:-rc.ngst].sum()) if rc.nd == 2 else int(nact.sum())
 
 
-def run_other(setup, steps, warmup, options=None, solver=None):
+def run_other(setup, steps, warmup, options=None, solver=None, surrogate=None):
     """cell-updates/s of a secondary configuration (no per-kernel timing)."""
     from gamma_amd import hip
     g = hip.Grid(setup, solver=hip.C2P_BRENT if solver is None else solver)
+    if surrogate is not None:
+        g.load_surrogate(surrogate)
     if options:
         g.set_options(**options)
     g.step(warmup)
@@ -131,6 +133,7 @@ def run_other(setup, steps, warmup, options=None, solver=None):
     return {"value": 0.5 * (n0 + n1) * steps / el, "ms_per_step": 1e3 * el / steps, "cells": n1,
             "track_cells_min": int(lens.min()) - 4, "track_cells_max": int(lens.max()) - 4,
             "c2p_iters_per_call": st["c2p_iters"] / max(st["c2p_calls"], 1),
+            "network_starts": st["mlp_used"],
             "failures": {k: st[k] for k in ("c2p_nobracket", "c2p_maxiter", "nan_flux", "negative_dt")},
             "splits": st["splits"], "merges": st["merges"]}
 
@@ -151,13 +154,36 @@ def other_configs(steps, warmup):
         print(f"[bench] {name}: autotuning and batched", file=sys.stderr, flush=True)
         scratch = hip.Grid(s, solver=hip.C2P_NEWTON)
         scratch.step(warmup)
-        tuned = autotune.tune(scratch, steps=4, settle=2, sweeps=1)
+        from gamma_amd import surrogate as SG
+        w, _ = SG.train(np.concatenate([SG.states_from_grid(scratch, s.cfg.gamma, max_states=1 << 20),
+                                        SG.synthetic_states(1 << 20, s.cfg.gamma, seed=3)]),
+                        steps=3000, batch=4096, lr=2e-3, seed=5)
+        tuned = autotune.tune(scratch, steps=4, settle=2, sweeps=1, surrogate=w)
         scratch.close()
-        batched = {k: tuned["best"][k] for k in ("block", "remap", "graph", "persist", "occ")}
-        b = run_other(s, steps, warmup, batched, hip.C2P_NEWTON)
+        batched = {k: tuned["best"][k] for k in ("block", "remap", "graph", "persist", "occ", "mlp_thr")}
+        b = run_other(s, steps, warmup, batched, hip.C2P_NEWTON,
+                      surrogate=w if batched["mlp_thr"] is not None else None)
         out[name] = {"plain": p, "batched": dict(b, options=batched),
                      "speedup": b["value"] / p["value"]}
     return out
+
+
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "riemann_2d_2048_hbm_traffic.json")
+
+
+def measured_traffic(kernel: str):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC
+    summary of this workload (FETCH_SIZE and WRITE_SIZE, each scaled by what
+    the counter reads for a copy kernel of known size), or None."""
+    try:
+        import json as _j
+        t = _j.load(open(TRAFFIC_FILE))
+    except (OSError, ValueError):
+        return None
+    for name, v in t.get("kernels", {}).items():
+        if name.split("<")[0] == kernel:
+            return v["bytes_per_launch"]
+    return None
 
 
 def kernel_table(res, nq):
@@ -201,8 +227,9 @@ def main():
     dom = max(kern, key=lambda k: kern[k]["ms"] * kern[k]["launches"])
     ach = kern[dom]["GBs"]
     roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
-            "kernel_ms": round(kern[dom]["ms"], 4)}
+            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": measured_traffic(dom), "kernel": dom,
+            "kernel_ms": round(kern[dom]["ms"], 4),
+            "algorithmic_bytes": kern[dom]["bytes"]}
 
     batched = None
     if not args.no_batched:

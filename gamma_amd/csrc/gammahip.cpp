@@ -1161,13 +1161,14 @@ __global__ void __launch_bounds__(KB) k_f1_scan(Dev d) {
     base += tot;
   }
 }
-// The applyRegrid loop of one track on one workgroup.  The decisions and
-// the arithmetic of each split/merge stay on thread 0, in the sequential
-// order; the O(ntrack) work per action -- moving the cells after it one slot
-// and locating the shock -- is spread over the workgroup.  Moves go in chunks
-// of RGB cells, each chunk read into registers before any of it is written,
-// chunks taken from the far end towards the action, which is the order in
-// which the sequential loops copy.
+// The applyRegrid loop of one track on one workgroup, in the sequential
+// order.  The rule is evaluated for RGB cells at once (nothing it reads
+// changes before the next action), the arithmetic of each split/merge runs on
+// one lane, and the O(ntrack) work per action -- moving the cells after it
+// one slot and locating the shock -- is spread over the workgroup.  Moves go
+// in chunks of RGB cells, each chunk read into registers before any of it is
+// written, chunks taken from the far end towards the action, which is the
+// order in which the sequential loops copy.
 constexpr int RGB = 256;
 template <int ND, int NQ>
 struct CellRec {
@@ -1240,7 +1241,7 @@ __device__ double wg_shock_index(const Dev& d, int j, int* red) {
 template <int ND, int NQ>
 __global__ void __launch_bounds__(RGB) k_rg_apply(Dev d) {
   __shared__ int red[RGB];
-  __shared__ int s_act;
+  __shared__ int s_act, s_pos;
   int j = blockIdx.x;
   if (!active_track(d, j)) return;
   int i = d.tfirst[j];
@@ -1248,14 +1249,31 @@ __global__ void __launch_bounds__(RGB) k_rg_apply(Dev d) {
   bool bm = d.trg[j] == RG_BM1D;
   long long b = CI(d, j, 0);
   double iS = bm ? wg_shock_index(d, j, red) : 0.;
-  for (;; ++i) {
+  for (;;) {
     int nt = d.ntrack[j];
-    if (i > nt - d.ngst - 1) break;
-    if (threadIdx.x == 0) {
-      int a = rg_check<ND, NQ>(d, j, i, iS);
-      s_act = (a == RG_SPLIT && nt < d.nax - 1) ? 1 : (a == RG_MERGE && nt > 2 * d.ngst + 2) ? 2 : 0;
+    int last = nt - d.ngst - 1;
+    if (i > last) break;
+    // The rule of a cell reads that cell, the track's last active cell and the
+    // shock index, none of which change until an action is taken: evaluate
+    // the next RGB cells at once and take the first action among them, which
+    // is what the sequential loop would reach next.
+    int m = i + (int)threadIdx.x;
+    int a = 0;
+    if (m <= last) {
+      int r = rg_check<ND, NQ>(d, j, m, iS);
+      a = (r == RG_SPLIT && nt < d.nax - 1) ? 1 : (r == RG_MERGE && nt > 2 * d.ngst + 2) ? 2 : 0;
     }
+    red[threadIdx.x] = a ? (int)threadIdx.x : RGB;
     __syncthreads();
+    for (int w = RGB / 2; w > 0; w >>= 1) {
+      if ((int)threadIdx.x < w && red[threadIdx.x + w] < red[threadIdx.x]) red[threadIdx.x] = red[threadIdx.x + w];
+      __syncthreads();
+    }
+    int first = red[0];
+    if (first < RGB && (int)threadIdx.x == first) s_act = a;
+    __syncthreads();
+    if (first == RGB) { i += RGB; continue; }
+    i += first;
     int act = s_act;
     __syncthreads();
     if (act == 1) {
@@ -1267,7 +1285,7 @@ __global__ void __launch_bounds__(RGB) k_rg_apply(Dev d) {
       }
       __syncthreads();
       if (bm) iS = wg_shock_index(d, j, red);
-    } else if (act == 2) {
+    } else {
       if (threadIdx.x == 0) s_act = merge_fill<ND, NQ>(d, j, i) ? 1 : 0;
       __syncthreads();
       bool left = s_act == 1;
@@ -1280,6 +1298,7 @@ __global__ void __launch_bounds__(RGB) k_rg_apply(Dev d) {
       __syncthreads();
       if (bm) iS = wg_shock_index(d, j, red);
     }
+    ++i;
   }
 }
 }  // namespace

@@ -204,8 +204,8 @@ GH_HD double brent(double D, double S, double E, double G, int geom, double lo, 
 // iterate well inside the reference's 1e-13 tolerance), or once the bracket is
 // that narrow.  Near v = 1 the rounding of f is W^2 larger than its terms
 // suggest and Newton ends up bouncing across the root on noise: once |f| is
-// within that larger bound and a step no longer halves the one before, Newton
-// has nothing more to say.  If the bracket is already tight (1e-10 relative)
+// within that larger bound and a step, already below 1e-11 relative, no
+// longer halves the one before, Newton has nothing more to say.  If the bracket is already tight (1e-10 relative)
 // it is bisected down to 1e-15, so that the answer is the sign change of f
 // whatever the start; otherwise the solve stops at x.
 GH_HD double newton(double D, double S, double E, double G, int geom, double lo, double hi,
@@ -223,7 +223,7 @@ GH_HD double newton(double D, double S, double E, double G, int geom, double lo,
     double xn = x - f / df;
     if (!(xn > lo && xn < hi)) xn = 0.5 * (lo + hi);
     double dx = fabs(xn - x);
-    if (it > 1 && fabs(f) <= 4. * eps * fn && dx > 0.5 * dxold) {
+    if (it > 1 && fabs(f) <= 4. * eps * fn && dx > 0.5 * dxold && dx <= 1e-11 * fabs(x)) {
       if (hi - lo > 1e-10 * fabs(x)) break;
       while (hi - lo > 1e-15 * fabs(x) && it < maxit) {
         it++;
