@@ -2,8 +2,9 @@
 
 Options: threads per workgroup of the per-cell kernels (64, 128, 256),
 XCD-aware tile order (off/on), two-step hipGraph replay (off/on), persistent
-packed launch (off, 4 or 8 workgroups per CU), the 4-wavefront build of the F1
-face kernel (off/on) and, when a trained network is given, the learned
+packed launch (off, 4 or 8 workgroups per CU), the register-capped builds of
+the F1 face kernel and of the update kernel (bit set, 0-3) and, when a trained
+network is given, the learned
 cons2prim start (off, or on with trigger exponents 2 and 4).
 
 Search: coordinate descent from the defaults -- each option in turn is set to
@@ -20,7 +21,7 @@ from __future__ import annotations
 import time
 
 SPACE = (("block", (256, 128, 64)), ("remap", (0, 1)), ("graph", (0, 1)),
-         ("persist", (0, 4, 8)), ("occ", (0, 1)))
+         ("persist", (0, 4, 8)), ("occ", (0, 1, 2, 3)))
 DEFAULTS = {"block": 256, "remap": 0, "graph": 0, "persist": 0, "occ": 0, "mlp_thr": None}
 
 

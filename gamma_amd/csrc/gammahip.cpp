@@ -1046,7 +1046,12 @@ __global__ void __launch_bounds__(KB, W) k_f1faces(Dev d) {
   int tot = min(d.f1tot[j], d.fcap);
   for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < tot; s += stride) body_f1face<NQ>(d, j, s);
 }
-template <int ND, int NQ, int SV> GH_KERNEL2SP(k_update, (body_update_s<ND, NQ, SV>))
+// W: minimum wavefronts per SIMD (1 = the compiler's choice; 5 caps the
+// registers at 96 and spills the rest), picked by the launch option
+template <int ND, int NQ, int SV, int W>
+__global__ void __launch_bounds__(KB, W) k_update(Dev d, int ni, int remap, int stage) {
+  GH_TILE_LOOP(if (i < ni && j < d.nj) body_update_s<ND, NQ, SV>(d, j, i, stage))
+}
 template <int ND, int NQ> GH_KERNEL2P(k_inject, (body_inject<ND, NQ>))
 template <int ND, int NQ> GH_KERNEL2(k_init_gammas, (body_init_gammas<ND, NQ>))
 template <int ND, int NQ> GH_KERNEL2P(k_rg_screen, (body_rg_screen<ND, NQ>))
@@ -1543,7 +1548,7 @@ struct gammahip_ctx {
   int remap = 0;           // XCD-aware tile order
   int use_graph = 0;       // replay steps from a captured hipGraph
   int persist = 0;         // packed launch: persistent workgroups per CU (0: off)
-  int occ4 = 0;            // F1 face kernel built for 4 wavefronts per SIMD
+  int occ = 0;             // bit 0: F1 face kernel built for 4 wavefronts per SIMD; bit 1: update kernel for 5
   int ncu = 256;           // compute units of the device
 #ifndef GH_EMU
   hipStream_t stream = nullptr;
@@ -1728,7 +1733,7 @@ struct Pipe {
       {
         int bs = c->block;
         dim3 g((d.nax + bs - 1) / bs, d.nj);
-        if (c->occ4) hipLaunchKernelGGL((k_f1faces<NQ, 4>), g, dim3(bs), 0, c->stream, d);
+        if (c->occ & 1) hipLaunchKernelGGL((k_f1faces<NQ, 4>), g, dim3(bs), 0, c->stream, d);
         else hipLaunchKernelGGL((k_f1faces<NQ, 1>), g, dim3(bs), 0, c->stream, d);
       }
 #endif
@@ -1764,8 +1769,13 @@ struct Pipe {
     L2S(c, (body_update<ND, NQ>), d.nj, d.nax, stage);
 #else
     if (d.mw) L2SP(c, (k_update_mlp<ND, NQ>), d.nj, d.nax, stage);
-    else if (d.solver == C2P_NEWTON) L2SP(c, (k_update<ND, NQ, C2P_NEWTON>), d.nj, d.nax, stage);
-    else L2SP(c, (k_update<ND, NQ, C2P_BRENT>), d.nj, d.nax, stage);
+    else if (d.solver == C2P_NEWTON) {
+      if (c->occ & 2) L2SP(c, (k_update<ND, NQ, C2P_NEWTON, 5>), d.nj, d.nax, stage);
+      else L2SP(c, (k_update<ND, NQ, C2P_NEWTON, 1>), d.nj, d.nax, stage);
+    } else {
+      if (c->occ & 2) L2SP(c, (k_update<ND, NQ, C2P_BRENT, 5>), d.nj, d.nax, stage);
+      else L2SP(c, (k_update<ND, NQ, C2P_BRENT, 1>), d.nj, d.nax, stage);
+    }
 #endif
     tend(c, 0);
     std::swap(d.fx, d.fxn);
@@ -2118,7 +2128,9 @@ int gammahip_set_option(gammahip_ctx* c, int key, long long value) {
       c->block = (int)value;
       build_tiles(c);
       return post(c);
-    case GAMMAHIP_OPT_OCC: c->occ4 = value ? 1 : 0; return 0;
+    case GAMMAHIP_OPT_OCC:
+      if (value < 0 || value > 3) return fail(c, "occupancy option must be in 0..3");
+      c->occ = (int)value; return 0;
     case GAMMAHIP_OPT_PERSIST:
       if (value < 0 || value > 32) return fail(c, "persistent workgroups per CU must be in 0..32");
       c->persist = (int)value;

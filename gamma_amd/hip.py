@@ -217,8 +217,8 @@ class Grid:
     def set_options(self, block=None, remap=None, graph=None, mlp_thr=None, persist=None, occ=None):
         """Launch options (threads per workgroup, XCD-aware tile order, hipGraph
         replay, trigger exponent n of the learned guess: threshold 10^-n,
-        persistent packed launch with n workgroups per CU or 0, F1 face kernel
-        capped at 4 wavefronts per SIMD)."""
+        persistent packed launch with n workgroups per CU or 0, register-capped
+        builds: bit 0 the F1 face kernel, bit 1 the update kernel)."""
         for key, val in ((OPT_BLOCK, block), (OPT_REMAP, remap), (OPT_GRAPH, graph),
                          (OPT_MLP_THR, mlp_thr), (OPT_PERSIST, persist), (OPT_OCC, occ)):
             if val is not None:

@@ -132,9 +132,10 @@ int  gammahip_sync(gammahip_ctx* ctx);
  * (default 3), and the packed launch: n > 0 runs the per-cell kernels as
  * persistent grids of n workgroups per CU walking a list of (track, chunk)
  * tiles that covers only each track's live cells (uneven tracks, ensembles),
- * 0 launches one workgroup per chunk of the nmax-wide track slots; OCC 1
+ * 0 launches one workgroup per chunk of the nmax-wide track slots; OCC bit 0
  * runs the F1 face kernel in a build capped at 128 registers (4 wavefronts
- * per SIMD, spilling what does not fit). */
+ * per SIMD), bit 1 the update kernel in one capped at 96 (5 wavefronts),
+ * spilling what does not fit. */
 enum { GAMMAHIP_OPT_BLOCK = 0, GAMMAHIP_OPT_REMAP = 1, GAMMAHIP_OPT_GRAPH = 2,
        GAMMAHIP_OPT_MLP_THR = 3, GAMMAHIP_OPT_PERSIST = 4, GAMMAHIP_OPT_OCC = 5 };
 int  gammahip_set_option(gammahip_ctx* ctx, int key, long long value);
