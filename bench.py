@@ -26,7 +26,7 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0      # MI355X HBM3E, GB/s
 
 # HIP-event timed kernels (ids of gammahip_kernel_times)
-KNAMES = {0: "k_update", 1: "k_faces", 2: "k_f1faces", 3: "k_f1grad"}
+KNAMES = {0: "k_update", 1: "k_faces", 2: "k_f1faces", 3: "k_f1grad", 4: "k_f1sum"}
 
 
 def algorithmic_bytes(kid: int, cells: float, f1faces: float, nq: int, stage_mix: float) -> float:
@@ -36,10 +36,10 @@ def algorithmic_bytes(kid: int, cells: float, f1faces: float, nq: int, stage_mix
     that are RK stages 2/3 (which also read the RK base)."""
     d, i4 = 8.0, 4.0
     if kid == 0:   # k_update: prim, x, dx, face x, v, flux in; prim, x, dx, face x out;
-        # RK base (U0, dV0, x0) written in stage 1 / read in stages 2-3; F1 face
-        # fluxes read by both cells of each F1 face; F1 slot offsets and counts
+        # RK base (U0, dV0, x0) written in stage 1 / read in stages 2-3; net F1 flux in (2D)
         per = (nq + 2 + 2 + nq) * d + (nq + 3) * d + (nq + 2) * d
-        per += 2 * (f1faces / cells) * nq * d + 4 * i4
+        if f1faces > 0:
+            per += nq * d
         return cells * per
     if kid == 1:   # k_faces: prim, centroid inputs (x, dx), face x in; v, lL, lR, flux, grad_mv out
         return cells * ((nq + 2 + 1) * d + (3 + nq + nq) * d)
@@ -50,6 +50,9 @@ def algorithmic_bytes(kid: int, cells: float, f1faces: float, nq: int, stage_mix
     if kid == 3:   # k_f1grad: prim, grad_mv, x, dx and the neighbour lists of each cell
         # (neighbours' values shared between cells), grad_f1 out
         return cells * ((3 * nq + 2) * d + 6 * i4)
+    if kid == 4:   # k_f1sum: each F1 face flux read twice (lower cell's slot, upper
+        # cell's down slot), slot counts/offsets in, net flux out
+        return cells * (nq * d + 4 * i4) + 2 * f1faces * nq * d
     return 0.0
 
 

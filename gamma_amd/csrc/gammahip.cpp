@@ -70,6 +70,7 @@ struct Dev {
   // the faces a cell receives from fresh lists of the track below sit in
   // down slots dnoff.. of its own track, in the order of their source cells
   double *ff1dn, *ff1sddn;
+  double *f1net;           // [NQ][nj][nax] net F1 flux (dn - up) of each cell, this stage
   unsigned long long* cnt;
   long long *nsplit, *nmerge;
   // learned cons2prim guess: FP32 parameters and the packed BF16 hidden-layer
@@ -554,6 +555,31 @@ GH_HD void body_f1_shock(const Dev& d, int j, int i) {
   d.pspec[k] = ps;
 }
 
+// Net F1 flux of a cell that the update will advance (2d.cpp:974-1036: the
+// faces of its own list, then the incoming ones in the reference's order),
+// as dn - up, ahead of the update kernel: the gathers over other cells' slots
+// run here, in a light kernel, not in the update's register-heavy one.
+template <int NQ>
+GH_HD void body_f1sum(const Dev& d, int j, int i) {
+  if (!active_track(d, j)) return;
+  int nt = d.ntrack[j];
+  int iL = d.ngst - 1, iR = nt - d.ngst;
+  if (i < iL + 1 || i > iR - 1) return;
+  long long k = CI(d, j, i);
+  double up[NQ], dn[NQ];
+  for (int q = 0; q < NQ; ++q) { up[q] = 0.; dn[q] = 0.; }
+  int cnt = d.nbucnt[k], off = d.f1off[k];
+  for (int n = 0; n < cnt && off + n < d.fcap; ++n) {
+    long long fk = (long long)j * d.fcap + off + n;
+    for (int q = 0; q < NQ; ++q) up[q] += d.ff1[q * d.fs1 + fk];
+  }
+  for_incoming_f1(d, j, i, [&](bool down, long long fk) {
+    const double* src = down ? d.ff1dn : d.ff1;
+    for (int q = 0; q < NQ; ++q) dn[q] += src[q * d.fs1 + fk];
+  });
+  for (int q = 0; q < NQ; ++q) d.f1net[q * d.cs + k] = dn[q] - up[q];
+}
+
 // ---------------------------------------------------------------------------
 // Starting-pressure provider of the update's cons2prim calls: NoGuess keeps
 // the previous pressure alone (the reference's rule).  A provider is called by
@@ -620,20 +646,8 @@ GH_HD void body_update_g(const Dev& d, int j, int i, int stage, bool valid, cons
       load_v<ND, NQ>(d, d.fflux, k, fb);
       for (int q = 0; q < NQ; ++q) U[q] += (fa[q] - fb[q]) * dt;
     }
-    if (ND == 2) {
-      double up[NQ], dn[NQ];
-      for (int q = 0; q < NQ; ++q) { up[q] = 0.; dn[q] = 0.; }
-      int cnt = d.nbucnt[k], off = d.f1off[k];
-      for (int n = 0; n < cnt && off + n < d.fcap; ++n) {
-        long long fk = (long long)j * d.fcap + off + n;
-        for (int q = 0; q < NQ; ++q) up[q] += d.ff1[q * d.fs1 + fk];
-      }
-      for_incoming_f1(d, j, i, [&](bool down, long long fk) {
-        const double* src = down ? d.ff1dn : d.ff1;
-        for (int q = 0; q < NQ; ++q) dn[q] += src[q * d.fs1 + fk];
-      });
-      for (int q = 0; q < NQ; ++q) U[q] += (dn[q] - up[q]) * dt;
-    }
+    if (ND == 2)   // net F1 flux, summed by body_f1sum
+      for (int q = 0; q < NQ; ++q) U[q] += d.f1net[q * d.cs + k] * dt;
     sources<ND, NQ>(p, U, d.cx[k], d.cdx[k], trig_at<ND>(d, j), g.dV, dt, geom, G);
     if (NQ == 10 && d.sync) radiative(p, U, g.dV, dt, d.P);
     cxn = (xnR + xnL) / 2.;
@@ -1034,6 +1048,7 @@ GH_KERNEL2(k_advance_time, body_advance_time)
 template <int ND, int NQ> GH_KERNEL2(k_ghost_mv, (body_ghost_mv<ND, NQ>))
 template <int ND, int NQ> GH_KERNEL2P(k_faces, (body_faces<ND, NQ>))
 template <int NQ> GH_KERNEL2P(k_f1grad, body_f1grad<NQ>)
+template <int NQ> GH_KERNEL2P(k_f1sum, body_f1sum<NQ>)
 // face pass over the F1 slots of each track: thread (j, c*bs + t) of a plain
 // (chunk, track) grid walks slots c*bs + t + m*(chunks*bs)
 // W = minimum wavefronts per SIMD the compiler must allow (registers capped
@@ -1739,6 +1754,9 @@ struct Pipe {
 #endif
       tend(c, 2);
       if (d.shock) L2P(c, KSEL(body_f1_shock, k_f1_shock), d.nj, d.nax);
+      tbeg(c, 4);
+      L2P(c, KSEL(body_f1sum<NQ>, k_f1sum<NQ>), d.nj, d.nax);
+      tend(c, 4);
       std::swap(d.gf1, d.gf1n);
     }
   }
@@ -1948,6 +1966,7 @@ int gammahip_create(const gammahip_config* cfg, gammahip_ctx** out) {
     e |= dalloc(c, &d.ff1, nqs * (size_t)d.fs1);
     if (d.shock) e |= dalloc(c, &d.ff1sd, 4 * (size_t)d.fs1);
     e |= dalloc(c, &d.ff1dn, nqs * (size_t)d.fs1);
+    e |= dalloc(c, &d.f1net, nqs * cs);
     if (d.shock) e |= dalloc(c, &d.ff1sddn, 2 * (size_t)d.fs1);
     e |= dalloc(c, &d.dncnt, cs); e |= dalloc(c, &d.dnoff, cs);
   }
