@@ -8,6 +8,9 @@ next to their sources, git-ignored but shipped with the tree).
 * ``oracle/libgamma_oracle.so`` -- the CPU oracle (test infrastructure).
 * ``gamma_amd/copy_probe`` -- a plain copy kernel of known size, the
   calibration point for the HBM traffic counters.
+* ``gamma_amd/host/shock_tube`` (and ``tests/shock_tube_emu``) -- a C++
+  caller of the hot path through ``gamma_amd/host/gammahip_grid.hpp``, linked
+  against the HIP library (and against the CPU build, for tests).
 """
 from __future__ import annotations
 
@@ -67,11 +70,24 @@ def build_copy_probe(out: str | None = None) -> str:
     return out
 
 
+def build_host() -> list:
+    src = os.path.join(HERE, "host", "shock_tube.cpp")
+    outs = []
+    for libdir, lib, out in ((HERE, "gammahip", os.path.join(HERE, "host", "shock_tube")),
+                             (os.path.join(ROOT, "tests"), "gammahip_emu", os.path.join(ROOT, "tests", "shock_tube_emu"))):
+        rpath = os.path.relpath(libdir, os.path.dirname(out))
+        _run(["g++", "-O2", "-std=c++17", src, "-L" + libdir, "-l" + lib,
+              "-Wl,-rpath,$ORIGIN/" + rpath, "-o", out])
+        outs.append(out)
+    return outs
+
+
 def build_all() -> None:
     build_hip()
     build_emu()
     build_oracle()
     build_copy_probe()
+    build_host()
 
 
 if __name__ == "__main__":
