@@ -214,6 +214,8 @@ def main():
     ap.add_argument("--no-other", action="store_true",
                     help="skip the uneven-track configurations")
     ap.add_argument("--other-steps", type=int, default=100)
+    ap.add_argument("--profile-config", choices=("bm_jet_2d", "ensemble_1d"), default=None,
+                    help="run only that configuration's plain form for --steps steps (for profilers)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; multi-GPU sharding is a "
@@ -221,6 +223,11 @@ def main():
         sys.exit(2)
 
     from gamma_amd import autotune, hip, setups
+    if args.profile_config:
+        s = setups.bm_jet_2d() if args.profile_config == "bm_jet_2d" else setups.ensemble_1d()[0]
+        print(json.dumps({"profile_config": args.profile_config,
+                          "plain": run_other(s, args.steps, args.warmup)}))
+        return
     s = setups.riemann_2d(args.nx, args.ny)
     nq = s.cfg.nq
     print(f"[bench] riemann_2d {args.nx}x{args.ny}: plain form", file=sys.stderr, flush=True)
