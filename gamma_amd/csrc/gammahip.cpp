@@ -41,6 +41,13 @@ namespace {
 
 enum { CNT_C2P_CALLS = 0, CNT_C2P_ITERS, CNT_NOBRACKET, CNT_MAXITER, CNT_PHICAP, CNT_NEGDT,
        CNT_NANFLUX, CNT_F1OVF, CNT_SPLITS, CNT_MERGES, CNT_MLP, CNT_N = 16 };
+// Counters live in CNT_PARTS copies of one 128-byte line each; a workgroup adds
+// into copy (linear workgroup id mod CNT_PARTS) and the host sums the copies.
+// Device-wide atomics to one address serialise at the memory side (the XCDs
+// have separate L2s), so a single copy would cost every kernel that counts on
+// each wavefront (the update counts every cons2prim) a queue of ~10^5
+// same-address atomics per launch.
+constexpr int CNT_PARTS = 1024;
 enum { KIN_NONE = 0, KIN_FIXED = 1 };
 enum { BND_NONE = 0, BND_BM = 1 };
 enum { RG_NONE = 0, RG_AR2D = 1, RG_BM1D = 2 };
@@ -87,6 +94,16 @@ struct Dev {
 };
 
 GH_HD long long CI(const Dev& d, int j, int i) { return (long long)j * d.nax + i; }
+
+// this workgroup's copy of counter id
+GH_HD unsigned long long* cnt_at(const Dev& d, int id) {
+#ifdef GH_EMU
+  return d.cnt + id;
+#else
+  int part = (int)((blockIdx.x + gridDim.x * blockIdx.y) & (CNT_PARTS - 1));
+  return d.cnt + (long long)part * CNT_N + id;
+#endif
+}
 
 template <int ND>
 GH_HD int sq(int q) { return (ND == 1 && q > 3) ? q - 1 : q; }
@@ -253,7 +270,7 @@ GH_HD void body_faces(const Dev& d, int j, int i) {
   double dA = face_area<ND>(0, xf, d.tdy[j], geom, T.s_y, T.s_hdy);
   double fl[NQ];
   hllc_flux<ND, NQ>(pL, cL, fL, pR, cR, fR, w, v, dA, 0, xf, geom, fl);
-  if (fl[0] != fl[0]) GH_ATOMIC_ADD(&d.cnt[CNT_NANFLUX], 1ull);
+  if (fl[0] != fl[0]) GH_ATOMIC_ADD(cnt_at(d, CNT_NANFLUX), 1ull);
   d.fv[k] = v;
   d.flL[k] = w.lL;
   d.flR[k] = w.lR;
@@ -278,7 +295,7 @@ GH_HD void body_dt(const Dev& d, int j, int i) {
   l = d.flL[k]; v = d.fv[k - 1];
   if (l < v) dtR = a / (v - l); else dtR = 1.e15;
   double cand = fmin(dtL, dtR);
-  if (cand < 0) GH_ATOMIC_ADD(&d.cnt[CNT_NEGDT], 1ull);
+  if (cand < 0) GH_ATOMIC_ADD(cnt_at(d, CNT_NEGDT), 1ull);
   d.dtloc[k] = fmin(d.dtloc[k], cand);
 }
 
@@ -687,12 +704,12 @@ GH_HD void body_update_g(const Dev& d, int j, int i, int stage, bool valid, cons
   store_v<ND, NQ>(d, d.prim, k, p);
   d.cx[k] = cxn;
   d.cdx[k] = cdxn;
-  GH_ATOMIC_ADD(&d.cnt[CNT_C2P_CALLS], (unsigned long long)ncalls);
-  if (st.iters) GH_ATOMIC_ADD(&d.cnt[CNT_C2P_ITERS], (unsigned long long)st.iters);
-  if (st.used) GH_ATOMIC_ADD(&d.cnt[CNT_MLP], (unsigned long long)st.used);
-  if (st.fail & 1) GH_ATOMIC_ADD(&d.cnt[CNT_NOBRACKET], 1ull);
-  if (st.fail & 2) GH_ATOMIC_ADD(&d.cnt[CNT_MAXITER], 1ull);
-  if (st.fail & 4) GH_ATOMIC_ADD(&d.cnt[CNT_PHICAP], 1ull);
+  GH_ATOMIC_ADD(cnt_at(d, CNT_C2P_CALLS), (unsigned long long)ncalls);
+  if (st.iters) GH_ATOMIC_ADD(cnt_at(d, CNT_C2P_ITERS), (unsigned long long)st.iters);
+  if (st.used) GH_ATOMIC_ADD(cnt_at(d, CNT_MLP), (unsigned long long)st.used);
+  if (st.fail & 1) GH_ATOMIC_ADD(cnt_at(d, CNT_NOBRACKET), 1ull);
+  if (st.fail & 2) GH_ATOMIC_ADD(cnt_at(d, CNT_MAXITER), 1ull);
+  if (st.fail & 4) GH_ATOMIC_ADD(cnt_at(d, CNT_PHICAP), 1ull);
 }
 
 // Grid::update + Cell::update + RK3 averages (1d.cpp:798-841, 2d.cpp:974-1036,
@@ -1160,7 +1177,7 @@ __global__ void __launch_bounds__(KB) k_f1_scan(Dev d) {
   }
   if (threadIdx.x == 0) {
     d.f1tot[j] = base;
-    if (base > d.fcap) atomicAdd(&d.cnt[CNT_F1OVF], 1ull);
+    if (base > d.fcap) atomicAdd(cnt_at(d, CNT_F1OVF), 1ull);
   }
   __syncthreads();
   base = 0;                    // down slots: the same scan over the fresh-source counts
@@ -1956,7 +1973,7 @@ int gammahip_create(const gammahip_config* cfg, gammahip_ctx** out) {
   e |= dalloc(c, &d.fx, cs); e |= dalloc(c, &d.fxn, cs); e |= dalloc(c, &d.fx0, cs);
   e |= dalloc(c, &d.fv, cs); e |= dalloc(c, &d.flL, cs); e |= dalloc(c, &d.flR, cs);
   e |= dalloc(c, &d.fflux, nqs * cs);
-  e |= dalloc(c, &d.cnt, CNT_N); e |= dalloc(c, &d.nsplit, nj); e |= dalloc(c, &d.nmerge, nj);
+  e |= dalloc(c, &d.cnt, (size_t)CNT_N * CNT_PARTS); e |= dalloc(c, &d.nsplit, nj); e |= dalloc(c, &d.nmerge, nj);
   if (d.nd == 2) {
     e |= dalloc(c, &d.gmv, nqs * cs); e |= dalloc(c, &d.gf1, nqs * cs); e |= dalloc(c, &d.gf1n, nqs * cs);
     e |= dalloc(c, &d.nbdj, cs); e |= dalloc(c, &d.nbdlo, cs); e |= dalloc(c, &d.nbdcnt, cs);
@@ -2338,8 +2355,17 @@ int gammahip_download(gammahip_ctx* c, int field, double* out) {
 }
 
 int gammahip_stats_get(gammahip_ctx* c, gammahip_stats* st) {
-  unsigned long long h[CNT_N];
-  if (d2h(c, h, c->d.cnt, sizeof(h))) return -1;
+  std::vector<unsigned long long> parts((size_t)CNT_N * CNT_PARTS);
+#ifdef GH_EMU
+  if (d2h(c, parts.data(), c->d.cnt, sizeof(unsigned long long) * CNT_N)) return -1;
+  const int np = 1;
+#else
+  if (d2h(c, parts.data(), c->d.cnt, sizeof(unsigned long long) * parts.size())) return -1;
+  const int np = CNT_PARTS;
+#endif
+  unsigned long long h[CNT_N] = {};
+  for (int pp = 0; pp < np; ++pp)
+    for (int q = 0; q < CNT_N; ++q) h[q] += parts[(size_t)pp * CNT_N + q];
   memset(st, 0, sizeof(*st));
   st->c2p_calls = h[CNT_C2P_CALLS]; st->c2p_iters = h[CNT_C2P_ITERS];
   st->c2p_nobracket = h[CNT_NOBRACKET]; st->c2p_maxiter = h[CNT_MAXITER]; st->c2p_phicap = h[CNT_PHICAP];
