@@ -371,10 +371,14 @@ GH_HD void body_nb(const Dev& d, int j, int i) {
   }
 }
 
-// Grid::computeTransGradients (2d.cpp:366-430) of cell (j,i) into gf
-template <int NQ>
+// Grid::computeTransGradients (2d.cpp:366-430) of cell (j,i) into gf.  The
+// second pass (minmod against every neighbour's gradient) reuses the
+// gradients of the first KC neighbours from cache (KC x NQ doubles with
+// stride cs, thread-private) and recomputes the rest, so that a neighbour's
+// state is read once instead of twice.
+template <int NQ, int KC>
 GH_HD void trans_grad(const Dev& d, int j, long long k, const double* p0, const double* m0,
-                      const Geo& g0, double* gf) {
+                      const Geo& g0, double* gf, double* cache, int cs) {
   int geom = d.tgeom[j];
   Trig Tj = trig_at<2>(d, j);
   double x0 = d.cx[k];
@@ -383,11 +387,16 @@ GH_HD void trans_grad(const Dev& d, int j, long long k, const double* p0, const 
   for (int q = 0; q < NQ; ++q) gavg[q] = 0.;
   double dAtot = 0;
   for (int pass = 0; pass < 2; ++pass) {
+    int idx = 0;
     for (int side = 0; side < 2; ++side) {
       int tj = side == 0 ? d.nbdj[k] : d.nbuj[k];
       int lo = side == 0 ? d.nbdlo[k] : d.nbulo[k];
       int cnt = side == 0 ? d.nbdcnt[k] : d.nbucnt[k];
-      for (int n = 0; n < cnt; ++n) {
+      for (int n = 0; n < cnt; ++n, ++idx) {
+        if (pass == 1 && idx < KC) {
+          for (int q = 0; q < NQ; ++q) gf[q] = minmod(gf[q], cache[(idx * NQ + q) * cs]);
+          continue;
+        }
         long long kn = CI(d, tj, lo + n);
         double xn = d.cx[kn];
         double xLn = xn - d.cdx[kn] / 2., xRn = xn + d.cdx[kn] / 2.;
@@ -406,8 +415,12 @@ GH_HD void trans_grad(const Dev& d, int j, long long k, const double* p0, const 
           double qL = p0[q] + m0[q] * dxmL;
           double qR = pn[q] + mn[q] * dxmR;
           double gr = (qR - qL) / (gn.cen1 - g0.cen1);
-          if (pass == 0) gavg[q] += gr * dA;
-          else gf[q] = minmod(gf[q], gr);
+          if (pass == 0) {
+            gavg[q] += gr * dA;
+            if (idx < KC) cache[(idx * NQ + q) * cs] = gr;
+          } else {
+            gf[q] = minmod(gf[q], gr);
+          }
         }
       }
     }
@@ -419,8 +432,12 @@ GH_HD void trans_grad(const Dev& d, int j, long long k, const double* p0, const 
 // The F1 loop of Grid::computeFluxes (2d.cpp:901-950) in two passes.
 // Cell pass: each cell's transverse gradient (computeTransGradients), stored
 // as the new value.
+// neighbours per cell whose gradients the cell pass caches (LDS budget:
+// KC x NQ doubles per thread)
+template <int NQ> constexpr int f1_kc() { return NQ <= 6 ? 4 : 2; }
+
 template <int NQ>
-GH_HD void body_f1grad(const Dev& d, int j, int i) {
+GH_HD void body_f1grad_c(const Dev& d, int j, int i, double* cache, int cs) {
   if (j > d.nj - 2) return;
   int nt = d.ntrack[j];
   if (i >= nt) return;
@@ -429,8 +446,13 @@ GH_HD void body_f1grad(const Dev& d, int j, int i) {
   double pc[NQ], mc[NQ], fc[NQ];
   load_v<2, NQ>(d, d.prim, k, pc);
   load_v<2, NQ>(d, d.gmv, k, mc);
-  trans_grad<NQ>(d, j, k, pc, mc, g0, fc);
+  trans_grad<NQ, f1_kc<NQ>()>(d, j, k, pc, mc, g0, fc, cache, cs);
   store_v<2, NQ>(d, d.gf1n, k, fc);
+}
+template <int NQ>
+GH_HD void body_f1grad(const Dev& d, int j, int i) {
+  double cache[f1_kc<NQ>() * NQ];
+  body_f1grad_c<NQ>(d, j, i, cache, 1);
 }
 
 // Face pass: one F1 face slot s of track j, i.e. the face between its owner
@@ -1064,7 +1086,15 @@ GH_KERNEL2P(k_rg_shockpos, body_rg_shockpos)
 GH_KERNEL2(k_advance_time, body_advance_time)
 template <int ND, int NQ> GH_KERNEL2(k_ghost_mv, (body_ghost_mv<ND, NQ>))
 template <int ND, int NQ> GH_KERNEL2P(k_faces, (body_faces<ND, NQ>))
-template <int NQ> GH_KERNEL2P(k_f1grad, body_f1grad<NQ>)
+// cell pass with the gradient cache in LDS: thread t's entry m at
+// cache[m * blockDim + t] (consecutive lanes, consecutive banks); the launch
+// passes blockDim * f1_kc * NQ doubles of dynamic LDS
+template <int NQ>
+__global__ void __launch_bounds__(KB) k_f1grad(Dev d, int ni, int remap) {
+  extern __shared__ double f1cache[];
+  double* cache = f1cache + threadIdx.x;
+  GH_TILE_LOOP(if (i < ni && j < d.nj) body_f1grad_c<NQ>(d, j, i, cache, (int)blockDim.x))
+}
 template <int NQ> GH_KERNEL2P(k_f1sum, body_f1sum<NQ>)
 // face pass over the F1 slots of each track: thread (j, c*bs + t) of a plain
 // (chunk, track) grid walks slots c*bs + t + m*(chunks*bs)
@@ -1755,7 +1785,16 @@ struct Pipe {
       hipLaunchKernelGGL(k_f1_scan, dim3(d.nj), dim3(KB), 0, c->stream, d);
 #endif
       tbeg(c, 3);
-      L2P(c, KSEL(body_f1grad<NQ>, k_f1grad<NQ>), d.nj, d.nax);
+#ifdef GH_EMU
+      L2(c, body_f1grad<NQ>, d.nj, d.nax);
+#else
+      {
+        int bs = c->block;
+        dim3 g = d.packed ? dim3(c->persist * c->ncu) : dim3((d.nax + bs - 1) / bs, d.nj);
+        size_t lds = sizeof(double) * bs * f1_kc<NQ>() * NQ;
+        hipLaunchKernelGGL(k_f1grad<NQ>, g, dim3(bs), lds, c->stream, d, d.nax, c->remap);
+      }
+#endif
       tend(c, 3);
       tbeg(c, 2);
 #ifdef GH_EMU
