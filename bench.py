@@ -163,7 +163,7 @@ def other_configs(steps, warmup):
                         steps=3000, batch=4096, lr=2e-3, seed=5)
         tuned = autotune.tune(scratch, steps=4, settle=2, sweeps=1, surrogate=w)
         scratch.close()
-        batched = {k: tuned["best"][k] for k in ("block", "remap", "graph", "persist", "occ", "mlp_thr")}
+        batched = {k: tuned["best"][k] for k in ("block", "remap", "graph", "persist", "occ", "f1fuse", "mlp_thr")}
         b = run_other(s, steps, warmup, batched, hip.C2P_NEWTON,
                       surrogate=w if batched["mlp_thr"] is not None else None)
         out[name] = {"plain": p, "batched": dict(b, options=batched),
@@ -256,7 +256,7 @@ def main():
         print("[bench] autotuning the batched form", file=sys.stderr, flush=True)
         tuned = autotune.tune(scratch, surrogate=w)
         scratch.close()
-        opts = {k: tuned["best"][k] for k in ("block", "remap", "graph", "persist", "occ", "mlp_thr")}
+        opts = {k: tuned["best"][k] for k in ("block", "remap", "graph", "persist", "occ", "f1fuse", "mlp_thr")}
         b = run_form(s, hip.C2P_NEWTON, opts, args.steps, args.warmup, args.timing_steps,
                      surrogate=w if opts["mlp_thr"] is not None else None)
         print(f"[bench] batched {b['ms_per_step']:.3f} ms/step with {opts}", file=sys.stderr, flush=True)

@@ -3,7 +3,8 @@
 Options: threads per workgroup of the per-cell kernels (64, 128, 256),
 XCD-aware tile order (off/on), two-step hipGraph replay (off/on), persistent
 packed launch (off, 4 or 8 workgroups per CU), the register-capped builds of
-the F1 face kernel and of the update kernel (bit set, 0-3) and, when a trained
+the F1 face kernel and of the update kernel (bit set, 0-3), the F1 flux sums
+inside the update kernel (off/on) and, when a trained
 network is given, the learned
 cons2prim start (off, or on with trigger exponents 2 and 4).
 
@@ -21,13 +22,13 @@ from __future__ import annotations
 import time
 
 SPACE = (("block", (256, 128, 64)), ("remap", (0, 1)), ("graph", (0, 1)),
-         ("persist", (0, 4, 8)), ("occ", (0, 1, 2, 3)))
-DEFAULTS = {"block": 256, "remap": 0, "graph": 0, "persist": 0, "occ": 0, "mlp_thr": None}
+         ("persist", (0, 4, 8)), ("occ", (0, 1, 2, 3)), ("f1fuse", (0, 1)))
+DEFAULTS = {"block": 256, "remap": 0, "graph": 0, "persist": 0, "occ": 0, "f1fuse": 0, "mlp_thr": None}
 
 
 def apply(grid, cand: dict, surrogate=None):
     grid.set_options(block=cand["block"], remap=cand["remap"], graph=cand["graph"],
-                     persist=cand["persist"], occ=cand["occ"])
+                     persist=cand["persist"], occ=cand["occ"], f1fuse=cand["f1fuse"])
     if cand.get("mlp_thr") is None:
         if surrogate is not None:
             grid.load_surrogate(None)

@@ -78,6 +78,7 @@ struct Dev {
   // down slots dnoff.. of its own track, in the order of their source cells
   double *ff1dn, *ff1sddn;
   double *f1net;           // [NQ][nj][nax] net F1 flux (dn - up) of each cell, this stage
+  int f1fused;             // 1: the update kernel sums the F1 fluxes itself (no k_f1sum pass)
   unsigned long long* cnt;
   long long *nsplit, *nmerge;
   // learned cons2prim guess: FP32 parameters and the packed BF16 hidden-layer
@@ -599,12 +600,7 @@ GH_HD void body_f1_shock(const Dev& d, int j, int i) {
 // as dn - up, ahead of the update kernel: the gathers over other cells' slots
 // run here, in a light kernel, not in the update's register-heavy one.
 template <int NQ>
-GH_HD void body_f1sum(const Dev& d, int j, int i) {
-  if (!active_track(d, j)) return;
-  int nt = d.ntrack[j];
-  int iL = d.ngst - 1, iR = nt - d.ngst;
-  if (i < iL + 1 || i > iR - 1) return;
-  long long k = CI(d, j, i);
+GH_HD void f1_net(const Dev& d, int j, int i, long long k, double* net) {
   double up[NQ], dn[NQ];
   for (int q = 0; q < NQ; ++q) { up[q] = 0.; dn[q] = 0.; }
   int cnt = d.nbucnt[k], off = d.f1off[k];
@@ -616,7 +612,18 @@ GH_HD void body_f1sum(const Dev& d, int j, int i) {
     const double* src = down ? d.ff1dn : d.ff1;
     for (int q = 0; q < NQ; ++q) dn[q] += src[q * d.fs1 + fk];
   });
-  for (int q = 0; q < NQ; ++q) d.f1net[q * d.cs + k] = dn[q] - up[q];
+  for (int q = 0; q < NQ; ++q) net[q] = dn[q] - up[q];
+}
+template <int NQ>
+GH_HD void body_f1sum(const Dev& d, int j, int i) {
+  if (!active_track(d, j)) return;
+  int nt = d.ntrack[j];
+  int iL = d.ngst - 1, iR = nt - d.ngst;
+  if (i < iL + 1 || i > iR - 1) return;
+  long long k = CI(d, j, i);
+  double net[NQ];
+  f1_net<NQ>(d, j, i, k, net);
+  for (int q = 0; q < NQ; ++q) d.f1net[q * d.cs + k] = net[q];
 }
 
 // ---------------------------------------------------------------------------
@@ -685,8 +692,12 @@ GH_HD void body_update_g(const Dev& d, int j, int i, int stage, bool valid, cons
       load_v<ND, NQ>(d, d.fflux, k, fb);
       for (int q = 0; q < NQ; ++q) U[q] += (fa[q] - fb[q]) * dt;
     }
-    if (ND == 2)   // net F1 flux, summed by body_f1sum
-      for (int q = 0; q < NQ; ++q) U[q] += d.f1net[q * d.cs + k] * dt;
+    if (ND == 2) {   // net F1 flux: summed here, or read from body_f1sum's pass
+      double f1[NQ];
+      if (d.f1fused) f1_net<NQ>(d, j, i, k, f1);
+      else for (int q = 0; q < NQ; ++q) f1[q] = d.f1net[q * d.cs + k];
+      for (int q = 0; q < NQ; ++q) U[q] += f1[q] * dt;
+    }
     sources<ND, NQ>(p, U, d.cx[k], d.cdx[k], trig_at<ND>(d, j), g.dV, dt, geom, G);
     if (NQ == 10 && d.sync) radiative(p, U, g.dV, dt, d.P);
     cxn = (xnR + xnL) / 2.;
@@ -1810,9 +1821,11 @@ struct Pipe {
 #endif
       tend(c, 2);
       if (d.shock) L2P(c, KSEL(body_f1_shock, k_f1_shock), d.nj, d.nax);
-      tbeg(c, 4);
-      L2P(c, KSEL(body_f1sum<NQ>, k_f1sum<NQ>), d.nj, d.nax);
-      tend(c, 4);
+      if (!d.f1fused) {
+        tbeg(c, 4);
+        L2P(c, KSEL(body_f1sum<NQ>, k_f1sum<NQ>), d.nj, d.nax);
+        tend(c, 4);
+      }
       std::swap(d.gf1, d.gf1n);
     }
   }
@@ -2203,6 +2216,7 @@ int gammahip_set_option(gammahip_ctx* c, int key, long long value) {
       c->block = (int)value;
       build_tiles(c);
       return post(c);
+    case GAMMAHIP_OPT_F1FUSE: c->d.f1fused = value ? 1 : 0; return 0;
     case GAMMAHIP_OPT_OCC:
       if (value < 0 || value > 3) return fail(c, "occupancy option must be in 0..3");
       c->occ = (int)value; return 0;

@@ -135,9 +135,12 @@ int  gammahip_sync(gammahip_ctx* ctx);
  * 0 launches one workgroup per chunk of the nmax-wide track slots; OCC bit 0
  * runs the F1 face kernel in a build capped at 128 registers (4 wavefronts
  * per SIMD), bit 1 the update kernel in one capped at 96 (5 wavefronts),
- * spilling what does not fit. */
+ * spilling what does not fit; F1FUSE 1 sums each cell's F1 face fluxes inside
+ * the update kernel instead of in a pass of its own (fewer bytes, a longer
+ * dependent-load chain in the update). */
 enum { GAMMAHIP_OPT_BLOCK = 0, GAMMAHIP_OPT_REMAP = 1, GAMMAHIP_OPT_GRAPH = 2,
-       GAMMAHIP_OPT_MLP_THR = 3, GAMMAHIP_OPT_PERSIST = 4, GAMMAHIP_OPT_OCC = 5 };
+       GAMMAHIP_OPT_MLP_THR = 3, GAMMAHIP_OPT_PERSIST = 4, GAMMAHIP_OPT_OCC = 5,
+       GAMMAHIP_OPT_F1FUSE = 6 };
 int  gammahip_set_option(gammahip_ctx* ctx, int key, long long value);
 
 /* ---- learned cons2prim guess (no counterpart in the reference) ----

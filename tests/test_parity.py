@@ -206,6 +206,20 @@ def _ensemble_check(lib_path, n_runs=6, sizes=(64, 256), nsteps=40):
     return out
 
 
+@pytest.mark.parametrize("case", ["riemann_2d", "bm_jet_2d"])
+def test_emu_parity_f1_sums_in_update(case, emu_lib):
+    """With the F1 sums done inside the update (no separate pass) the kernel
+    bodies still reproduce the oracle bit for bit."""
+    mk, n = CASES[case]
+    s = mk()
+    o = oc.OracleGrid(s)
+    o.run(n)
+    g = hip.Grid(s, lib_path=emu_lib)
+    g.set_options(f1fuse=1)
+    g.step(n)
+    check_pair(o, g, shock=bool(s.cfg.shock), exact_expected=True)
+
+
 def test_emu_parity_ensemble(emu_lib):
     res = _ensemble_check(emu_lib)
     assert all(bit for _, _, bit in res), res
@@ -242,7 +256,7 @@ def test_gpu_pow_equals_oracle_bitwise():
 @pytest.mark.parametrize("case", ["shock_tube_cart", "riemann_2d", "bm_jet_2d"])
 @pytest.mark.parametrize("opts", [
     dict(block=128, remap=1, graph=1, persist=0),
-    dict(block=256, remap=0, graph=0, persist=4),
+    dict(block=256, remap=0, graph=0, persist=4, f1fuse=1),
     dict(block=64, remap=1, graph=1, persist=8, occ=3),
 ], ids=["remap-graph", "packed", "packed-remap-graph-capped"])
 def test_gpu_parity_launch_options(case, opts):
