@@ -115,7 +115,7 @@ GH_HD double c2p_f(double p, double D, double S, double E, double G, int geom) {
 // carries a relative error of about W^2 eps (W^2 ~ 1e4-1e6 in the
 // ultra-relativistic shells).
 GH_HD double c2p_df(double p, double D, double S, double E, double G, int geom, double* fval,
-                    double* fscale, double* fnoise) {
+                    double* fscale, double* fnoise, double* lf2) {
   double Ep = E + p;
   double w = 1. - (S * S) / (Ep * Ep);
   double aw = geom == SPH ? fabs(w) : w;
@@ -125,6 +125,7 @@ GH_HD double c2p_df(double p, double D, double S, double E, double G, int geom, 
   double tw = fabs(D * lf) + fabs((G * p * lf * lf) / (G - 1.));
   *fscale = fabs(E) + fabs(p) + tw;
   *fnoise = *fscale + lf * lf * tw;
+  *lf2 = lf * lf;
   // d lf / dp = -lf^3 S^2 / (E+p)^3 (sign of w folded in for |w|)
   double dw = 2. * (S * S) / (Ep * Ep * Ep);
   double dlf = -0.5 * lf * lf * lf * (geom == SPH && w < 0 ? -dw : dw);
@@ -204,8 +205,8 @@ GH_HD double brent(double D, double S, double E, double G, int geom, double lo, 
 // iterate well inside the reference's 1e-13 tolerance), or once the bracket is
 // that narrow.  Near v = 1 the rounding of f is W^2 larger than its terms
 // suggest and Newton ends up bouncing across the root on noise: once |f| is
-// within that larger bound and a step, already below 1e-11 relative, no
-// longer halves the one before, Newton has nothing more to say.  If the bracket is already tight (1e-10 relative)
+// within that larger bound and a step, already below 1e-11 + 4 W^2 eps
+// relative, no longer halves the one before, Newton has nothing more to say.  If the bracket is already tight (1e-10 relative)
 // it is bisected down to 1e-15, so that the answer is the sign change of f
 // whatever the start; otherwise the solve stops at x.
 GH_HD double newton(double D, double S, double E, double G, int geom, double lo, double hi,
@@ -215,15 +216,15 @@ GH_HD double newton(double D, double S, double E, double G, int geom, double lo,
   int it = 0;
   for (;;) {
     it++;
-    double f, fs, fn;
-    double df = c2p_df(x, D, S, E, G, geom, &f, &fs, &fn);
+    double f, fs, fn, w2;
+    double df = c2p_df(x, D, S, E, G, geom, &f, &fs, &fn, &w2);
     const double eps = 2.2204460492503131e-16;
     if (fabs(f) <= 4. * eps * fs) break;
     if (f > 0) lo = x; else hi = x;      // f decreases through the root
     double xn = x - f / df;
     if (!(xn > lo && xn < hi)) xn = 0.5 * (lo + hi);
     double dx = fabs(xn - x);
-    if (it > 1 && fabs(f) <= 4. * eps * fn && dx > 0.5 * dxold && dx <= 1e-11 * fabs(x)) {
+    if (it > 1 && fabs(f) <= 4. * eps * fn && dx > 0.5 * dxold && dx <= (1e-11 + 4. * eps * w2) * fabs(x)) {
       if (hi - lo > 1e-10 * fabs(x)) break;
       while (hi - lo > 1e-15 * fabs(x) && it < maxit) {
         it++;
@@ -236,7 +237,11 @@ GH_HD double newton(double D, double S, double E, double G, int geom, double lo,
     dxold = dx;
     x = xn;
     if (dx <= 1.e-15 * fabs(x) || hi - lo <= 1.e-15 * fabs(lo)) break;
-    if (it >= maxit) { st->fail |= 2; break; }
+    if (it >= maxit) {
+      // out of iterations: a failure unless f is already at its rounding floor
+      if (!(fabs(f) <= 4. * eps * fn)) st->fail |= 2;
+      break;
+    }
   }
   st->iters += it;
   return x;
