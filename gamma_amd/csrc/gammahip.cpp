@@ -1385,6 +1385,24 @@ __global__ void __launch_bounds__(RGB) k_rg_apply(Dev d) {
 
 // ===========================================================================
 // learned cons2prim guess: training and batch inference kernels
+// pointwise cons2prim of raw states (D, |S|, tau) from given starting
+// pressures; with guess2 != null each state starts from whichever of the two
+// guesses has the smaller residual |f| (used[s] = 1 when guess[s] won)
+GH_HD void body_c2p_batch(long long s, const double* st, const double* guess, const double* guess2, double G,
+                          int solver, int maxit, double* p_out, int* iters, int* fail, int* used) {
+  double D = st[3 * s], S = st[3 * s + 1], tau = st[3 * s + 2];
+  double prim[6] = {0., 0., 0., 0., 0., 0.}, cons[6] = {0., 0., 0., 0., 0., 0.};
+  // with a second start, guess2 plays the previous pressure and guess the challenger
+  prim[PPP] = guess2 ? guess2[s] : guess[s];
+  cons[DEN] = D; cons[SS1] = S; cons[TAU] = tau;
+  C2PStat cs = {0, 0, 0, 0};
+  cons2prim<1, 6>(prim, cons, 1., CART, G, solver, guess2 ? guess[s] : -1., maxit, &cs);
+  p_out[s] = prim[PPP];
+  iters[s] = cs.iters + cs.evals;
+  fail[s] = cs.fail;
+  if (used) used[s] = guess2 ? cs.used : 1;
+}
+
 #ifndef GH_EMU
 namespace {
 using namespace gh::mlp;
@@ -1514,26 +1532,12 @@ __global__ void __launch_bounds__(64) k_mlp_infer(const float* __restrict__ w, c
   if (valid) out[s] = pressure_of(w, D, y);
 }
 
-// pointwise cons2prim of raw states (D, |S|, tau) from given starting
-// pressures; with guess2 != null each state starts from whichever of the two
-// guesses has the smaller residual |f| (used[s] = 1 when guess[s] won)
 __global__ void k_c2p_batch(const double* __restrict__ st, const double* __restrict__ guess,
                             const double* __restrict__ guess2, long long n, double G, int solver, int maxit,
                             double* __restrict__ p_out, int* __restrict__ iters, int* __restrict__ fail,
                             int* __restrict__ used) {
   long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= n) return;
-  double D = st[3 * s], S = st[3 * s + 1], tau = st[3 * s + 2];
-  double prim[6] = {0., 0., 0., 0., 0., 0.}, cons[6] = {0., 0., 0., 0., 0., 0.};
-  // with a second start, guess2 plays the previous pressure and guess the challenger
-  prim[PPP] = guess2 ? guess2[s] : guess[s];
-  cons[DEN] = D; cons[SS1] = S; cons[TAU] = tau;
-  C2PStat cs = {0, 0, 0, 0};
-  cons2prim<1, 6>(prim, cons, 1., CART, G, solver, guess2 ? guess[s] : -1., maxit, &cs);
-  p_out[s] = prim[PPP];
-  iters[s] = cs.iters + cs.evals;
-  fail[s] = cs.fail;
-  if (used) used[s] = guess2 ? cs.used : 1;
+  if (s < n) body_c2p_batch(s, st, guess, guess2, G, solver, maxit, p_out, iters, fail, used);
 }
 
 // Update stage with the learned starting pressure.  The network runs per
@@ -1551,12 +1555,9 @@ struct MlpGuess {
   __device__ double operator()(bool want, double D, double S, double tau, double pprev, int geom) const {
     bool hard = false;
     if (want) {
-      double E = D + tau;
       double pp = geom == SPH ? fmax(1.e-13, pprev) : pprev;
-      double Ep = E + pp;
-      double lf = 1. / sqrt(fabs(1. - S * S / (Ep * Ep)));
-      double f = E + pp - D * lf - (G * pp * lf * lf) / (G - 1.);
-      double sc = fabs(E) + fabs(pp) + fabs(D * lf) + fabs((G * pp * lf * lf) / (G - 1.));
+      double f, sc;
+      (void)c2p_df(pp, D, S, tau, G, geom, &f, &sc);
       hard = !(fabs(f) <= thr * sc);
     }
     if (__ballot(hard) == 0) return -1.;
@@ -2292,6 +2293,97 @@ int gammahip_dims(gammahip_ctx* c, int* ntracks, int* nax) {
   return 0;
 }
 
+// ---- exact state (binary resume) ----
+namespace {
+constexpr int ST_MAGIC = 0x54534847;   // "GHST"
+constexpr int ST_VERSION = 3;
+constexpr int ST_HDR = 16;             // ints
+// bytes: the array's size; pad: its size in the buffer (a multiple of 8)
+struct StatePart { void* p; size_t bytes, pad; };
+std::vector<StatePart> state_parts(const Dev& d) {
+  size_t cs = (size_t)d.cs, nj = (size_t)d.nj, nqs = (size_t)d.nqs, fs1 = (size_t)d.fs1;
+  std::vector<StatePart> v;
+  auto add = [&](void* p, size_t bytes) { v.push_back({p, bytes, (bytes + 7) & ~(size_t)7}); };
+  add(d.tt, sizeof(double) * nj); add(d.tdt, sizeof(double) * nj); add(d.ntrack, sizeof(int) * nj);
+  add(d.prim, sizeof(double) * nqs * cs); add(d.cx, sizeof(double) * cs); add(d.cdx, sizeof(double) * cs);
+  add(d.fx, sizeof(double) * cs); add(d.fv, sizeof(double) * cs);
+  add(d.dtloc, sizeof(double) * cs);   // the next step's CFL minimum still sees the last stage's
+  if (d.shock) {
+    add(d.sd, sizeof(double) * cs); add(d.pspec, sizeof(double) * cs); add(d.shk, sizeof(int) * cs);
+  }
+  if (d.nd == 2) {
+    // 2D keeps gradients, F1 neighbour lists and F1 face values in memory,
+    // and reads some of them a stage later: the face pass takes the upper
+    // track's transverse gradient of the previous stage (2d.cpp:901 loops
+    // tracks upward), ghost tracks keep the lists of earlier stages
+    add(d.gmv, sizeof(double) * nqs * cs); add(d.gf1, sizeof(double) * nqs * cs);
+    for (int* q : {d.nbdj, d.nbdlo, d.nbdcnt, d.nbuj, d.nbulo, d.nbucnt, d.f1off, d.incf, d.dncnt, d.dnoff})
+      add(q, sizeof(int) * cs);
+    add(d.f1tot, sizeof(int) * nj); add(d.f1own, sizeof(int) * nj * d.fcap);
+    add(d.ff1, sizeof(double) * nqs * fs1); add(d.ff1dn, sizeof(double) * nqs * fs1);
+    if (d.shock) { add(d.ff1sd, sizeof(double) * 4 * fs1); add(d.ff1sddn, sizeof(double) * 2 * fs1); }
+  }
+  return v;
+}
+void state_header(const Dev& d, int* h) {
+  for (int k = 0; k < ST_HDR; ++k) h[k] = 0;
+  h[0] = ST_MAGIC; h[1] = ST_VERSION; h[2] = d.nd; h[3] = d.nq; h[4] = d.nqs;
+  h[5] = d.nj; h[6] = d.nax; h[7] = d.shock;
+}
+}  // namespace
+
+int gammahip_state_size(gammahip_ctx* c, unsigned long long* nbytes) {
+  if (!c || !nbytes) return -1;
+  size_t n = sizeof(int) * ST_HDR;
+  for (const StatePart& s : state_parts(c->d)) n += s.pad;
+  *nbytes = n;
+  return 0;
+}
+
+int gammahip_state_save(gammahip_ctx* c, void* buf, unsigned long long nbytes) {
+  if (!c || !buf) return -1;
+  unsigned long long need = 0;
+  gammahip_state_size(c, &need);
+  if (nbytes < need) return fail(c, "state buffer smaller than gammahip_state_size");
+  char* o = (char*)buf;
+  state_header(c->d, (int*)o);
+  o += sizeof(int) * ST_HDR;
+  for (const StatePart& s : state_parts(c->d)) {
+    if (s.bytes < s.pad) memset(o + s.bytes, 0, s.pad - s.bytes);
+    if (d2h(c, o, s.p, s.bytes)) return -1;
+    o += s.pad;
+  }
+  return 0;
+}
+
+int gammahip_state_restore(gammahip_ctx* c, const void* buf, unsigned long long nbytes) {
+  if (!c || !buf) return -1;
+  Dev& d = c->d;
+  unsigned long long need = 0;
+  gammahip_state_size(c, &need);
+  if (nbytes != need) return fail(c, "state buffer size differs from gammahip_state_size");
+  int want[ST_HDR];
+  state_header(d, want);
+  const int* h = (const int*)buf;
+  if (h[0] != ST_MAGIC || h[1] != ST_VERSION) return fail(c, "not a gammahip state buffer (magic/version)");
+  for (int k = 2; k < 8; ++k)
+    if (h[k] != want[k]) return fail(c, "state buffer dimensions differ from the context's (nd, nq, tracks, nax, shock)");
+  const char* o = (const char*)buf + sizeof(int) * ST_HDR;
+  // track lengths must fit the context's slots
+  const int* nt = (const int*)(o + 2 * sizeof(double) * d.nj);
+  for (int j = 0; j < d.nj; ++j)
+    if (nt[j] < 2 * d.ngst + 2 || nt[j] > d.nax) return fail(c, "state buffer: track length out of range");
+#ifndef GH_EMU
+  HIPCHK(hipStreamSynchronize(c->stream));
+#endif
+  drop_graph(c);
+  for (const StatePart& s : state_parts(d)) {
+    if (h2d(c, s.p, o, s.bytes)) return -1;
+    o += s.pad;
+  }
+  return 0;
+}
+
 int gammahip_track_info(gammahip_ctx* c, int* ntrack, int* nact, int* iL, int* iR) {
   Dev& d = c->d;
   std::vector<int> nt(d.nj);
@@ -2527,13 +2619,15 @@ int gammahip_mlp_infer(const float* params, const double* states, long long n, d
 int gammahip_c2p_batch(const double* states, const double* guess, const double* guess2, long long n,
                        double gamma, int solver, int maxit, double* p_out, int* iters, int* fail,
                        int* used) {
+  if (n < 1 || !(gamma > 1.) || (solver != gh::C2P_BRENT && solver != gh::C2P_NEWTON) || maxit < 1) {
+    fprintf(stderr, "gammahip: bad c2p_batch arguments\n");
+    return -1;
+  }
 #ifdef GH_EMU
-  (void)states; (void)guess; (void)guess2; (void)n; (void)gamma; (void)solver; (void)maxit;
-  (void)p_out; (void)iters; (void)fail; (void)used;
-  return -1;
+  for (long long s = 0; s < n; ++s)
+    body_c2p_batch(s, states, guess, guess2, gamma, solver, maxit, p_out, iters, fail, used);
+  return 0;
 #else
-  if (n < 1 || !(gamma > 1.) || (solver != gh::C2P_BRENT && solver != gh::C2P_NEWTON) || maxit < 1)
-    return plain_fail("bad c2p_batch arguments");
   DevBuf B;
   double *st = B.get<double>(3 * n), *g = B.get<double>(n), *g2 = guess2 ? B.get<double>(n) : nullptr;
   double* p = B.get<double>(n);

@@ -108,28 +108,59 @@ GH_HD double c2p_f(double p, double D, double S, double E, double G, int geom) {
   return (E + p - D * lf - (G * p * lf * lf) / (G - 1.));
 }
 
-// d f / d p for the Newton solver (analytic derivative of c2p_f).  *fscale
-// is the sum of the magnitudes of the terms of f; *fnoise bounds its rounding
-// error more honestly, weighting the terms that hold the Lorentz factor by
-// 1 + W^2: W = 1/sqrt(w) with w = 1 - S^2/(E+p)^2 formed by cancellation
-// carries a relative error of about W^2 eps (W^2 ~ 1e4-1e6 in the
-// ultra-relativistic shells).
-GH_HD double c2p_df(double p, double D, double S, double E, double G, int geom, double* fval,
-                    double* fscale, double* fnoise, double* lf2) {
+// Newton's residual in double-double arithmetic.  Where E = D + tau holds it
+// is the same function of p as c2p_f, written without forming E - D:
+//   f = tau + p - D (W - 1) - G/(G-1) p W^2,  W - 1 = (1 - a) W^2 / (W + 1),
+// with W^2 = 1/a, a = w = 1 - S^2/(E+p)^2 (|w| in spherical geometry, as
+// c2p_f).  Evaluated in doubles, any form of f rounds at eps times the energy
+// of the state, which for a cold gas in motion is the kinetic energy, many
+// orders above the pressure's own scale: the root is then only known to
+// eps (kinetic / thermal) relative, and a root finder stops at a point of that
+// band that depends on where it started.  In double-double (the Dekker/Knuth
+// operations of ddpow.hpp, exact on host and device alike) the band shrinks
+// to eps^2 times the same ratio, below one unit in the last place of p for
+// every state the solver meets, so the sign of f at a double is reliable and
+// the root rounds to one double whatever the start.
+GH_HD double c2p_fdd(double p, double D, double S, double tau, double G, int geom) {
+  using namespace dd;
+  D2 E = two_sum(D, tau);
+  D2 Ep = add(E, {p, 0.});
+  D2 v2 = div(two_prod(S, S), mul(Ep, Ep));
+  D2 a = add({1., 0.}, {-v2.hi, -v2.lo});
+  if (geom == SPH && a.hi < 0.) a = {-a.hi, -a.lo};
+  D2 W2 = div({1., 0.}, a);
+  double s = sqrt(W2.hi);                           // W = sqrt(W2): one Newton correction
+  D2 s2 = two_prod(s, s);
+  D2 W = fast_two_sum(s, ((W2.hi - s2.hi) - s2.lo + W2.lo) / (2. * s));
+  D2 oma = add({1., 0.}, {-a.hi, -a.lo});           // 1 - a
+  D2 t1 = div(mul_d(mul(oma, W2), D), add(W, {1., 0.}));
+  D2 t2 = mul(mul_d(W2, p), div({G, 0.}, {G - 1., 0.}));
+  D2 f = add(two_sum(tau, p), {-t1.hi, -t1.lo});
+  f = add(f, {-t2.hi, -t2.lo});
+  return f.hi;
+}
+
+// d f / d p for the Newton steps (in doubles: the derivative only sets the
+// step), the residual in doubles and the magnitude of its terms (*fscale).
+GH_HD double c2p_df(double p, double D, double S, double tau, double G, int geom, double* fval,
+                    double* fscale) {
+  double E = D + tau;
   double Ep = E + p;
-  double w = 1. - (S * S) / (Ep * Ep);
+  double v2 = (S * S) / (Ep * Ep);
+  double w = 1. - v2;
   double aw = geom == SPH ? fabs(w) : w;
-  double lf = 1. / sqrt(aw);
+  double W2 = 1. / aw;
+  double lf = sqrt(W2);
+  double om = w >= 0. ? v2 : 2. - v2;      // 1 - |w|
   double gg = G / (G - 1.);
-  *fval = (E + p - D * lf - (G * p * lf * lf) / (G - 1.));
-  double tw = fabs(D * lf) + fabs((G * p * lf * lf) / (G - 1.));
-  *fscale = fabs(E) + fabs(p) + tw;
-  *fnoise = *fscale + lf * lf * tw;
-  *lf2 = lf * lf;
-  // d lf / dp = -lf^3 S^2 / (E+p)^3 (sign of w folded in for |w|)
+  double t1 = D * (om * W2 / (lf + 1.));
+  double t2 = gg * p * W2;
+  *fval = tau + p - t1 - t2;
+  *fscale = fabs(tau) + fabs(p) + fabs(t1) + fabs(t2);
+  // d W / dp = -W^3 S^2 / (E+p)^3 (sign of w folded in for |w|)
   double dw = 2. * (S * S) / (Ep * Ep * Ep);
   double dlf = -0.5 * lf * lf * lf * (geom == SPH && w < 0 ? -dw : dw);
-  return 1. - D * dlf - gg * (lf * lf + 2. * p * lf * dlf);
+  return 1. - D * dlf - gg * (W2 + 2. * p * lf * dlf);
 }
 
 enum { C2P_BRENT = 0, C2P_NEWTON = 1 };
@@ -198,51 +229,52 @@ GH_HD double brent(double D, double S, double E, double G, int geom, double lo, 
   return root;
 }
 
-// Safeguarded Newton on the same bracket: Newton steps from the guess, a
-// bisection step whenever Newton leaves the bracket.  Converged when |f| is at
-// the rounding level of its own terms (x is a root to working precision), or
-// after a step below 1e-15 relative (quadratic convergence puts the new
-// iterate well inside the reference's 1e-13 tolerance), or once the bracket is
-// that narrow.  Near v = 1 the rounding of f is W^2 larger than its terms
-// suggest and Newton ends up bouncing across the root on noise: once |f| is
-// within that larger bound and a step, already below 1e-11 + 4 W^2 eps
-// relative, no longer halves the one before, Newton has nothing more to say.  If the bracket is already tight (1e-10 relative)
-// it is bisected down to 1e-15, so that the answer is the sign change of f
-// whatever the start; otherwise the solve stops at x.
-GH_HD double newton(double D, double S, double E, double G, int geom, double lo, double hi,
+// Safeguarded Newton on the same bracket, on the double-double residual:
+// Newton steps from the guess, a bisection step whenever Newton leaves the
+// bracket, until a step or the bracket falls to two units in the last place.
+// The answer is then made canonical: the largest double at which f > 0 (f
+// decreases through the root), found by stepping one ulp at a time from the
+// last iterate (at most 8 steps each way).  The same state gives the same
+// bits from every start; a solve that runs out of iterations without
+// reaching the sign change is a failure (fail bit 2).
+GH_HD double newton(double D, double S, double tau, double G, int geom, double lo, double hi,
                     double x0, int maxit, C2PStat* st) {
   double x = (x0 >= lo && x0 <= hi) ? x0 : 0.5 * (lo + hi);
-  double dxold = hi - lo;
+  const double tol = 4.5e-16;
   int it = 0;
   for (;;) {
     it++;
-    double f, fs, fn, w2;
-    double df = c2p_df(x, D, S, E, G, geom, &f, &fs, &fn, &w2);
-    const double eps = 2.2204460492503131e-16;
-    if (fabs(f) <= 4. * eps * fs) break;
+    double f = c2p_fdd(x, D, S, tau, G, geom);
+    if (f == 0.) break;
     if (f > 0) lo = x; else hi = x;      // f decreases through the root
-    double xn = x - f / df;
+    double fd, fs;
+    double step = f / c2p_df(x, D, S, tau, G, geom, &fd, &fs);
+    // a step below the tolerance ends the iteration before the bracket test:
+    // rounded, x - step may equal x, which sits on the bracket's end
+    if (fabs(step) <= tol * fabs(x)) { x -= step; break; }
+    double xn = x - step;
     if (!(xn > lo && xn < hi)) xn = 0.5 * (lo + hi);
-    double dx = fabs(xn - x);
-    if (it > 1 && fabs(f) <= 4. * eps * fn && dx > 0.5 * dxold && dx <= (1e-11 + 4. * eps * w2) * fabs(x)) {
-      if (hi - lo > 1e-10 * fabs(x)) break;
-      while (hi - lo > 1e-15 * fabs(x) && it < maxit) {
-        it++;
-        double xm = 0.5 * (lo + hi);
-        if (c2p_f(xm, D, S, E, G, geom) > 0) lo = xm; else hi = xm;
-        x = 0.5 * (lo + hi);
-      }
-      break;
-    }
-    dxold = dx;
     x = xn;
-    if (dx <= 1.e-15 * fabs(x) || hi - lo <= 1.e-15 * fabs(lo)) break;
-    if (it >= maxit) {
-      // out of iterations: a failure unless f is already at its rounding floor
-      if (!(fabs(f) <= 4. * eps * fn)) st->fail |= 2;
-      break;
+    if (hi - lo <= tol * fabs(lo)) break;
+    if (it >= maxit) break;
+  }
+  bool found = false;
+  it++;
+  if (c2p_fdd(x, D, S, tau, G, geom) > 0) {
+    for (int k = 0; k < 8; ++k) {
+      double xu = nextafter(x, HUGE_VAL);
+      it++;
+      if (!(c2p_fdd(xu, D, S, tau, G, geom) > 0)) { found = true; break; }
+      x = xu;
+    }
+  } else {
+    for (int k = 0; k < 8 && x > 0.; ++k) {
+      x = nextafter(x, 0.);
+      it++;
+      if (c2p_fdd(x, D, S, tau, G, geom) > 0) { found = true; break; }
     }
   }
+  if (!found) st->fail |= 2;
   st->iters += it;
   return x;
 }
@@ -263,8 +295,9 @@ GH_HD void c2p_inputs(const double* cons, double r, int geom, double* D, double*
 // FluidState::cons2prim  rel_sph.cpp:213-320 / rel_cart.cpp:125-229.
 // prim[PPP] on entry is the previous pressure (the reference's guess).
 // guess_override > 0 (Newton only) is a second start, e.g. the learned guess:
-// the solve starts from whichever of the two has the smaller residual |f|, so
-// the previous pressure stays both the fallback and the judge.
+// the solve starts from whichever of the two has the smaller residual |f|
+// (Newton's double-double form), so the previous pressure stays the fallback
+// and the judge.  Newton's answer does not depend on its start (see newton).
 // cons on exit = prim2cons(prim) (rel_sph.cpp:318).
 template <int ND, int NQ>
 GH_HD void cons2prim(double* prim, double* cons, double r, int geom, double G, int solver,
@@ -278,25 +311,28 @@ GH_HD void cons2prim(double* prim, double* cons, double r, int geom, double G, i
   double pprev = geom == SPH ? fmax(1.e-13, prim[PPP]) : prim[PPP];
   double pCand = pprev;
   double start = pCand;
-  if (guess_override > 0. && solver == C2P_NEWTON) {
+  bool nw = solver == C2P_NEWTON;
+  // Brent: the reference's residual; Newton: its double-double form
+  auto F = [&](double pp) { return nw ? c2p_fdd(pp, D, S, tau, G, geom) : c2p_f(pp, D, S, E, G, geom); };
+  if (guess_override > 0. && nw) {
     st->evals += 2;
-    if (fabs(c2p_f(guess_override, D, S, E, G, geom)) < fabs(c2p_f(pCand, D, S, E, G, geom))) {
+    if (fabs(F(guess_override)) < fabs(F(pCand))) {
       start = guess_override;
       st->used++;
     }
   }
-  double f_init = c2p_f(solver == C2P_BRENT ? pCand : start, D, S, E, G, geom);
-  if (solver == C2P_NEWTON) pCand = start;
-  if (fabs(f_init) > 0.) {
+  double f_init = F(start);
+  if (nw) pCand = start;
+  if (fabs(f_init) > 0. || nw) {
     double p_lo = fmax(0., (1. + 1e-13) * S - E), p_hi;
     bool find_p = true;
     st->evals += 2;
-    if (c2p_f(p_lo, D, S, E, G, geom) < 0) find_p = false;
+    if (F(p_lo) < 0) find_p = false;
     if (f_init < 0) p_hi = 1. * pCand;
     else {
       p_hi = pCand;
       int guard = 0;
-      while (c2p_f(p_hi, D, S, E, G, geom) > 0) {
+      while (F(p_hi) > 0) {
         st->evals++;
         p_hi *= 10;
         if (++guard > 400) { find_p = false; st->fail |= 4; break; }
@@ -304,14 +340,12 @@ GH_HD void cons2prim(double* prim, double* cons, double r, int geom, double G, i
     }
     if (p_hi < p_lo) find_p = false;
     if (find_p) {
-      if (solver == C2P_BRENT) pCand = brent(D, S, E, G, geom, p_lo, p_hi, maxit, st);
-      else pCand = newton(D, S, E, G, geom, p_lo, p_hi, pCand, 100, st);
+      if (!nw) pCand = brent(D, S, E, G, geom, p_lo, p_hi, maxit, st);
+      else pCand = newton(D, S, tau, G, geom, p_lo, p_hi, pCand, 100, st);
     } else {
       st->fail |= 1;
       pCand = pprev;
     }
-  } else if (solver == C2P_NEWTON) {
-    pCand = start;
   }
   double p = pCand;
   double Ep2 = (E + p) * (E + p);

@@ -87,6 +87,9 @@ def load_library(path: str = LIB_PATH):
         "gammahip_c2p_batch": (I, [P, P, P, C.c_longlong, D, I, I, P, P, P, P]),
         "gammahip_mlp_load": (I, [P, P]),
         "gammahip_pow_batch": (I, [P, P, P, C.c_longlong]),
+        "gammahip_state_size": (I, [P, C.POINTER(C.c_ulonglong)]),
+        "gammahip_state_save": (I, [P, P, C.c_ulonglong]),
+        "gammahip_state_restore": (I, [P, P, C.c_ulonglong]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -234,6 +237,20 @@ class Grid:
             return
         self._mlp = np.ascontiguousarray(params, dtype=np.float32)
         self._check(self.L.gammahip_mlp_load(self.h, _p(self._mlp)))
+
+    # ---- exact state (binary resume) ----
+    def save_state(self) -> np.ndarray:
+        """The grid's exact state between steps as bytes (gammahip_state_save)."""
+        n = C.c_ulonglong()
+        self._check(self.L.gammahip_state_size(self.h, C.byref(n)))
+        buf = np.zeros(n.value, dtype=np.uint8)
+        self._check(self.L.gammahip_state_save(self.h, _p(buf), n.value))
+        return buf
+
+    def restore_state(self, buf: np.ndarray):
+        """Continue from a state saved by a grid of the same setup."""
+        b = np.ascontiguousarray(buf, dtype=np.uint8)
+        self._check(self.L.gammahip_state_restore(self.h, _p(b), b.size))
 
     # ---- state access ----
     def cells(self, field: int) -> np.ndarray:

@@ -152,8 +152,10 @@ def infer(params: np.ndarray, states: np.ndarray, mfma: bool = True) -> np.ndarr
 
 
 def c2p_batch(states: np.ndarray, guess: np.ndarray, gamma: float, guess2: np.ndarray | None = None,
-              solver: int = H.C2P_NEWTON, maxit: int = 100) -> dict:
-    """FP64 cons2prim of states [n][>=3] from the given starting pressures."""
+              solver: int = H.C2P_NEWTON, maxit: int = 100, lib_path: str = H.LIB_PATH) -> dict:
+    """FP64 cons2prim of states [n][>=3] from the given starting pressures
+    (the kernels' cons2prim; ``lib_path`` may name the CPU build of the same
+    bodies)."""
     st = np.ascontiguousarray(states[:, :3], dtype=np.float64)
     n = st.shape[0]
     g = np.ascontiguousarray(guess, dtype=np.float64)
@@ -162,8 +164,8 @@ def c2p_batch(states: np.ndarray, guess: np.ndarray, gamma: float, guess2: np.nd
     it = np.zeros(n, dtype=np.int32)
     fl = np.zeros(n, dtype=np.int32)
     us = np.zeros(n, dtype=np.int32)
-    r = _lib().gammahip_c2p_batch(_p(st), _p(g), None if g2 is None else _p(g2), n, gamma, solver, maxit,
-                                  _p(p), _p(it), _p(fl), _p(us))
+    r = H.load_library(lib_path).gammahip_c2p_batch(_p(st), _p(g), None if g2 is None else _p(g2), n, gamma,
+                                                    solver, maxit, _p(p), _p(it), _p(fl), _p(us))
     if r != 0:
         raise RuntimeError("gammahip_c2p_batch failed")
     return {"p": p, "iters": it, "fail": fl, "used": us}

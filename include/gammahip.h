@@ -113,6 +113,24 @@ int  gammahip_apply(gammahip_ctx* ctx, int what);              /* Grid::apply(&C
  * face fields: 1000 x, 1001 v, 1002 lL, 1003 lR, 1010+q flux. */
 int  gammahip_download(gammahip_ctx* ctx, int field, double* out /* [ntracks][nax] */);
 int  gammahip_track_info(gammahip_ctx* ctx, int* ntrack, int* nact, int* iLbnd, int* iRbnd);
+
+/* Exact state between two steps, for resuming a run bit for bit: the time
+ * and dt of every track, the track lengths, the primitive, position and width
+ * planes of all cells, the face positions and velocities, the cells' local
+ * time steps (the next step's CFL minimum still sees the last stage's), with
+ * shock detection Sd, pspec and isShocked, and in 2D the gradients, F1
+ * neighbour lists and F1 face values (the face pass reads the upper track's
+ * transverse gradient of the previous stage, ghost tracks keep the lists of
+ * earlier stages).
+ * The reference resumes from its text snapshots (Simu::reinitialise,
+ * src/IO/text1d.cpp:103-134, src/IO/text2d.cpp:108-145), which round the state
+ * to the printed digits; this is the exact form of the same step.  Restore
+ * into a context created with the same config and loaded with the same setup
+ * and hooks (they carry the per-track boundary and regrid parameters); the
+ * buffer's dimensions are checked against the context's.  Synchronous. */
+int  gammahip_state_size(gammahip_ctx* ctx, unsigned long long* nbytes);
+int  gammahip_state_save(gammahip_ctx* ctx, void* buf, unsigned long long nbytes);
+int  gammahip_state_restore(gammahip_ctx* ctx, const void* buf, unsigned long long nbytes);
 int  gammahip_dims(gammahip_ctx* ctx, int* ntracks, int* nax);
 int  gammahip_time(gammahip_ctx* ctx, double* t /* [ntracks] */, double* dt /* [ntracks] */);
 int  gammahip_stats_get(gammahip_ctx* ctx, gammahip_stats* st);

@@ -147,8 +147,22 @@ static void fs_state2flux(const og_config *c, FS *s, double r) {
 }
 
 /* residual f(p): rel_sph.cpp:191-210 (with fabs), rel_cart.cpp:108-123 */
-static double c2p_f(const og_config *c, double p, double D, double S, double E) {
+static double c2p_f(const og_config *c, double p, double D, double S, double E, double tau) {
   double lfac;
+  if (c->c2p_form == 1) {
+    /* W - 1 = (1 - a) W^2 / (W + 1), a = |1 - v^2| */
+    double v2 = (S * S) / ((E + p) * (E + p));
+    double a = c->geom == OG_GEOM_SPH ? fabs(1. - v2) : 1. - v2;
+    double W2 = 1. / a, W = sqrt(W2);
+    return tau + p - D * ((1. - a) * W2 / (W + 1.)) - c->gamma / (c->gamma - 1.) * p * W2;
+  }
+  if (c->c2p_form == 2) {
+    long double El = (long double)D + (long double)tau, Ep = El + p;
+    long double w = 1.L - ((long double)S * S) / (Ep * Ep);
+    if (c->geom == OG_GEOM_SPH) w = fabsl(w);
+    long double W = 1.L / sqrtl(w), g = c->gamma;
+    return (double)(El + p - D * W - (g * p * W * W) / (g - 1.L));
+  }
   if (c->geom == OG_GEOM_SPH) lfac = 1. / sqrt(fabs(1. - (S * S) / ((E + p) * (E + p))));
   else                        lfac = 1. / sqrt(1. - (S * S) / ((E + p) * (E + p)));
   double gma = c->gamma;
@@ -159,10 +173,10 @@ static double c2p_f(const og_config *c, double p, double D, double S, double E) 
  * Contract used by rel_sph.cpp:273-285: iterate; root = best estimate b;
  * bracket = [min(b,c), max(b,c)]; stop when the bracket passes the interval
  * test with epsabs 0, epsrel 1e-13, or after maxit iterations. */
-static double brent_root(const og_config *c, double D, double S, double E,
+static double brent_root(const og_config *c, double D, double S, double E, double tau,
                          double lo, double hi, int maxit, int *iters, int *hitmax) {
   double tolr = c->c2p_tol > 0 ? c->c2p_tol : 1.e-13;
-  double a = lo, b = hi, fa = c2p_f(c, lo, D, S, E), fb = c2p_f(c, hi, D, S, E);
+  double a = lo, b = hi, fa = c2p_f(c, lo, D, S, E, tau), fb = c2p_f(c, hi, D, S, E, tau);
   double cc = b, fc = fb, d = b - a, e = b - a;
   double root = 0.5 * (lo + hi), xl = lo, xu = hi;
   int it = 0;
@@ -205,7 +219,7 @@ static double brent_root(const og_config *c, double D, double S, double E,
       }
       a = b; fa = fb;
       if (fabs(d) > tol) b += d; else b += (m > 0 ? +tol : -tol);
-      fb = c2p_f(c, b, D, S, E);
+      fb = c2p_f(c, b, D, S, E, tau);
       root = b;
       if ((fb < 0 && fc < 0) || (fb > 0 && fc > 0)) cc = a;
       if (b < cc) { xl = b; xu = cc; } else { xl = cc; xu = b; }
@@ -236,18 +250,18 @@ static int fs_cons2prim(const og_config *c, FS *s, double r, long *cnt) {
   double S = sqrt(S2);
   double pCand = sph ? fmax(1.e-13, s->prim[OG_PPP]) : s->prim[OG_PPP];
   int iters = 0;
-  double f_init = c2p_f(c, pCand, D, S, E);
+  double f_init = c2p_f(c, pCand, D, S, E, tau);
   if (cnt) cnt[0]++;
   if (fabs(f_init) > 0.) {
     double p_lo = fmax(0., (1. + 1e-13) * S - E), p_hi;
     int find_p = 1;
-    if (c2p_f(c, p_lo, D, S, E) < 0) find_p = 0;
+    if (c2p_f(c, p_lo, D, S, E, tau) < 0) find_p = 0;
     if (f_init < 0) p_hi = 1. * pCand;
     else {
       p_hi = pCand;
       /* rel_cart.cpp:180 loops forever when p_hi == 0 and f(0) > 0; capped here */
       int guard = 0;
-      while (c2p_f(c, p_hi, D, S, E) > 0) {
+      while (c2p_f(c, p_hi, D, S, E, tau) > 0) {
         p_hi *= 10;
         if (++guard > 400) { find_p = 0; if (cnt) cnt[6]++; break; }
       }
@@ -255,7 +269,7 @@ static int fs_cons2prim(const og_config *c, FS *s, double r, long *cnt) {
     if (p_hi < p_lo) find_p = 0;
     if (find_p) {
       int hitmax;
-      pCand = brent_root(c, D, S, E, p_lo, p_hi, c->c2p_maxiter, &iters, &hitmax);
+      pCand = brent_root(c, D, S, E, tau, p_lo, p_hi, c->c2p_maxiter, &iters, &hitmax);
       if (cnt) { cnt[1] += iters; if (hitmax) cnt[3]++; }
     } else {
       iters = -1;
@@ -1472,7 +1486,7 @@ int og_cons2prim(const og_config *c, double *prim_io, double *cons_io, double r)
   return it;
 }
 double og_c2p_residual(const og_config *c, double p, double D, double S, double E) {
-  return c2p_f(c, p, D, S, E);
+  return c2p_f(c, p, D, S, E, E - D);
 }
 void og_face_flux(const og_config *c, int dim, const double *primL, const double *primR,
                   double xf, double v, double dA, double *lam, double *flux) {

@@ -54,6 +54,13 @@ typedef struct {
   double c2p_tol;     /* Brent interval tolerance, 1e-13 in the reference (rel_sph.cpp:281);
                          other values only measure how much the reference's own
                          tolerance moves its trajectories */
+  int    c2p_form;    /* 0: the reference's residual E + p - D W - G/(G-1) p W^2;
+                         1: the same function written tau + p - D (W - 1) - G/(G-1) p W^2,
+                         which rounds differently;
+                         2: the reference's residual in long double (E = D + tau unrounded),
+                         with c2p_tol 1e-16 the root to the last place of a double:
+                         1 and 2 only measure how far rounding inside the residual
+                         moves the reference's trajectories */
 } og_config;
 
 typedef struct og_grid og_grid;

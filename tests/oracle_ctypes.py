@@ -28,11 +28,11 @@ class OgConfig(C.Structure):
         ("kin_mode", C.c_int), ("v_in", C.c_double), ("v_out", C.c_double),
         ("bnd_mode", C.c_int), ("bm", C.c_double * 6),
         ("rg_mode", C.c_int), ("rg", C.c_double * 6),
-        ("c2p_maxiter", C.c_int), ("c2p_tol", C.c_double),
+        ("c2p_maxiter", C.c_int), ("c2p_tol", C.c_double), ("c2p_form", C.c_int),
     ]
 
 
-def make_config(rc, c2p_tol=1e-13) -> OgConfig:
+def make_config(rc, c2p_tol=1e-13, c2p_form=0) -> OgConfig:
     c = OgConfig()
     c.nd, c.geom, c.nq, c.shock, c.sync = rc.nd, rc.geom, rc.nq, rc.shock, rc.sync
     c.ngst, c.nmax, c.ncell_mv, c.ncell_f1 = rc.ngst, rc.nmax, rc.ncell_mv, rc.ncell_f1
@@ -47,6 +47,7 @@ def make_config(rc, c2p_tol=1e-13) -> OgConfig:
     c.rg_mode = rc.rg_mode
     c.c2p_maxiter = rc.c2p_maxiter
     c.c2p_tol = c2p_tol
+    c.c2p_form = c2p_form
     return c
 
 
@@ -99,10 +100,10 @@ I_X, I_V, I_LL, I_LR, I_LS, I_DA, I_FLUX0 = 0, 1, 2, 3, 4, 5, 10
 class OracleGrid:
     """Grid + Simu::run loop of the oracle for one setup."""
 
-    def __init__(self, setup, c2p_tol=1e-13):
+    def __init__(self, setup, c2p_tol=1e-13, c2p_form=0):
         self.setup = setup
         self.rc = setup.cfg
-        self.cfg = make_config(self.rc, c2p_tol)
+        self.cfg = make_config(self.rc, c2p_tol, c2p_form)
         self.h = lib().og_create(C.byref(self.cfg))
         if not self.h:
             raise MemoryError("oracle allocation failed")

@@ -142,16 +142,13 @@ def test_training_converges_and_gains_over_cold_start():
           f"cold start {fc.sum()} (bits {np.unique(rc['fail'])}) of {len(test)}")
     assert fm.sum() <= fc.sum() and fm.sum() <= 1e-3 * len(test)
     ok = ~(fm | fc)
+    # Newton returns the double below the root of the conserved state whatever
+    # its start: both starts give the same bits, a root to working precision
+    assert np.array_equal(rm["p"][ok], rc["p"][ok])
     D, S, tau = test[ok, 0], test[ok, 1], test[ok, 2]
-    pm, pc = rm["p"][ok], rc["p"][ok]
     eps = np.finfo(np.float64).eps
-    for pp in (pm, pc):
-        f, _, fs = R.c2p_residual(pp, D, S, tau, G53)
-        assert np.all(np.abs(f) <= 16 * eps * fs)          # a root to working precision
-    # both starts reach the same root, to the conditioning of the state
-    _, df, fs = R.c2p_residual(pc, D, S, tau, G53)
-    cond = fs / np.abs(pc * df)
-    assert np.all(np.abs(pm - pc) <= (128 * eps * cond + 1e-13) * pc)
+    f, _, fs = R.c2p_residual(rc["p"][ok], D, S, tau, G53)
+    assert np.all(np.abs(f) <= 16 * eps * fs)
     it_m, it_c = rm["iters"][ok].mean(), rc["iters"][ok].mean()
     print(f"residual evaluations per state: network start {it_m:.2f}, cold start {it_c:.2f}, "
           f"final loss {loss[-100:].mean():.4g}")
