@@ -144,7 +144,7 @@ def run_other(setup, steps, warmup, options=None, solver=None, surrogate=None):
 def other_configs(steps, warmup):
     """The uneven-track configurations: the 2D spherical jet blast wave (1024
     theta tracks, seeded lengths 500-4000) and the ensemble of 256 1D runs
-    (512-16384 cells), each plain and batched (Newton cons2prim, launch
+    (512-16384 cells), each plain and batched (cons2prim solver and launch
     options -- packed persistent grids, XCD order, graphs, block size --
     picked by the autotuner on a scratch copy)."""
     from gamma_amd import autotune, hip, setups
@@ -163,8 +163,8 @@ def other_configs(steps, warmup):
                         steps=3000, batch=4096, lr=2e-3, seed=5)
         tuned = autotune.tune(scratch, steps=4, settle=2, sweeps=1, surrogate=w)
         scratch.close()
-        batched = {k: tuned["best"][k] for k in ("block", "remap", "graph", "persist", "occ", "f1fuse", "mlp_thr")}
-        b = run_other(s, steps, warmup, batched, hip.C2P_NEWTON,
+        batched = {k: tuned["best"][k] for k in autotune.OPTION_KEYS}
+        b = run_other(s, steps, warmup, batched, batched["solver"],
                       surrogate=w if batched["mlp_thr"] is not None else None)
         out[name] = {"plain": p, "batched": dict(b, options=batched),
                      "speedup": b["value"] / p["value"]}
@@ -256,15 +256,16 @@ def main():
         print("[bench] autotuning the batched form", file=sys.stderr, flush=True)
         tuned = autotune.tune(scratch, surrogate=w)
         scratch.close()
-        opts = {k: tuned["best"][k] for k in ("block", "remap", "graph", "persist", "occ", "f1fuse", "mlp_thr")}
-        b = run_form(s, hip.C2P_NEWTON, opts, args.steps, args.warmup, args.timing_steps,
+        opts = {k: tuned["best"][k] for k in autotune.OPTION_KEYS}
+        b = run_form(s, opts["solver"], opts, args.steps, args.warmup, args.timing_steps,
                      surrogate=w if opts["mlp_thr"] is not None else None)
         print(f"[bench] batched {b['ms_per_step']:.3f} ms/step with {opts}", file=sys.stderr, flush=True)
         bst = b["stats"]
         batched = {"value": b["value"], "ms_per_step": b["ms_per_step"], "options": opts,
-                   "cons2prim": "safeguarded Newton, previous-pressure start"
-                                + ("" if opts["mlp_thr"] is None else
-                                   ", learned start judged against it by |f|"),
+                   "cons2prim": ("Brent (the reference's algorithm)" if opts["solver"] == hip.C2P_BRENT else
+                                 "Newton to the rounded exact root, previous-pressure start"
+                                 + ("" if opts["mlp_thr"] is None else
+                                    ", learned start judged against it by |f|")),
                    "speedup_vs_plain": b["value"] / plain["value"],
                    "c2p_iters_per_call": bst["c2p_iters"] / max(bst["c2p_calls"], 1),
                    "network_starts": bst["mlp_used"], "c2p_calls": bst["c2p_calls"],

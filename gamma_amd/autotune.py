@@ -4,9 +4,10 @@ Options: threads per workgroup of the per-cell kernels (64, 128, 256),
 XCD-aware tile order (off/on), two-step hipGraph replay (off/on), persistent
 packed launch (off, 4 or 8 workgroups per CU), the register-capped builds of
 the F1 face kernel and of the update kernel (bit set, 0-3), the F1 flux sums
-inside the update kernel (off/on) and, when a trained
-network is given, the learned
-cons2prim start (off, or on with trigger exponents 2 and 4).
+inside the update kernel (off/on), the cons2prim solver (Brent, the
+reference's algorithm, or Newton) and, when a trained network is given, the
+learned cons2prim start (off, or on with trigger exponents 2 and 4; Newton
+only).
 
 Search: coordinate descent from the defaults -- each option in turn is set to
 each of its values with the others held, the fastest kept -- repeated for
@@ -21,18 +22,19 @@ from __future__ import annotations
 
 import time
 
-SPACE = (("block", (256, 128, 64)), ("remap", (0, 1)), ("graph", (0, 1)),
+SPACE = (("solver", (0, 1)), ("block", (256, 128, 64)), ("remap", (0, 1)), ("graph", (0, 1)),
          ("persist", (0, 4, 8)), ("occ", (0, 1, 2, 3)), ("f1fuse", (0, 1)))
-DEFAULTS = {"block": 256, "remap": 0, "graph": 0, "persist": 0, "occ": 0, "f1fuse": 0, "mlp_thr": None}
+DEFAULTS = {"solver": 1, "block": 256, "remap": 0, "graph": 0, "persist": 0, "occ": 0, "f1fuse": 0,
+            "mlp_thr": None}
+OPTION_KEYS = tuple(DEFAULTS)
 
 
 def apply(grid, cand: dict, surrogate=None):
-    grid.set_options(block=cand["block"], remap=cand["remap"], graph=cand["graph"],
+    if cand.get("mlp_thr") is None and surrogate is not None:
+        grid.load_surrogate(None)
+    grid.set_options(solver=cand["solver"], block=cand["block"], remap=cand["remap"], graph=cand["graph"],
                      persist=cand["persist"], occ=cand["occ"], f1fuse=cand["f1fuse"])
-    if cand.get("mlp_thr") is None:
-        if surrogate is not None:
-            grid.load_surrogate(None)
-    else:
+    if cand.get("mlp_thr") is not None:
         grid.load_surrogate(surrogate)
         grid.set_options(mlp_thr=cand["mlp_thr"])
 
@@ -58,6 +60,8 @@ def tune(grid, steps: int = 6, settle: int = 2, sweeps: int = 2, surrogate=None,
                 if v == best[key]:
                     continue
                 cand = dict(best, **{key: v})
+                if cand["mlp_thr"] is not None and cand["solver"] != 1:
+                    continue                     # the learned start seeds Newton only
                 t_inc = _time(grid, best, surrogate, steps, settle)
                 t_new = _time(grid, cand, surrogate, steps, settle)
                 table.append(dict(cand, ms_per_step=round(t_new, 4), incumbent_ms=round(t_inc, 4)))

@@ -2227,6 +2227,10 @@ int gammahip_set_option(gammahip_ctx* c, int key, long long value) {
       c->d.packed = value > 0;
       build_tiles(c);
       return post(c);
+    case GAMMAHIP_OPT_SOLVER:
+      if (value != gh::C2P_BRENT && value != gh::C2P_NEWTON) return fail(c, "solver must be 0 (Brent) or 1 (Newton)");
+      if (value == gh::C2P_BRENT && c->d.mw) return fail(c, "the learned guess is loaded: it seeds the Newton solver");
+      c->d.solver = (int)value; return 0;
     case GAMMAHIP_OPT_REMAP: c->remap = value ? 1 : 0; return 0;
     case GAMMAHIP_OPT_GRAPH: c->use_graph = value ? 1 : 0; return 0;
     case GAMMAHIP_OPT_MLP_THR:

@@ -108,42 +108,49 @@ GH_HD double c2p_f(double p, double D, double S, double E, double G, int geom) {
   return (E + p - D * lf - (G * p * lf * lf) / (G - 1.));
 }
 
-// Newton's residual in double-double arithmetic.  Where E = D + tau holds it
-// is the same function of p as c2p_f, written without forming E - D:
-//   f = tau + p - D (W - 1) - G/(G-1) p W^2,  W - 1 = (1 - a) W^2 / (W + 1),
-// with W^2 = 1/a, a = w = 1 - S^2/(E+p)^2 (|w| in spherical geometry, as
-// c2p_f).  Evaluated in doubles, any form of f rounds at eps times the energy
-// of the state, which for a cold gas in motion is the kinetic energy, many
-// orders above the pressure's own scale: the root is then only known to
+// Newton's residual in double-double arithmetic, scaled by a = 1/W^2 > 0
+// (same sign as f, same root).  Where E = D + tau holds, f is the same
+// function of p as c2p_f, written without forming E - D:
+//   f = tau + p - D (W - 1) - G/(G-1) p W^2,
+//   a f = a (tau + p) - D r (1 - a) / (1 + r) - G/(G-1) p,   r = sqrt(a) = 1/W,
+// with a = w = 1 - S^2/(E+p)^2 (|w| in spherical geometry, as c2p_f).
+// Evaluated in doubles, any form of f rounds at eps times the energy of the
+// state, which for a cold gas in motion is the kinetic energy, many orders
+// above the pressure's own scale: the root is then only known to
 // eps (kinetic / thermal) relative, and a root finder stops at a point of that
 // band that depends on where it started.  In double-double (the Dekker/Knuth
 // operations of ddpow.hpp, exact on host and device alike) the band shrinks
 // to eps^2 times the same ratio, below one unit in the last place of p for
 // every state the solver meets, so the sign of f at a double is reliable and
-// the root rounds to one double whatever the start.
-GH_HD double c2p_fdd(double p, double D, double S, double tau, double G, int geom) {
+// the root rounds to one double whatever the start.  gg = G/(G-1) in
+// double-double; *a_out = a rounded to a double.
+GH_HD double c2p_afdd(double p, double D, double S, double tau, dd::D2 gg, int geom, double* a_out) {
   using namespace dd;
-  D2 E = two_sum(D, tau);
-  D2 Ep = add(E, {p, 0.});
+  D2 tp = two_sum(tau, p);
+  D2 Ep = add(two_sum(D, tau), {p, 0.});
   D2 v2 = div(two_prod(S, S), mul(Ep, Ep));
   D2 a = add({1., 0.}, {-v2.hi, -v2.lo});
-  if (geom == SPH && a.hi < 0.) a = {-a.hi, -a.lo};
-  D2 W2 = div({1., 0.}, a);
-  double s = sqrt(W2.hi);                           // W = sqrt(W2): one Newton correction
+  D2 oma = v2;                                      // 1 - a
+  if (geom == SPH && a.hi < 0.) { a = {-a.hi, -a.lo}; oma = add({2., 0.}, {-v2.hi, -v2.lo}); }
+  double s = sqrt(a.hi);                            // r = sqrt(a): one Newton correction
   D2 s2 = two_prod(s, s);
-  D2 W = fast_two_sum(s, ((W2.hi - s2.hi) - s2.lo + W2.lo) / (2. * s));
-  D2 oma = add({1., 0.}, {-a.hi, -a.lo});           // 1 - a
-  D2 t1 = div(mul_d(mul(oma, W2), D), add(W, {1., 0.}));
-  D2 t2 = mul(mul_d(W2, p), div({G, 0.}, {G - 1., 0.}));
-  D2 f = add(two_sum(tau, p), {-t1.hi, -t1.lo});
-  f = add(f, {-t2.hi, -t2.lo});
-  return f.hi;
+  D2 r = fast_two_sum(s, ((a.hi - s2.hi) - s2.lo + a.lo) / (2. * s));
+  D2 t1 = mul_d(div(mul(r, oma), add({1., 0.}, r)), D);
+  D2 g = add(mul(a, tp), {-t1.hi, -t1.lo});
+  D2 t2 = mul_d(gg, p);
+  g = add(g, {-t2.hi, -t2.lo});
+  *a_out = a.hi;
+  return g.hi;
 }
+GH_HD dd::D2 c2p_gg(double G) { return dd::div({G, 0.}, {G - 1., 0.}); }
 
 // d f / d p for the Newton steps (in doubles: the derivative only sets the
-// step), the residual in doubles and the magnitude of its terms (*fscale).
+// step), the residual in doubles, the magnitude of its terms (*fscale), a
+// bound on its rounding (*fnoise: the terms holding W weighted by 1 + W^2, as
+// w, formed by cancellation, carries a relative error of about W^2 eps) and
+// a bound on the derivative's own rounding (*dferr).
 GH_HD double c2p_df(double p, double D, double S, double tau, double G, int geom, double* fval,
-                    double* fscale) {
+                    double* fscale, double* fnoise = nullptr, double* dferr = nullptr) {
   double E = D + tau;
   double Ep = E + p;
   double v2 = (S * S) / (Ep * Ep);
@@ -156,11 +163,15 @@ GH_HD double c2p_df(double p, double D, double S, double tau, double G, int geom
   double t1 = D * (om * W2 / (lf + 1.));
   double t2 = gg * p * W2;
   *fval = tau + p - t1 - t2;
-  *fscale = fabs(tau) + fabs(p) + fabs(t1) + fabs(t2);
+  double tw = fabs(t1) + fabs(t2);
+  *fscale = fabs(tau) + fabs(p) + tw;
+  if (fnoise) *fnoise = *fscale + W2 * tw;
   // d W / dp = -W^3 S^2 / (E+p)^3 (sign of w folded in for |w|)
   double dw = 2. * (S * S) / (Ep * Ep * Ep);
   double dlf = -0.5 * lf * lf * lf * (geom == SPH && w < 0 ? -dw : dw);
-  return 1. - D * dlf - gg * (W2 + 2. * p * lf * dlf);
+  double a1 = D * dlf, a2 = gg * W2, a3 = gg * 2. * p * lf * dlf;
+  if (dferr) *dferr = 16. * 2.2204460492503131e-16 * (1. + W2) * (1. + fabs(a1) + fabs(a2) + fabs(a3));
+  return 1. - a1 - (a2 + a3);
 }
 
 enum { C2P_BRENT = 0, C2P_NEWTON = 1 };
@@ -229,52 +240,111 @@ GH_HD double brent(double D, double S, double E, double G, int geom, double lo, 
   return root;
 }
 
-// Safeguarded Newton on the same bracket, on the double-double residual:
-// Newton steps from the guess, a bisection step whenever Newton leaves the
-// bracket, until a step or the bracket falls to two units in the last place.
-// The answer is then made canonical: the largest double at which f > 0 (f
-// decreases through the root), found by stepping one ulp at a time from the
-// last iterate (at most 8 steps each way).  The same state gives the same
-// bits from every start; a solve that runs out of iterations without
-// reaching the sign change is a failure (fail bit 2).
-GH_HD double newton(double D, double S, double tau, double G, int geom, double lo, double hi,
-                    double x0, int maxit, C2PStat* st) {
-  double x = (x0 >= lo && x0 <= hi) ? x0 : 0.5 * (lo + hi);
-  const double tol = 4.5e-16;
+// Newton cons2prim: the largest double p with f(p) > 0 (f decreases through
+// the root), i.e. the root rounded down, whatever the start.
+//   1. Newton on the residual in doubles (c2p_df), safeguarded by the bracket
+//      [lo, hi] the iterates establish (lo from p_lo, the reference's lower
+//      bound, hi open until f < 0 is seen), until the step falls inside that
+//      residual's rounding band;
+//   2. Newton on the double-double residual until the step is within two
+//      units in the last place;
+//   3. the decision: from f and f' at x, the sign of f one unit above (or
+//      below) x is clear whenever f' is known well enough and f's rounding is
+//      far below f' times one unit; otherwise f is evaluated there, stepping one
+//      unit at a time (at most 8 steps each way).  Where even the
+//      double-double residual is noisy at that scale (see below) the last
+//      iterate is the answer.
+// *fail gets 1 when f(p_lo) < 0 (no root above the lower bound, the
+// reference's test), 2 when the iteration or the steps run out.
+GH_HD double newton(double D, double S, double tau, double G, int geom, double p_lo, double x0, int maxit,
+                    C2PStat* st) {
+  const double eps = 2.2204460492503131e-16;
+  const dd::D2 gg = c2p_gg(G);
+  double lo = p_lo, hi = HUGE_VAL;
+  double x = x0 > p_lo && x0 < HUGE_VAL ? x0 : (p_lo > 0. ? 2. * p_lo : fmax((G - 1.) * tau, 1e-300));
   int it = 0;
+  bool ok = false;
+  // 1. doubles
   for (;;) {
     it++;
-    double f = c2p_fdd(x, D, S, tau, G, geom);
-    if (f == 0.) break;
-    if (f > 0) lo = x; else hi = x;      // f decreases through the root
-    double fd, fs;
-    double step = f / c2p_df(x, D, S, tau, G, geom, &fd, &fs);
-    // a step below the tolerance ends the iteration before the bracket test:
-    // rounded, x - step may equal x, which sits on the bracket's end
-    if (fabs(step) <= tol * fabs(x)) { x -= step; break; }
+    double f, fs, fn;
+    double df = c2p_df(x, D, S, tau, G, geom, &f, &fs, &fn);
+    if (!(f == f)) break;
+    double step = f / df;
+    // inside the rounding band the sign of f means nothing: stop before the
+    // bracket test (a zero step would sit on the bracket's end)
+    if (fabs(f) <= 4. * eps * fn || fabs(step) <= 1e-15 * fabs(x)) {
+      if (x - step > p_lo) x -= step;
+      ok = true;
+      break;
+    }
+    if (f > 0) lo = x; else hi = x;
     double xn = x - step;
-    if (!(xn > lo && xn < hi)) xn = 0.5 * (lo + hi);
+    if (!(xn > lo && xn < hi)) xn = hi < HUGE_VAL ? 0.5 * (lo + hi) : 10. * x;
     x = xn;
-    if (hi - lo <= tol * fabs(lo)) break;
+    if (hi - lo <= 1e-15 * lo) { ok = true; break; }
     if (it >= maxit) break;
   }
+  // 2. double-double
   bool found = false;
-  it++;
-  if (c2p_fdd(x, D, S, tau, G, geom) > 0) {
+  double g = 0., a = 1., f = 0., df = -1., dfe = 0., fn = 0.;
+  if (ok) {
+    // the double residual's signs inside its rounding band are not to be
+    // trusted: start the bracket afresh
+    ok = false;
+    lo = p_lo;
+    hi = HUGE_VAL;
     for (int k = 0; k < 8; ++k) {
-      double xu = nextafter(x, HUGE_VAL);
       it++;
-      if (!(c2p_fdd(xu, D, S, tau, G, geom) > 0)) { found = true; break; }
-      x = xu;
-    }
-  } else {
-    for (int k = 0; k < 8 && x > 0.; ++k) {
-      x = nextafter(x, 0.);
-      it++;
-      if (c2p_fdd(x, D, S, tau, G, geom) > 0) { found = true; break; }
+      g = c2p_afdd(x, D, S, tau, gg, geom, &a);
+      double fd, fs;
+      df = c2p_df(x, D, S, tau, G, geom, &fd, &fs, &fn, &dfe);
+      f = g / a;
+      double step = f / df;
+      double up = nextafter(x, HUGE_VAL) - x;
+      if (fabs(step) <= 2. * up || fabs(f) <= 4. * eps * eps * fn) { ok = true; break; }
+      if (g > 0) lo = fmax(lo, x); else hi = fmin(hi, x);
+      double xn = x - step;
+      if (!(xn > lo && xn < hi)) xn = hi < HUGE_VAL ? 0.5 * (lo + hi) : 10. * x;
+      x = xn;
     }
   }
-  if (!found) st->fail |= 2;
+  // 3. the root rounded down
+  double x2 = x;
+  if (ok) {
+    double up = nextafter(x, HUGE_VAL) - x, dn = x - nextafter(x, 0.);
+    double m = 0.25 * fabs(df) * fmin(up, dn);
+    bool sure = df < 0. && dfe < 0.1 * fabs(df) && 4. * eps * eps * fn < 0.05 * m;
+    if (sure && g > 0. && f + df * up < -m) found = true;                     // f(x + up) < 0
+    else if (sure && !(g > 0.) && f - df * dn > m) { x -= dn; found = true; }  // f(x - dn) > 0
+    else if (g > 0.) {
+      for (int k = 0; k < 8; ++k) {
+        double xu = nextafter(x, HUGE_VAL), au;
+        it++;
+        if (!(c2p_afdd(xu, D, S, tau, gg, geom, &au) > 0.)) { found = true; break; }
+        x = xu;
+      }
+    } else {
+      for (int k = 0; k < 8 && x > 0.; ++k) {
+        double ad;
+        x = nextafter(x, 0.);
+        it++;
+        if (c2p_afdd(x, D, S, tau, gg, geom, &ad) > 0.) { found = true; break; }
+      }
+    }
+  }
+  // Near W ~ 1e3 with p/rho ~ 1e-10 the double-double residual itself rounds
+  // at eps^2 W^2 times its terms, which spans a few units in the last place
+  // of p: no double is the rounded root, and the point where f reached that
+  // floor is the answer (it depends on the start by those few units).
+  if (!found && ok && fabs(f) <= 8. * eps * eps * fn) { x = x2; found = true; }
+  // the reference's bracket test: no root above p_lo
+  if (!found || x < 2. * p_lo) {
+    double al;
+    it++;
+    if (c2p_afdd(p_lo, D, S, tau, gg, geom, &al) < 0.) st->fail |= 1;
+    else if (!found) st->fail |= 2;
+  }
   st->iters += it;
   return x;
 }
@@ -295,9 +365,9 @@ GH_HD void c2p_inputs(const double* cons, double r, int geom, double* D, double*
 // FluidState::cons2prim  rel_sph.cpp:213-320 / rel_cart.cpp:125-229.
 // prim[PPP] on entry is the previous pressure (the reference's guess).
 // guess_override > 0 (Newton only) is a second start, e.g. the learned guess:
-// the solve starts from whichever of the two has the smaller residual |f|
-// (Newton's double-double form), so the previous pressure stays the fallback
-// and the judge.  Newton's answer does not depend on its start (see newton).
+// the solve starts from whichever of the two has the smaller residual |f|,
+// so the previous pressure stays the fallback and the judge.  Newton's answer
+// does not depend on its start (see newton).
 // cons on exit = prim2cons(prim) (rel_sph.cpp:318).
 template <int ND, int NQ>
 GH_HD void cons2prim(double* prim, double* cons, double r, int geom, double G, int solver,
@@ -310,41 +380,46 @@ GH_HD void cons2prim(double* prim, double* cons, double r, int geom, double G, i
   double S = sqrt(S2);
   double pprev = geom == SPH ? fmax(1.e-13, prim[PPP]) : prim[PPP];
   double pCand = pprev;
-  double start = pCand;
-  bool nw = solver == C2P_NEWTON;
-  // Brent: the reference's residual; Newton: its double-double form
-  auto F = [&](double pp) { return nw ? c2p_fdd(pp, D, S, tau, G, geom) : c2p_f(pp, D, S, E, G, geom); };
-  if (guess_override > 0. && nw) {
-    st->evals += 2;
-    if (fabs(F(guess_override)) < fabs(F(pCand))) {
-      start = guess_override;
-      st->used++;
+  if (solver == C2P_NEWTON) {
+    double start = pCand;
+    if (guess_override > 0.) {
+      // judge the two starts by the residual in doubles (the answer does not
+      // depend on the start, only the iteration count does)
+      double f1, f2, s1, s2;
+      st->evals += 2;
+      (void)c2p_df(guess_override, D, S, tau, G, geom, &f1, &s1);
+      (void)c2p_df(pCand, D, S, tau, G, geom, &f2, &s2);
+      if (fabs(f1) < fabs(f2)) { start = guess_override; st->used++; }
     }
-  }
-  double f_init = F(start);
-  if (nw) pCand = start;
-  if (fabs(f_init) > 0. || nw) {
-    double p_lo = fmax(0., (1. + 1e-13) * S - E), p_hi;
-    bool find_p = true;
-    st->evals += 2;
-    if (F(p_lo) < 0) find_p = false;
-    if (f_init < 0) p_hi = 1. * pCand;
-    else {
-      p_hi = pCand;
-      int guard = 0;
-      while (F(p_hi) > 0) {
-        st->evals++;
-        p_hi *= 10;
-        if (++guard > 400) { find_p = false; st->fail |= 4; break; }
+    double p_lo = fmax(0., (1. + 1e-13) * S - E);
+    int f0 = st->fail;
+    st->fail = 0;
+    pCand = newton(D, S, tau, G, geom, p_lo, start, 100, st);
+    if (st->fail & 1) pCand = pprev;                 // no bracket: keep the previous pressure
+    st->fail |= f0;
+  } else {
+    double f_init = c2p_f(pCand, D, S, E, G, geom);
+    if (fabs(f_init) > 0.) {
+      double p_lo = fmax(0., (1. + 1e-13) * S - E), p_hi;
+      bool find_p = true;
+      st->evals += 2;
+      if (c2p_f(p_lo, D, S, E, G, geom) < 0) find_p = false;
+      if (f_init < 0) p_hi = 1. * pCand;
+      else {
+        p_hi = pCand;
+        int guard = 0;
+        while (c2p_f(p_hi, D, S, E, G, geom) > 0) {
+          st->evals++;
+          p_hi *= 10;
+          if (++guard > 400) { find_p = false; st->fail |= 4; break; }
+        }
       }
-    }
-    if (p_hi < p_lo) find_p = false;
-    if (find_p) {
-      if (!nw) pCand = brent(D, S, E, G, geom, p_lo, p_hi, maxit, st);
-      else pCand = newton(D, S, tau, G, geom, p_lo, p_hi, pCand, 100, st);
-    } else {
-      st->fail |= 1;
-      pCand = pprev;
+      if (p_hi < p_lo) find_p = false;
+      if (find_p) pCand = brent(D, S, E, G, geom, p_lo, p_hi, maxit, st);
+      else {
+        st->fail |= 1;
+        pCand = pprev;
+      }
     }
   }
   double p = pCand;

@@ -4,8 +4,11 @@
 // (initialGeometry / initialValues / userKinematics), advanced by
 // gammahip::Simu, i.e. by the reference's Grid method sequence.
 //
-// usage: shock_tube [ncells=200] [nsteps=100] [mode=methods|step]
-// prints the final time and, per active cell, x rho p u as C99 hex floats.
+// usage: shock_tube [ncells=200] [nsteps=100] [mode=methods|step|split] [dir]
+// prints the final time and, per active cell, x rho p u as C99 hex floats;
+// mode split runs nsteps/2 steps, saves the exact state, restores it into a
+// second grid and runs the rest there; with dir, Grid::printCols writes
+// dir/phys%010d.out of the final state.
 #include "gammahip_grid.hpp"
 
 #include <cstdio>
@@ -16,6 +19,8 @@ int main(int argc, char** argv) {
   int n = argc > 1 ? atoi(argv[1]) : 200;
   long nsteps = argc > 2 ? atol(argv[2]) : 100;
   bool by_step = argc > 3 && strcmp(argv[3], "step") == 0;
+  bool split = argc > 3 && strcmp(argv[3], "split") == 0;
+  const char* dir = argc > 4 ? argv[4] : nullptr;
   if (n < 8 || nsteps < 0) { fprintf(stderr, "bad arguments\n"); return 2; }
   gammahip_config c{};
   c.nd = 1; c.nq = 6; c.geom = 0; c.shock = 0; c.sync = 0; c.ngst = 2;
@@ -40,15 +45,24 @@ int main(int argc, char** argv) {
     h.kin_mode = 1; h.v_in = 0.; h.v_out = 0.; h.rg_mode = -1;
     g.prepForRun(x, dx, y, dy, prim, {h}, 0.);
     gammahip::Simu sim{g, false, false, c.cfl};
+    gammahip::Grid g2(c);
+    gammahip::Grid* out = &g;
     if (by_step) g.step((int)nsteps);
-    else sim.run(nsteps);
+    else if (split) {
+      g.step((int)(nsteps / 2));
+      g2.prepForRun(x, dx, y, dy, prim, {h}, 0.);
+      g2.restoreState(g.saveState());
+      g2.step((int)(nsteps - nsteps / 2));
+      out = &g2;
+    } else sim.run(nsteps);
     std::vector<int> nt, na, iL, iR;
-    g.tracks(nt, na, iL, iR);
-    auto X = g.download(200), R = g.download(0), P = g.download(1), U = g.download(2);
+    out->tracks(nt, na, iL, iR);
+    auto X = out->download(200), R = out->download(0), P = out->download(1), U = out->download(2);
     double t[1], dt[1];
-    if (gammahip_time(g.handle(), t, dt) != 0) throw gammahip::Error("gammahip_time");
+    if (gammahip_time(out->handle(), t, dt) != 0) throw gammahip::Error("gammahip_time");
     printf("t %a\n", t[0]);
     for (int i = iL[0] + 1; i < iR[0]; ++i) printf("%a %a %a %a\n", X[i], R[i], P[i], U[i]);
+    if (dir) out->printCols((int)nsteps, t[0], dir);
   } catch (const std::exception& e) {
     fprintf(stderr, "shock_tube: %s\n", e.what());
     return 1;

@@ -155,10 +155,12 @@ int  gammahip_sync(gammahip_ctx* ctx);
  * per SIMD), bit 1 the update kernel in one capped at 96 (5 wavefronts),
  * spilling what does not fit; F1FUSE 1 sums each cell's F1 face fluxes inside
  * the update kernel instead of in a pass of its own (fewer bytes, a longer
- * dependent-load chain in the update). */
+ * dependent-load chain in the update); SOLVER switches cons2prim between
+ * Brent (0, the reference's algorithm) and Newton (1; required while the
+ * learned guess is loaded). */
 enum { GAMMAHIP_OPT_BLOCK = 0, GAMMAHIP_OPT_REMAP = 1, GAMMAHIP_OPT_GRAPH = 2,
        GAMMAHIP_OPT_MLP_THR = 3, GAMMAHIP_OPT_PERSIST = 4, GAMMAHIP_OPT_OCC = 5,
-       GAMMAHIP_OPT_F1FUSE = 6 };
+       GAMMAHIP_OPT_F1FUSE = 6, GAMMAHIP_OPT_SOLVER = 7 };
 int  gammahip_set_option(gammahip_ctx* ctx, int key, long long value);
 
 /* ---- learned cons2prim guess (no counterpart in the reference) ----
