@@ -1112,12 +1112,14 @@ template <int NQ> GH_KERNEL2P(k_f1sum, body_f1sum<NQ>)
 // W = minimum wavefronts per SIMD the compiler must allow (registers capped
 // accordingly, spilling if it must): 1 leaves the choice to the compiler, 4
 // trades spills for latency hiding.  The launch option picks one.
+// (XCD-aware with remap, as the cell passes: the face reads the upper track)
 template <int NQ, int W>
-__global__ void __launch_bounds__(KB, W) k_f1faces(Dev d) {
-  int j = blockIdx.y;
+__global__ void __launch_bounds__(KB, W) k_f1faces(Dev d, int remap) {
+  int j, s0;
+  tile_of(remap, j, s0);
   int stride = gridDim.x * blockDim.x;
   int tot = min(d.f1tot[j], d.fcap);
-  for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < tot; s += stride) body_f1face<NQ>(d, j, s);
+  for (int s = s0; s < tot; s += stride) body_f1face<NQ>(d, j, s);
 }
 // W: minimum wavefronts per SIMD (1 = the compiler's choice; 5 caps the
 // registers at 96 and spills the rest), picked by the launch option
@@ -1816,8 +1818,8 @@ struct Pipe {
       {
         int bs = c->block;
         dim3 g((d.nax + bs - 1) / bs, d.nj);
-        if (c->occ & 1) hipLaunchKernelGGL((k_f1faces<NQ, 4>), g, dim3(bs), 0, c->stream, d);
-        else hipLaunchKernelGGL((k_f1faces<NQ, 1>), g, dim3(bs), 0, c->stream, d);
+        if (c->occ & 1) hipLaunchKernelGGL((k_f1faces<NQ, 4>), g, dim3(bs), 0, c->stream, d, c->remap);
+        else hipLaunchKernelGGL((k_f1faces<NQ, 1>), g, dim3(bs), 0, c->stream, d, c->remap);
       }
 #endif
       tend(c, 2);

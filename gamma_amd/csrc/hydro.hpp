@@ -245,7 +245,9 @@ GH_HD double brent(double D, double S, double E, double G, int geom, double lo, 
 //   1. Newton on the residual in doubles (c2p_df), safeguarded by the bracket
 //      [lo, hi] the iterates establish (lo from p_lo, the reference's lower
 //      bound, hi open until f < 0 is seen), until the step falls inside that
-//      residual's rounding band;
+//      residual's rounding band; where that band is far narrower than f' times
+//      one unit, the decision of step 3 is taken on the doubles and the
+//      solve ends here;
 //   2. Newton on the double-double residual until the step is within two
 //      units in the last place;
 //   3. the decision: from f and f' at x, the sign of f one unit above (or
@@ -285,10 +287,25 @@ GH_HD double newton(double D, double S, double tau, double G, int geom, double p
     if (hi - lo <= 1e-15 * lo) { ok = true; break; }
     if (it >= maxit) break;
   }
-  // 2. double-double
   bool found = false;
-  double g = 0., a = 1., f = 0., df = -1., dfe = 0., fn = 0.;
+  // 1b. where the double residual's rounding is far below f' times one unit
+  // (hot or static gas: the thermal energy is not small next to the kinetic
+  // one), the doubles already decide the sign of f one unit either side of x
+  bool dd_needed = true;
   if (ok) {
+    it++;
+    double f1, fs1, fn1, dfe1;
+    double df1 = c2p_df(x, D, S, tau, G, geom, &f1, &fs1, &fn1, &dfe1);
+    double up = nextafter(x, HUGE_VAL) - x, dn = x - nextafter(x, 0.);
+    double m = 0.25 * fabs(df1) * fmin(up, dn), nb = 4. * eps * fn1;
+    if (df1 < 0. && dfe1 < 0.1 * fabs(df1) && nb < 0.05 * m) {
+      if (f1 > nb && f1 + df1 * up < -(m + nb)) { found = true; dd_needed = false; }
+      else if (f1 < -nb && f1 - df1 * dn > m + nb) { x -= dn; found = true; dd_needed = false; }
+    }
+  }
+  // 2. double-double
+  double g = 0., a = 1., f = 0., df = -1., dfe = 0., fn = 0.;
+  if (ok && dd_needed) {
     // the double residual's signs inside its rounding band are not to be
     // trusted: start the bracket afresh
     ok = false;
@@ -311,7 +328,7 @@ GH_HD double newton(double D, double S, double tau, double G, int geom, double p
   }
   // 3. the root rounded down
   double x2 = x;
-  if (ok) {
+  if (ok && dd_needed) {
     double up = nextafter(x, HUGE_VAL) - x, dn = x - nextafter(x, 0.);
     double m = 0.25 * fabs(df) * fmin(up, dn);
     bool sure = df < 0. && dfe < 0.1 * fabs(df) && 4. * eps * eps * fn < 0.05 * m;
@@ -337,7 +354,7 @@ GH_HD double newton(double D, double S, double tau, double G, int geom, double p
   // at eps^2 W^2 times its terms, which spans a few units in the last place
   // of p: no double is the rounded root, and the point where f reached that
   // floor is the answer (it depends on the start by those few units).
-  if (!found && ok && fabs(f) <= 8. * eps * eps * fn) { x = x2; found = true; }
+  if (!found && ok && dd_needed && fabs(f) <= 8. * eps * eps * fn) { x = x2; found = true; }
   // the reference's bracket test: no root above p_lo
   if (!found || x < 2. * p_lo) {
     double al;
