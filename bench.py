@@ -26,7 +26,7 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0      # MI355X HBM3E, GB/s
 
 # HIP-event timed kernels (ids of gammahip_kernel_times)
-KNAMES = {0: "k_update", 1: "k_faces", 2: "k_f1faces", 3: "k_f1grad", 4: "k_f1sum"}
+KNAMES = {0: "k_update", 1: "k_faces", 2: "k_f1faces", 3: "k_f1grad", 4: "k_f1sum", 5: "k_f1cell"}
 
 
 def algorithmic_bytes(kid: int, cells: float, f1faces: float, nq: int, stage_mix: float) -> float:
@@ -53,6 +53,11 @@ def algorithmic_bytes(kid: int, cells: float, f1faces: float, nq: int, stage_mix
     if kid == 4:   # k_f1sum: each F1 face flux read twice (lower cell's slot, upper
         # cell's down slot), slot counts/offsets in, net flux out
         return cells * (nq * d + 4 * i4) + 2 * f1faces * nq * d
+    if kid == 5:   # k_f1cell (k_f1grad + k_f1faces fused): per cell prim, grad_mv, x, dx and
+        # the neighbour lists once, grad_f1 out; per face the upper cell's old grad_f1,
+        # slot ints and the flux out (its prim, grad_mv, x, dx are neighbour reads
+        # already counted once per cell)
+        return cells * ((3 * nq + 2) * d + 6 * i4) + f1faces * (nq * d + nq * d + 6 * i4)
     return 0.0
 
 

@@ -4,7 +4,8 @@ Options: threads per workgroup of the per-cell kernels (64, 128, 256),
 XCD-aware tile order (off/on), two-step hipGraph replay (off/on), persistent
 packed launch (off, 4 or 8 workgroups per CU), the register-capped builds of
 the F1 face kernel and of the update kernel (bit set, 0-3), the F1 flux sums
-inside the update kernel (off/on), the cons2prim solver (Brent, the
+inside the update kernel (off/on), the fused F1 gradient-and-face pass
+(off/on), the cons2prim solver (Brent, the
 reference's algorithm, or Newton) and, when a trained network is given, the
 learned cons2prim start (off, or on with trigger exponents 2 and 4; Newton
 only).
@@ -22,10 +23,10 @@ from __future__ import annotations
 
 import time
 
-SPACE = (("solver", (0, 1)), ("block", (256, 128, 64)), ("remap", (0, 1)), ("graph", (0, 1)),
-         ("persist", (0, 4, 8)), ("occ", (0, 1, 2, 3)), ("f1fuse", (0, 1)))
-DEFAULTS = {"solver": 1, "block": 256, "remap": 0, "graph": 0, "persist": 0, "occ": 0, "f1fuse": 0,
-            "mlp_thr": None}
+SPACE = (("solver", (0, 1)), ("block", (64, 128, 256)), ("remap", (1, 0)), ("graph", (0, 1)),
+         ("persist", (0, 4, 8)), ("occ", (0, 1, 2, 3)), ("f1fuse", (0, 1)), ("f1cell", (0, 1)))
+DEFAULTS = {"solver": 1, "block": 64, "remap": 1, "graph": 0, "persist": 0, "occ": 0, "f1fuse": 0,
+            "f1cell": 0, "mlp_thr": None}
 OPTION_KEYS = tuple(DEFAULTS)
 
 
@@ -33,7 +34,7 @@ def apply(grid, cand: dict, surrogate=None):
     if cand.get("mlp_thr") is None and surrogate is not None:
         grid.load_surrogate(None)
     grid.set_options(solver=cand["solver"], block=cand["block"], remap=cand["remap"], graph=cand["graph"],
-                     persist=cand["persist"], occ=cand["occ"], f1fuse=cand["f1fuse"])
+                     persist=cand["persist"], occ=cand["occ"], f1fuse=cand["f1fuse"], f1cell=cand["f1cell"])
     if cand.get("mlp_thr") is not None:
         grid.load_surrogate(surrogate)
         grid.set_options(mlp_thr=cand["mlp_thr"])

@@ -148,6 +148,18 @@ GH_HD void cell_state(const Dev& d, int j, int i, int dim, double* p, double* c,
   state2flux<ND, NQ>(p, c, f, dim, r, d.tgeom[j]);
 }
 
+// prim and cons of a cell as cell_state forms them; returns the radius its
+// physical flux takes
+template <int ND, int NQ>
+GH_HD double cell_pc(const Dev& d, int j, int i, double* p, double* c) {
+  long long k = CI(d, j, i);
+  load_v<ND, NQ>(d, d.prim, k, p);
+  Geo g = geo_of<ND>(d, j, k);
+  double r = sweep_r<ND>(d, k, g);
+  prim2cons<ND, NQ>(p, c, r, d.tgeom[j], d.P.gamma);
+  return r;
+}
+
 GH_HD bool active_track(const Dev& d, int j) { return d.nd == 1 || (j > d.jL && j < d.jR); }
 
 // ---------------------------------------------------------------------------
@@ -215,8 +227,8 @@ GH_HD void body_ghost_mv(const Dev& d, int j, int kk) {
 // ---------------------------------------------------------------------------
 // Grid::movDir_ComputeLambda + updateKinematics + computeFluxes, MV faces
 // (1d.cpp:370-461, 731-776; 2d.cpp:451-490, 835-888; hllc.cpp:12-39)
-template <int ND, int NQ>
-GH_HD void body_faces(const Dev& d, int j, int i) {
+template <int ND, int NQ, bool SHK>
+GH_HD void body_faces_s(const Dev& d, int j, int i) {
   int nt = d.ntrack[j];
   if (i > nt - 2) return;
   int geom = d.tgeom[j];
@@ -228,10 +240,14 @@ GH_HD void body_faces(const Dev& d, int j, int i) {
     if (i == nt - 2) { d.incf[k + 1] = INT_MAX; d.dncnt[k + 1] = 0; }
   }
   double xf = d.fx[k];
-  double pL[NQ], cL[NQ], fL[NQ], pR[NQ], cR[NQ], fR[NQ];
-  if (i == 0 || i == nt - 2) {
-    cell_state<ND, NQ>(d, j, i, 0, pL, cL, fL);
-    cell_state<ND, NQ>(d, j, i + 1, 0, pR, cR, fR);
+  double pL[NQ], cL[NQ], pR[NQ], cR[NQ];
+  // side fluxes: at the track ends the cells' own states (sweep radius), inside
+  // the reconstructed states at the face
+  bool edge = i == 0 || i == nt - 2;
+  double rL = xf, rR = xf;
+  if (edge) {
+    rL = cell_pc<ND, NQ>(d, j, i, pL, cL);
+    rR = cell_pc<ND, NQ>(d, j, i + 1, pR, cR);
   } else {
     double p0[NQ], p3[NQ];
     load_v<ND, NQ>(d, d.prim, k - 1, p0);
@@ -256,8 +272,8 @@ GH_HD void body_faces(const Dev& d, int j, int i) {
       pL[q] = pL[q] + gLs[q] * (xf - x1);
       pR[q] = pR[q] + gRs[q] * (xf - x2);
     }
-    prim2cons<ND, NQ>(pL, cL, xf, geom, G); state2flux<ND, NQ>(pL, cL, fL, 0, xf, geom);
-    prim2cons<ND, NQ>(pR, cR, xf, geom, G); state2flux<ND, NQ>(pR, cR, fR, 0, xf, geom);
+    prim2cons<ND, NQ>(pL, cL, xf, geom, G);
+    prim2cons<ND, NQ>(pR, cR, xf, geom, G);
   }
   Waves w = lambdas<ND, NQ>(pL, cL, pR, cR, 0, xf, geom, G);
   double v = d.P.vi * w.lS;
@@ -270,18 +286,27 @@ GH_HD void body_faces(const Dev& d, int j, int i) {
   Trig T = trig_at<ND>(d, j);
   double dA = face_area<ND>(0, xf, d.tdy[j], geom, T.s_y, T.s_hdy);
   double fl[NQ];
-  hllc_flux<ND, NQ>(pL, cL, fL, pR, cR, fR, w, v, dA, 0, xf, geom, fl);
+  hllc_flux_lazy<ND, NQ>(pL, cL, pR, cR, w, v, dA, 0, xf, geom, [&](bool right, double* f) {
+    if (right) state2flux<ND, NQ>(pR, cR, f, 0, rR, geom);
+    else state2flux<ND, NQ>(pL, cL, f, 0, rL, geom);
+  }, fl);
   if (fl[0] != fl[0]) GH_ATOMIC_ADD(cnt_at(d, CNT_NANFLUX), 1ull);
   d.fv[k] = v;
   d.flL[k] = w.lL;
   d.flR[k] = w.lR;
   store_v<ND, NQ>(d, d.fflux, k, fl);
-  if (d.shock) {
+  if (SHK) {
     double ps = -1.e300;
     double Sd = shock_sd<ND, NQ>(pL, pR, 0, false, G, &ps);
     d.sd[k] = fmax(d.sd[k], Sd);
     if (ps > d.pspec[k]) d.pspec[k] = ps;
   }
+}
+
+template <int ND, int NQ>
+GH_HD void body_faces(const Dev& d, int j, int i) {
+  if (d.shock) body_faces_s<ND, NQ, true>(d, j, i);
+  else body_faces_s<ND, NQ, false>(d, j, i);
 }
 
 // Cell::update_dt, MV candidates (cell.cpp:47-99)
@@ -456,24 +481,22 @@ GH_HD void body_f1grad(const Dev& d, int j, int i) {
   body_f1grad_c<NQ>(d, j, i, cache, 1);
 }
 
-// Face pass: one F1 face slot s of track j, i.e. the face between its owner
-// cell (j, i) and upper neighbour n of that cell.  The owner's gradient is the
+// Face n of cell (j, i)'s own F1 list (slot s = f1off + n), between the
+// cell and its upper neighbour n, given the owner's primitive state pc, MV
+// gradient mc, new transverse gradient fc and geometry g0 -- from memory
+// (face pass) or registers (fused cell pass).  The owner's gradient is the
 // one just computed; the upper track's is the previous stage's (2d.cpp:901
 // loops j upward, so track j+1 has not been recomputed yet); stale lists of
 // the top ghost track pointing into their own track read the previous values
 // too (their faces only feed ghost cells).
-template <int NQ>
-GH_HD void body_f1face(const Dev& d, int j, int s) {
-  if (j > d.nj - 2) return;
-  int tot = d.f1tot[j];
-  if (s >= tot || s >= d.fcap) return;
-  int i = d.f1own[(long long)j * d.fcap + s];
+template <int NQ, bool SHK>
+GH_HD void f1face_at(const Dev& d, int j, int i, int s, const double* pc, const double* mc, const double* fc,
+                     const Geo& g0) {
   int geom = d.tgeom[j];
   Trig Tj = trig_at<2>(d, j);
   double G = d.P.gamma;
   long long k = CI(d, j, i);
   int tj = d.nbuj[k], lo = d.nbulo[k], n = s - d.f1off[k];
-  Geo g0 = geo_of<2>(d, j, k);
   double x0 = d.cx[k];
   double xL0 = x0 - d.cdx[k] / 2., xR0 = x0 + d.cdx[k] / 2.;
   double jpos = d.ty[j] + d.tdy[j] / 2.;
@@ -484,18 +507,16 @@ GH_HD void body_f1face(const Dev& d, int j, int s) {
   double xm = (fmax(xL0, xLn) + fmin(xR0, xRn)) / 2.;
   double ov = fmax(0, fmin(xR0, xRn) - fmax(xL0, xLn));
   double dA = face_area<2>(1, xm, ov, geom, Tj.s_2, 0.);
-  double pL[NQ], cL[NQ], fL[NQ], pR[NQ], cR[NQ], fR[NQ];
+  double pL[NQ], cL[NQ], pR[NQ], cR[NQ];
+  double rL = xm, rR = xm;
   if (pc0 || d.nbucnt[kn] == 0) {
-    cell_state<2, NQ>(d, j, i, 1, pL, cL, fL);
-    cell_state<2, NQ>(d, tj, lo + n, 1, pR, cR, fR);
+    rL = cell_pc<2, NQ>(d, j, i, pL, cL);
+    rR = cell_pc<2, NQ>(d, tj, lo + n, pR, cR);
   } else {
     Geo gn = geo_of<2>(d, tj, kn);
     double dxmL = xm - g0.cen0, dxmR = xm - gn.cen0;
     double dxfL = jpos - g0.cen1, dxfR = jpos - gn.cen1;
-    double pc[NQ], mc[NQ], fc[NQ], pn[NQ], mn[NQ], fn[NQ];
-    load_v<2, NQ>(d, d.prim, k, pc);
-    load_v<2, NQ>(d, d.gmv, k, mc);
-    load_v<2, NQ>(d, d.gf1n, k, fc);
+    double pn[NQ], mn[NQ], fn[NQ];
     load_v<2, NQ>(d, d.prim, kn, pn);
     load_v<2, NQ>(d, d.gmv, kn, mn);
     load_v<2, NQ>(d, d.gf1, kn, fn);
@@ -507,12 +528,13 @@ GH_HD void body_f1face(const Dev& d, int j, int s) {
     }
     prim2cons<2, NQ>(pL, cL, xm, geom, G);
     prim2cons<2, NQ>(pR, cR, xm, geom, G);
-    state2flux<2, NQ>(pL, cL, fL, 1, xm, geom);
-    state2flux<2, NQ>(pR, cR, fR, 1, xm, geom);
   }
   Waves w = lambdas<2, NQ>(pL, cL, pR, cR, 1, xm, geom, G);
   double fl[NQ];
-  hllc_flux<2, NQ>(pL, cL, fL, pR, cR, fR, w, 0., dA, 1, xm, geom, fl);
+  hllc_flux_lazy<2, NQ>(pL, cL, pR, cR, w, 0., dA, 1, xm, geom, [&](bool right, double* f) {
+    if (right) state2flux<2, NQ>(pR, cR, f, 1, rR, geom);
+    else state2flux<2, NQ>(pL, cL, f, 1, rL, geom);
+  }, fl);
   long long fk = (long long)j * d.fcap + s;
   for (int q = 0; q < NQ; ++q) d.ff1[q * d.fs1 + fk] = fl[q];
   // fresh lists (cells 1..nt-2): also the upper cell's down slot
@@ -524,7 +546,7 @@ GH_HD void body_f1face(const Dev& d, int j, int s) {
   }
   if (fd >= 0)
     for (int q = 0; q < NQ; ++q) d.ff1dn[q * d.fs1 + fd] = fl[q];
-  if (d.shock) {
+  if (SHK) {
     double psf = -1.e300, psr = -1.e300;
     double sdf = shock_sd<2, NQ>(pL, pR, 1, false, G, &psf);
     double sdr = shock_sd<2, NQ>(pR, pL, 1, true, G, &psr);
@@ -532,6 +554,54 @@ GH_HD void body_f1face(const Dev& d, int j, int s) {
     d.ff1sd[2 * d.fs1 + fk] = sdr; d.ff1sd[3 * d.fs1 + fk] = psr;
     if (fd >= 0) { d.ff1sddn[fd] = sdr; d.ff1sddn[d.fs1 + fd] = psr; }
   }
+}
+
+// Face pass: one F1 face slot s of track j.
+template <int NQ, bool SHK>
+GH_HD void body_f1face_s(const Dev& d, int j, int s) {
+  if (j > d.nj - 2) return;
+  int tot = d.f1tot[j];
+  if (s >= tot || s >= d.fcap) return;
+  int i = d.f1own[(long long)j * d.fcap + s];
+  long long k = CI(d, j, i);
+  double pc[NQ], mc[NQ], fc[NQ];
+  load_v<2, NQ>(d, d.prim, k, pc);
+  load_v<2, NQ>(d, d.gmv, k, mc);
+  load_v<2, NQ>(d, d.gf1n, k, fc);
+  f1face_at<NQ, SHK>(d, j, i, s, pc, mc, fc, geo_of<2>(d, j, k));
+}
+template <int NQ>
+GH_HD void body_f1face(const Dev& d, int j, int s) {
+  if (d.shock) body_f1face_s<NQ, true>(d, j, s);
+  else body_f1face_s<NQ, false>(d, j, s);
+}
+
+// Fused cell pass: the cell's transverse gradient (as body_f1grad_c), then
+// the faces of its own F1 list with the owner's state kept in registers.
+// The same arithmetic as the two passes; the gradient is still stored, as the
+// next stage's face pass reads it for the track above.
+template <int NQ>
+GH_HD void body_f1cell_c(const Dev& d, int j, int i, double* cache, int cs) {
+  if (j > d.nj - 2) return;
+  int nt = d.ntrack[j];
+  if (i >= nt) return;
+  long long k = CI(d, j, i);
+  Geo g0 = geo_of<2>(d, j, k);
+  double pc[NQ], mc[NQ], fc[NQ];
+  load_v<2, NQ>(d, d.prim, k, pc);
+  load_v<2, NQ>(d, d.gmv, k, mc);
+  trans_grad<NQ, f1_kc<NQ>()>(d, j, k, pc, mc, g0, fc, cache, cs);
+  store_v<2, NQ>(d, d.gf1n, k, fc);
+  int cnt = d.nbucnt[k], off = d.f1off[k];
+  for (int n = 0; n < cnt && off + n < d.fcap; ++n) {
+    if (d.shock) f1face_at<NQ, true>(d, j, i, off + n, pc, mc, fc, g0);
+    else f1face_at<NQ, false>(d, j, i, off + n, pc, mc, fc, g0);
+  }
+}
+template <int NQ>
+GH_HD void body_f1cell(const Dev& d, int j, int i) {
+  double cache[f1_kc<NQ>() * NQ];
+  body_f1cell_c<NQ>(d, j, i, cache, 1);
 }
 
 // Visit, in the reference's accumulation order (track j' ascending, cell i'
@@ -1096,7 +1166,7 @@ GH_KERNEL2(k_rg_init, body_rg_init)
 GH_KERNEL2P(k_rg_shockpos, body_rg_shockpos)
 GH_KERNEL2(k_advance_time, body_advance_time)
 template <int ND, int NQ> GH_KERNEL2(k_ghost_mv, (body_ghost_mv<ND, NQ>))
-template <int ND, int NQ> GH_KERNEL2P(k_faces, (body_faces<ND, NQ>))
+template <int ND, int NQ, bool SHK> GH_KERNEL2P(k_faces, (body_faces_s<ND, NQ, SHK>))
 // cell pass with the gradient cache in LDS: thread t's entry m at
 // cache[m * blockDim + t] (consecutive lanes, consecutive banks); the launch
 // passes blockDim * f1_kc * NQ doubles of dynamic LDS
@@ -1106,6 +1176,13 @@ __global__ void __launch_bounds__(KB) k_f1grad(Dev d, int ni, int remap) {
   double* cache = f1cache + threadIdx.x;
   GH_TILE_LOOP(if (i < ni && j < d.nj) body_f1grad_c<NQ>(d, j, i, cache, (int)blockDim.x))
 }
+// fused gradient + own-face pass (same LDS cache as k_f1grad)
+template <int NQ>
+__global__ void __launch_bounds__(KB) k_f1cell(Dev d, int ni, int remap) {
+  extern __shared__ double f1cache[];
+  double* cache = f1cache + threadIdx.x;
+  GH_TILE_LOOP(if (i < ni && j < d.nj) body_f1cell_c<NQ>(d, j, i, cache, (int)blockDim.x))
+}
 template <int NQ> GH_KERNEL2P(k_f1sum, body_f1sum<NQ>)
 // face pass over the F1 slots of each track: thread (j, c*bs + t) of a plain
 // (chunk, track) grid walks slots c*bs + t + m*(chunks*bs)
@@ -1113,13 +1190,13 @@ template <int NQ> GH_KERNEL2P(k_f1sum, body_f1sum<NQ>)
 // accordingly, spilling if it must): 1 leaves the choice to the compiler, 4
 // trades spills for latency hiding.  The launch option picks one.
 // (XCD-aware with remap, as the cell passes: the face reads the upper track)
-template <int NQ, int W>
+template <int NQ, int W, bool SHK>
 __global__ void __launch_bounds__(KB, W) k_f1faces(Dev d, int remap) {
   int j, s0;
   tile_of(remap, j, s0);
   int stride = gridDim.x * blockDim.x;
   int tot = min(d.f1tot[j], d.fcap);
-  for (int s = s0; s < tot; s += stride) body_f1face<NQ>(d, j, s);
+  for (int s = s0; s < tot; s += stride) body_f1face_s<NQ, SHK>(d, j, s);
 }
 // W: minimum wavefronts per SIMD (1 = the compiler's choice; 5 caps the
 // registers at 96 and spills the rest), picked by the launch option
@@ -1616,15 +1693,18 @@ struct gammahip_ctx {
   std::string err;
   int ntr;                 // tracks incl. ghost tracks
   std::vector<void*> allocs;
-  // events for kernel timing (0 = update, 1 = faces, 2 = f1faces, 3 = gradf1)
+  // events for kernel timing (0 = update, 1 = faces, 2 = f1faces, 3 = f1grad, 4 = f1sum, 5 = f1cell)
   bool timing = false;
   KTimer kt[16];
   // launch options (gammahip_set_option; the autotuner picks them)
-  int block = 256;         // threads per workgroup of the per-cell kernels (64..256)
-  int remap = 0;           // XCD-aware tile order
+  // defaults: the fastest plain launch measured on the 2D grids (64-thread
+  // workgroups, adjacent tracks kept on one XCD's L2)
+  int block = 64;          // threads per workgroup of the per-cell kernels (64..256)
+  int remap = 1;           // XCD-aware tile order
   int use_graph = 0;       // replay steps from a captured hipGraph
   int persist = 0;         // packed launch: persistent workgroups per CU (0: off)
   int occ = 0;             // bit 0: F1 face kernel built for 4 wavefronts per SIMD; bit 1: update kernel for 5
+  int f1cell = 0;          // fused transverse-gradient + own-face pass (k_f1cell) instead of k_f1grad + k_f1faces
   int ncu = 256;           // compute units of the device
 #ifndef GH_EMU
   hipStream_t stream = nullptr;
@@ -1771,7 +1851,12 @@ struct Pipe {
   static void prep(gammahip_ctx* c) {
     Dev& d = c->d;
     tbeg(c, 1);
-    L2P(c, KSEL((body_faces<ND, NQ>), (k_faces<ND, NQ>)), d.nj, d.nax);
+#ifdef GH_EMU
+    L2P(c, (body_faces<ND, NQ>), d.nj, d.nax);
+#else
+    if (d.shock) L2P(c, (k_faces<ND, NQ, true>), d.nj, d.nax);
+    else L2P(c, (k_faces<ND, NQ, false>), d.nj, d.nax);
+#endif
     tend(c, 1);
     L2P(c, KSEL(body_dt, k_dt), d.nj, d.nax);
     if (ND == 2) {
@@ -1798,31 +1883,49 @@ struct Pipe {
 #else
       hipLaunchKernelGGL(k_f1_scan, dim3(d.nj), dim3(KB), 0, c->stream, d);
 #endif
-      tbeg(c, 3);
+      if (c->f1cell) {
+        tbeg(c, 5);
 #ifdef GH_EMU
-      L2(c, body_f1grad<NQ>, d.nj, d.nax);
+        L2(c, body_f1cell<NQ>, d.nj, d.nax);
 #else
-      {
         int bs = c->block;
         dim3 g = d.packed ? dim3(c->persist * c->ncu) : dim3((d.nax + bs - 1) / bs, d.nj);
         size_t lds = sizeof(double) * bs * f1_kc<NQ>() * NQ;
-        hipLaunchKernelGGL(k_f1grad<NQ>, g, dim3(bs), lds, c->stream, d, d.nax, c->remap);
-      }
+        hipLaunchKernelGGL(k_f1cell<NQ>, g, dim3(bs), lds, c->stream, d, d.nax, c->remap);
 #endif
-      tend(c, 3);
-      tbeg(c, 2);
+        tend(c, 5);
+      } else {
+        tbeg(c, 3);
 #ifdef GH_EMU
-      for (int j = 0; j < d.nj; ++j)
-        for (int sl = 0; sl < d.fcap; ++sl) body_f1face<NQ>(d, j, sl);
+        L2(c, body_f1grad<NQ>, d.nj, d.nax);
 #else
-      {
-        int bs = c->block;
-        dim3 g((d.nax + bs - 1) / bs, d.nj);
-        if (c->occ & 1) hipLaunchKernelGGL((k_f1faces<NQ, 4>), g, dim3(bs), 0, c->stream, d, c->remap);
-        else hipLaunchKernelGGL((k_f1faces<NQ, 1>), g, dim3(bs), 0, c->stream, d, c->remap);
-      }
+        {
+          int bs = c->block;
+          dim3 g = d.packed ? dim3(c->persist * c->ncu) : dim3((d.nax + bs - 1) / bs, d.nj);
+          size_t lds = sizeof(double) * bs * f1_kc<NQ>() * NQ;
+          hipLaunchKernelGGL(k_f1grad<NQ>, g, dim3(bs), lds, c->stream, d, d.nax, c->remap);
+        }
 #endif
-      tend(c, 2);
+        tend(c, 3);
+        tbeg(c, 2);
+#ifdef GH_EMU
+        for (int j = 0; j < d.nj; ++j)
+          for (int sl = 0; sl < d.fcap; ++sl) body_f1face<NQ>(d, j, sl);
+#else
+        {
+          int bs = c->block;
+          dim3 g((d.nax + bs - 1) / bs, d.nj);
+          if (d.shock) {
+            if (c->occ & 1) hipLaunchKernelGGL((k_f1faces<NQ, 4, true>), g, dim3(bs), 0, c->stream, d, c->remap);
+            else hipLaunchKernelGGL((k_f1faces<NQ, 1, true>), g, dim3(bs), 0, c->stream, d, c->remap);
+          } else {
+            if (c->occ & 1) hipLaunchKernelGGL((k_f1faces<NQ, 4, false>), g, dim3(bs), 0, c->stream, d, c->remap);
+            else hipLaunchKernelGGL((k_f1faces<NQ, 1, false>), g, dim3(bs), 0, c->stream, d, c->remap);
+          }
+        }
+#endif
+        tend(c, 2);
+      }
       if (d.shock) L2P(c, KSEL(body_f1_shock, k_f1_shock), d.nj, d.nax);
       if (!d.f1fused) {
         tbeg(c, 4);
@@ -2220,6 +2323,7 @@ int gammahip_set_option(gammahip_ctx* c, int key, long long value) {
       build_tiles(c);
       return post(c);
     case GAMMAHIP_OPT_F1FUSE: c->d.f1fused = value ? 1 : 0; return 0;
+    case GAMMAHIP_OPT_F1CELL: c->f1cell = value ? 1 : 0; return 0;
     case GAMMAHIP_OPT_OCC:
       if (value < 0 || value > 3) return fail(c, "occupancy option must be in 0..3");
       c->occ = (int)value; return 0;

@@ -24,7 +24,7 @@ F_GRADMV0, F_GRADF10 = 300, 400
 FACE_X, FACE_V, FACE_LL, FACE_LR, FACE_FLUX0 = 1000, 1001, 1002, 1003, 1010
 RESET_SHOCK, DETECT_SHOCK, INJECT, INIT_GAMMAS, ADVANCE_TIME = range(5)
 C2P_BRENT, C2P_NEWTON = 0, 1
-OPT_BLOCK, OPT_REMAP, OPT_GRAPH, OPT_MLP_THR, OPT_PERSIST, OPT_OCC, OPT_F1FUSE, OPT_SOLVER = range(8)
+OPT_BLOCK, OPT_REMAP, OPT_GRAPH, OPT_MLP_THR, OPT_PERSIST, OPT_OCC, OPT_F1FUSE, OPT_SOLVER, OPT_F1CELL = range(9)
 
 
 class Config(C.Structure):
@@ -219,15 +219,16 @@ class Grid:
         self._check(self.L.gammahip_sync(self.h))
 
     def set_options(self, block=None, remap=None, graph=None, mlp_thr=None, persist=None, occ=None,
-                    f1fuse=None, solver=None):
+                    f1fuse=None, solver=None, f1cell=None):
         """Launch options (threads per workgroup, XCD-aware tile order, hipGraph
         replay, trigger exponent n of the learned guess: threshold 10^-n,
         persistent packed launch with n workgroups per CU or 0, register-capped
         builds: bit 0 the F1 face kernel, bit 1 the update kernel; F1 flux sums
-        inside the update kernel; cons2prim solver, C2P_BRENT or C2P_NEWTON)."""
+        inside the update kernel; cons2prim solver, C2P_BRENT or C2P_NEWTON;
+        fused transverse-gradient and own-face F1 pass)."""
         for key, val in ((OPT_SOLVER, solver), (OPT_BLOCK, block), (OPT_REMAP, remap), (OPT_GRAPH, graph),
                          (OPT_MLP_THR, mlp_thr), (OPT_PERSIST, persist), (OPT_OCC, occ),
-                         (OPT_F1FUSE, f1fuse)):
+                         (OPT_F1FUSE, f1fuse), (OPT_F1CELL, f1cell)):
             if val is not None:
                 self._check(self.L.gammahip_set_option(self.h, key, int(val)))
         if solver is not None:

@@ -136,14 +136,17 @@ int  gammahip_time(gammahip_ctx* ctx, double* t /* [ntracks] */, double* dt /* [
 int  gammahip_stats_get(gammahip_ctx* ctx, gammahip_stats* st);
 int  gammahip_regrid_counts(gammahip_ctx* ctx, long long* splits, long long* merges);
 /* Average duration (ms) of the dominant kernel launches since the last call,
- * measured with HIP events on the context's stream. */
+ * measured with HIP events on the context's stream: entry 0 the update, 1 the
+ * MV face pass, 2 the F1 face pass, 3 the transverse-gradient pass, 4 the F1
+ * flux sums, 5 the fused gradient-and-face pass. */
 int  gammahip_kernel_times(gammahip_ctx* ctx, double* ms_per_launch /* [16] */,
                            long long* launches /* [16] */);
 int  gammahip_sync(gammahip_ctx* ctx);
 
 /* Launch options (no counterpart in the reference; chosen by the autotuner in
  * gamma_amd/autotune.py): threads per workgroup of the per-cell kernels (64,
- * 128, 256), XCD-aware ordering of the (track, chunk) tiles, replay of
+ * 128, 256; default 64), XCD-aware ordering of the (track, chunk) tiles
+ * (default on), replay of
  * two-step hipGraphs by gammahip_step, and n for the learned guess's trigger:
  * a wavefront runs the network when one of its cells' previous pressures
  * leaves a residual |f| above 10^-n times the magnitude of f's terms
@@ -157,10 +160,12 @@ int  gammahip_sync(gammahip_ctx* ctx);
  * the update kernel instead of in a pass of its own (fewer bytes, a longer
  * dependent-load chain in the update); SOLVER switches cons2prim between
  * Brent (0, the reference's algorithm) and Newton (1; required while the
- * learned guess is loaded). */
+ * learned guess is loaded); F1CELL 1 computes each cell's transverse gradient
+ * and then the fluxes of its own F1 faces in one pass, the owner's state kept
+ * in registers, instead of a gradient pass and a one-thread-per-face pass. */
 enum { GAMMAHIP_OPT_BLOCK = 0, GAMMAHIP_OPT_REMAP = 1, GAMMAHIP_OPT_GRAPH = 2,
        GAMMAHIP_OPT_MLP_THR = 3, GAMMAHIP_OPT_PERSIST = 4, GAMMAHIP_OPT_OCC = 5,
-       GAMMAHIP_OPT_F1FUSE = 6, GAMMAHIP_OPT_SOLVER = 7 };
+       GAMMAHIP_OPT_F1FUSE = 6, GAMMAHIP_OPT_SOLVER = 7, GAMMAHIP_OPT_F1CELL = 8 };
 int  gammahip_set_option(gammahip_ctx* ctx, int key, long long value);
 
 /* ---- learned cons2prim guess (no counterpart in the reference) ----

@@ -274,6 +274,20 @@ def test_emu_parity_f1_sums_in_update(case, emu_lib):
     check_pair(o, g, shock=bool(s.cfg.shock), exact_expected=True)
 
 
+@pytest.mark.parametrize("case", ["riemann_2d", "bm_jet_2d"])
+def test_emu_parity_fused_f1_cell_pass(case, emu_lib):
+    """With the transverse gradient and the cell's own F1 faces in one pass
+    the kernel bodies still reproduce the oracle bit for bit."""
+    mk, n = CASES[case]
+    s = mk()
+    o = oc.OracleGrid(s)
+    o.run(n)
+    g = hip.Grid(s, lib_path=emu_lib)
+    g.set_options(f1cell=1)
+    g.step(n)
+    check_pair(o, g, shock=bool(s.cfg.shock), exact_expected=True)
+
+
 def test_emu_parity_ensemble(emu_lib):
     res = _ensemble_check(emu_lib)
     assert all(bit for _, _, bit in res), res
@@ -312,7 +326,9 @@ def test_gpu_pow_equals_oracle_bitwise():
     dict(block=128, remap=1, graph=1, persist=0),
     dict(block=256, remap=0, graph=0, persist=4, f1fuse=1),
     dict(block=64, remap=1, graph=1, persist=8, occ=3),
-], ids=["remap-graph", "packed", "packed-remap-graph-capped"])
+    dict(block=64, remap=1, graph=1, f1cell=1),
+    dict(block=128, remap=0, graph=0, persist=4, f1cell=1, f1fuse=1),
+], ids=["remap-graph", "packed", "packed-remap-graph-capped", "f1cell", "packed-f1cell-f1fuse"])
 def test_gpu_parity_launch_options(case, opts):
     """Launch options change where work runs, never the arithmetic: the
     trajectory under every option matches the oracle as the default launch does."""
