@@ -56,6 +56,8 @@ enum { RG_NONE = 0, RG_AR2D = 1, RG_BM1D = 2 };
 // device view (passed by value to every kernel)
 struct Dev {
   int nd, nq, nqs, ngst, nj, nax, jL, jR, fcap, shock, sync, solver, guess, maxit;
+  int euler;               // TIMESTEP_ Euler (one update per step) instead of RK3
+  int pc;                  // piecewise-constant reconstruction instead of PLM
   int rg_mode;             // bit set of the tracks' regrid rules (1 << RG_*)
   long long cs;            // nj * nax
   long long fs1;           // nj * fcap
@@ -243,7 +245,7 @@ GH_HD void body_faces_s(const Dev& d, int j, int i) {
   double pL[NQ], cL[NQ], pR[NQ], cR[NQ];
   // side fluxes: at the track ends the cells' own states (sweep radius), inside
   // the reconstructed states at the face
-  bool edge = i == 0 || i == nt - 2;
+  bool edge = d.pc || i == 0 || i == nt - 2;     // piecewise constant: 1d.cpp:378-381
   double rL = xf, rR = xf;
   if (edge) {
     rL = cell_pc<ND, NQ>(d, j, i, pL, cL);
@@ -500,7 +502,7 @@ GH_HD void f1face_at(const Dev& d, int j, int i, int s, const double* pc, const 
   double x0 = d.cx[k];
   double xL0 = x0 - d.cdx[k] / 2., xR0 = x0 + d.cdx[k] / 2.;
   double jpos = d.ty[j] + d.tdy[j] / 2.;
-  bool pc0 = (j == 0 || j == d.nj - 2 || d.nbdcnt[k] == 0);
+  bool pc0 = (d.pc || j == 0 || j == d.nj - 2 || d.nbdcnt[k] == 0);
   long long kn = CI(d, tj, lo + n);
   double xn = d.cx[kn];
   double xLn = xn - d.cdx[kn] / 2., xRn = xn + d.cdx[kn] / 2.;
@@ -1083,6 +1085,9 @@ GH_HD void body_advance_time(const Dev& d, int j, int i) {
 // round-robin over the 8 XCDs by linear id, so linear id L runs on XCD L % 8;
 // L is renumbered so that each XCD walks a contiguous band of tiles (whole
 // tracks in order), keeping adjacent tracks -- the F1 neighbours -- in one L2.
+// On the uniform 2048^2 grid this cuts the transverse-gradient pass's HBM
+// traffic from 1.87 to 0.81 GB per launch; on tracks of uneven length the
+// bands are uneven work, so it is a tuned option, off by default.
 __device__ __forceinline__ void tile_of(int remap, int& j, int& i) {
   int gx = gridDim.x, gy = gridDim.y;
   int L = blockIdx.x + gx * blockIdx.y;
@@ -1697,10 +1702,11 @@ struct gammahip_ctx {
   bool timing = false;
   KTimer kt[16];
   // launch options (gammahip_set_option; the autotuner picks them)
-  // defaults: the fastest plain launch measured on the 2D grids (64-thread
-  // workgroups, adjacent tracks kept on one XCD's L2)
+  // defaults: 64-thread workgroups, the fastest plain launch measured on the
+  // 2D grids; hardware round-robin order (the XCD bands help the uniform grid
+  // by a few percent and cost the uneven ones up to 2x)
   int block = 64;          // threads per workgroup of the per-cell kernels (64..256)
-  int remap = 1;           // XCD-aware tile order
+  int remap = 0;           // XCD-aware tile order
   int use_graph = 0;       // replay steps from a captured hipGraph
   int persist = 0;         // packed launch: persistent workgroups per CU (0: off)
   int occ = 0;             // bit 0: F1 face kernel built for 4 wavefronts per SIMD; bit 1: update kernel for 5
@@ -1895,11 +1901,13 @@ struct Pipe {
 #endif
         tend(c, 5);
       } else {
+        // piecewise constant: the F1 faces take the cells' own states, the
+        // transverse gradient is never read
         tbeg(c, 3);
 #ifdef GH_EMU
-        L2(c, body_f1grad<NQ>, d.nj, d.nax);
+        if (!d.pc) L2(c, body_f1grad<NQ>, d.nj, d.nax);
 #else
-        {
+        if (!d.pc) {
           int bs = c->block;
           dim3 g = d.packed ? dim3(c->persist * c->ncu) : dim3((d.nax + bs - 1) / bs, d.nj);
           size_t lds = sizeof(double) * bs * f1_kc<NQ>() * NQ;
@@ -1999,6 +2007,7 @@ struct Pipe {
   // Grid::evolve (rk3.cpp:12-195)
   static void evolve(gammahip_ctx* c) {
     update(c, 1);
+    if (c->d.euler) return;              // Grid::evolve of euler.cpp:12-19
     ghosts(c); prep(c); update(c, 2);
     ghosts(c); prep(c); update(c, 3);
   }
@@ -2080,6 +2089,8 @@ int gammahip_create(const gammahip_config* cfg, gammahip_ctx** out) {
   if (cfg->ntracks_active < 1) return fail(c, "bad ntracks_active");
   if (cfg->nd == 2 && cfg->geom < 0) return fail(c, "2D needs a single geometry");
   if (cfg->rg_mode < RG_NONE || cfg->rg_mode > RG_BM1D) return fail(c, "unknown rg_mode");
+  if (cfg->timestep != 0 && cfg->timestep != 1) return fail(c, "timestep must be 0 (RK3) or 1 (Euler)");
+  if (cfg->recon != 0 && cfg->recon != 1) return fail(c, "recon must be 0 (piecewise linear) or 1 (piecewise constant)");
   if (cfg->c2p_guess != 0) return fail(c, "c2p_guess must be 0 at creation; gammahip_mlp_load switches the learned guess on");
 #ifndef GH_EMU
   {
@@ -2108,6 +2119,7 @@ int gammahip_create(const gammahip_config* cfg, gammahip_ctx** out) {
   d.fcap = 4 * d.nax;
   d.shock = cfg->shock; d.sync = cfg->sync;
   d.solver = cfg->c2p_solver; d.guess = cfg->c2p_guess;
+  d.euler = cfg->timestep == 1; d.pc = cfg->recon == 1;
   d.mw = nullptr; d.mw2b = nullptr; d.mlp_thr = 1e-3;
   d.maxit = cfg->c2p_maxiter > 0 ? cfg->c2p_maxiter : 1000;
   d.rg_mode = 0;            // set from the tracks' rules by gammahip_load

@@ -28,6 +28,8 @@ RHO, PPP, UU1, UU2 = 0, 1, 2, 3
 TR1, GMN, GMX, GMD, PSN = 4, 6, 7, 8, 9
 
 GEOM_CART, GEOM_SPH = 0, 1
+TS_RK3, TS_EULER = 0, 1
+RECON_PLM, RECON_PC = 0, 1
 KIN_NONE, KIN_FIXED = 0, 1
 BND_NONE, BND_BM = 0, 1
 RG_NONE, RG_AR2D, RG_BM1D = 0, 1, 2
@@ -74,6 +76,8 @@ class RunConfig:
     rg_mode: int = RG_NONE
     rg: tuple = (0., 0., 0., 0., 0., 0.)
     c2p_maxiter: int = 1000
+    timestep: int = TS_RK3        # TIMESTEP_
+    recon: int = RECON_PLM        # SPATIAL_RECONSTRUCTION_
 
     @property
     def nq(self) -> int:          # NUM_Q = NUM_C(4) + NUM_TR(2) + NUM_S

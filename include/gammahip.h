@@ -49,6 +49,9 @@ typedef struct gammahip_config {
   int    c2p_solver;  /* 0 Brent (reference algorithm), 1 safeguarded Newton          */
   int    c2p_guess;   /* 0 previous pressure (reference), 1 learned MLP guess         */
   int    c2p_maxiter; /* 1000 in the reference                                        */
+  int    timestep;    /* TIMESTEP_: 0 RK3 (rk3.cpp:12-195), 1 Euler (euler.cpp:12-19) */
+  int    recon;       /* SPATIAL_RECONSTRUCTION_: 0 piecewise linear, 1 piecewise      */
+                      /* constant (1d.cpp:378-381, 2d.cpp:439-447)                    */
 } gammahip_config;
 
 /* Per-track hook parameters (userKinematics / userBoundaries /
@@ -146,7 +149,7 @@ int  gammahip_sync(gammahip_ctx* ctx);
 /* Launch options (no counterpart in the reference; chosen by the autotuner in
  * gamma_amd/autotune.py): threads per workgroup of the per-cell kernels (64,
  * 128, 256; default 64), XCD-aware ordering of the (track, chunk) tiles
- * (default on), replay of
+ * (default off), replay of
  * two-step hipGraphs by gammahip_step, and n for the learned guess's trigger:
  * a wavefront runs the network when one of its cells' previous pressures
  * leaves a residual |f| above 10^-n times the magnitude of f's terms

@@ -802,7 +802,8 @@ static void recon_mv(og_grid *g, int j, int i) {
   const og_config *c = &g->c;
   Face *f = FACE(g, j, i);
   int nt = g->ntrack[j];
-  if (i == 0 || i == nt - 2) { f->SL = CELL(g, j, i)->S; f->SR = CELL(g, j, i + 1)->S; return; }
+  /* piecewise constant everywhere (1d.cpp:378-381), else at the track ends */
+  if (c->recon == 1 || i == 0 || i == nt - 2) { f->SL = CELL(g, j, i)->S; f->SR = CELL(g, j, i + 1)->S; return; }
   Cell *cL = CELL(g, j, i), *cLL = CELL(g, j, i - 1 > 0 ? i - 1 : 0);
   Cell *cR = CELL(g, j, i + 1), *cRR = CELL(g, j, i + 2 < nt - 1 ? i + 2 : nt - 1);
   double gL[QM], gR[QM], gN[QM];
@@ -923,7 +924,8 @@ static void trans_gradients(og_grid *g, int j, int i) {
 /* Grid::reconstructStates, F1 branch  2d.cpp:491-527 */
 static void recon_f1(og_grid *g, int j, Cell *cL, Cell *cR, Face *f) {
   const og_config *c = &g->c;
-  if (j == 0 || j == g->nj - 2 || cL->neigh[0].cnt == 0 || cR->neigh[1].cnt == 0) {
+  /* piecewise constant (2d.cpp:439-447): the cells' own states */
+  if (c->recon == 1 || j == 0 || j == g->nj - 2 || cL->neigh[0].cnt == 0 || cR->neigh[1].cnt == 0) {
     f->SL = cL->S; f->SR = cR->S; return;
   }
   double xm = f->x[MV], xf = f->x[F1];
@@ -1092,9 +1094,10 @@ static void grid_rk_average(og_grid *g, double w0, double w1) {
   }
 }
 
-/* Grid::evolve (RK3)  rk3.cpp:12-96 (1D), 98-195 (2D) */
+/* Grid::evolve (RK3)  rk3.cpp:12-96 (1D), 98-195 (2D); Euler: euler.cpp:12-19 */
 static int grid_evolve(og_grid *g, double dt) {
   int e;
+  if (g->c.timestep == 1) { grid_update(g, dt); return 0; }
   grid_copy_state0(g);
   grid_update(g, dt);
   grid_update_ghosts(g);

@@ -51,6 +51,8 @@ typedef struct {
   int    bnd_mode;  double bm[6];              /* userBoundaries: BM params */
   int    rg_mode;   double rg[6];              /* checkCellForRegrid params */
   int    c2p_maxiter;                          /* 1000 in the reference */
+  int    timestep;    /* TIMESTEP_: 0 RK3, 1 Euler (euler.cpp:12-19) */
+  int    recon;       /* SPATIAL_RECONSTRUCTION_: 0 piecewise linear, 1 piecewise constant */
   double c2p_tol;     /* Brent interval tolerance, 1e-13 in the reference (rel_sph.cpp:281);
                          other values only measure how much the reference's own
                          tolerance moves its trajectories */

@@ -26,6 +26,8 @@ transposed operand fails at least one check:
 """
 import math
 
+import dataclasses
+
 import numpy as np
 import pytest
 from scipy.integrate import quad
@@ -335,6 +337,61 @@ def test_scheme_converges_to_exact_riemann_solution():
     x = g.active(oc.F_X)[0]
     k = int(np.argmax(tr > 1.5))
     assert abs(x[k] - (0.5 + vs * g.t)) < 3 * g.active(oc.F_DX)[0][k]
+
+
+def _first_order(s):
+    return dataclasses.replace(s, cfg=s.cfg.replace(timestep=setups.TS_EULER, recon=setups.RECON_PC))
+
+
+def test_piecewise_constant_faces_carry_the_cell_states():
+    """Piecewise-constant reconstruction (1d.cpp:378-381) with the Euler step
+    (euler.cpp:12-19): every face flux of a step is the HLLC flux of the two
+    cells' states before that step, at the face's speed."""
+    s = _first_order(setups.shock_tube_1d(200))
+    g = oc.OracleGrid(s)
+    g.run(20)
+    prim = [g.cells(q) for q in range(s.cfg.nq)]
+    nt = int(g.tracks()["ntrack"][0])
+    g.run(1)
+    v = g.faces(oc.I_V)[0]
+    flux = np.array([g.faces(oc.I_FLUX0 + q)[0] for q in range(s.cfg.nq)])
+    checked = 0
+    for i in range(nt - 1):
+        pl = np.array([prim[q][0, i] for q in range(s.cfg.nq)])
+        pr = np.array([prim[q][0, i + 1] for q in range(s.cfg.nq)])
+        _, fl = oc.face_flux(s.cfg, 0, pl, pr, v=v[i])
+        np.testing.assert_array_equal(fl, flux[:, i])
+        checked += 1
+    assert checked == nt - 1
+
+
+def test_first_order_scheme_converges_less_accurately():
+    """Euler + piecewise constant is the first-order scheme: it still converges
+    to the exact solution, with larger errors than PLM + RK3."""
+    def err(n, first):
+        s = setups.shock_tube_1d(n)
+        if first:
+            s = _first_order(s)
+        g = oc.OracleGrid(s)
+        g.run(int(n * 0.5))
+        x, dx, rho = g.active(oc.F_X)[0], g.active(oc.F_DX)[0], g.active(0)[0]
+        rex, _, _ = er.sample(x, g.t, 0.5, 10., 13.33, 0., 1., 1e-6, 0., G53)
+        return float(np.sum(np.abs(rho - rex) * dx) / np.sum(rex * dx))
+    e1, e2 = err(200, True), err(400, True)
+    assert e2 < 0.06 and e2 < 0.8 * e1
+    assert err(400, False) < e2
+
+
+def test_first_order_scheme_conserves():
+    s = _first_order(setups.shock_tube_1d(300))
+    g = oc.OracleGrid(s)
+    D0 = (g.active(100)[0] * g.active(oc.F_DV)[0]).sum()
+    E0 = ((g.active(100)[0] + g.active(101)[0]) * g.active(oc.F_DV)[0]).sum()
+    g.run(120)
+    D1 = (g.active(100)[0] * g.active(oc.F_DV)[0]).sum()
+    E1 = ((g.active(100)[0] + g.active(101)[0]) * g.active(oc.F_DV)[0]).sum()
+    assert D1 == pytest.approx(D0, rel=1e-13)
+    assert E1 == pytest.approx(E0, rel=1e-13)
 
 
 def test_conservation_with_static_ends():

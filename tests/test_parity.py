@@ -20,6 +20,11 @@ import oracle_ctypes as oc
 
 TOL = 1e-10
 
+
+def _variant(setup, **kw):
+    """The setup run with other compile-time switches of the reference."""
+    return dataclasses.replace(setup, cfg=setup.cfg.replace(**kw))
+
 CASES = {
     "shock_tube_cart": (lambda: setups.shock_tube_1d(200), 100),
     "shock_tube_sph": (lambda: setups.shock_tube_1d(200, xmin=0.2, xmax=1.2, xmid=0.7,
@@ -31,6 +36,14 @@ CASES = {
     # 2D spherical blast wave with an angular Lorentz-factor profile: uneven
     # seeded track lengths, per-track BM inflow, shock detection, synchrotron
     "bm_jet_2d": (lambda: setups.bm_jet_2d(12, 0.3, 40, 160), 40),
+    # compile-time variants of the reference: Euler step (euler.cpp), piecewise-
+    # constant reconstruction (1d.cpp:378-381, 2d.cpp:439-447)
+    "shock_tube_euler_pc": (lambda: _variant(setups.shock_tube_1d(200), timestep=setups.TS_EULER,
+                                             recon=setups.RECON_PC), 150),
+    "riemann_2d_pc": (lambda: _variant(setups.riemann_2d(64, 16), recon=setups.RECON_PC), 60),
+    "bm_jet_2d_euler": (lambda: _variant(setups.bm_jet_2d(12, 0.3, 40, 160), timestep=setups.TS_EULER), 60),
+    "blandford_mckee_euler_pc": (lambda: _variant(setups.blandford_mckee_1d(400), timestep=setups.TS_EULER,
+                                                  recon=setups.RECON_PC), 80),
 }
 
 
