@@ -1,4 +1,5 @@
-// ddpow.hpp -- x^y from IEEE-exact operations only, identical on host and device.
+// ddpow.hpp -- x^y, tanh and log10 from IEEE-exact operations only, identical
+// on host and device.
 //
 // The hot path raises to non-integer powers in the synchrotron update, the
 // shock detector, the Blandford-McKee inflow and the BM regrid rule.  A
@@ -87,10 +88,9 @@ GH_HD D2 log_dd(double x) {
   return add(mul_d(ln2, (double)e), lm);
 }
 
-// exp of a double-double, rounded to double
-GH_HD double exp_dd(D2 z) {
-  if (z.hi > 709.8) return HUGE_VAL;
-  if (z.hi < -745.2) return 0.;
+// exp of a double-double as E * 2^k (E a double-double in [0.7, 1.5)),
+// for |z| within the range of doubles
+GH_HD D2 exp_dd2(D2 z, int* k2) {
   D2 ln2 = {0x1.62e42fefa39efp-1, 0x1.abc9e3b39803fp-56};
   double k = rint(z.hi * 0x1.71547652b82fep+0);
   D2 r = add(z, mul_d({-ln2.hi, -ln2.lo}, k));
@@ -103,7 +103,17 @@ GH_HD double exp_dd(D2 z) {
   E = add({1., 0.}, mul(r, E));
   E = add({1., 0.}, mul(r, E));
   for (int n = 0; n < 8; ++n) E = mul(E, E);
-  return ldexp(E.hi, (int)k);
+  *k2 = (int)k;
+  return E;
+}
+
+// exp of a double-double, rounded to double
+GH_HD double exp_dd(D2 z) {
+  if (z.hi > 709.8) return HUGE_VAL;
+  if (z.hi < -745.2) return 0.;
+  int k;
+  D2 E = exp_dd2(z, &k);
+  return ldexp(E.hi, k);
 }
 
 }  // namespace dd
@@ -117,6 +127,31 @@ GH_HD double gh_pow(double x, double y) {
   if (x == 0.) return y > 0. ? 0. : HUGE_VAL;
   if (x == HUGE_VAL) return y > 0. ? HUGE_VAL : 0.;
   return dd::exp_dd(dd::mul_d(dd::log_dd(x), y));
+}
+
+// tanh y = (e^{2|y|} - 1) / (e^{2|y|} + 1) with the sign of y, in double-double
+// (no cancellation for |y| above 1e-8, below which tanh y rounds to y); the
+// shock detector's tanh (radiation_sph.cpp:90, :29)
+GH_HD double gh_tanh(double y) {
+  if (!(y == y)) return y;
+  double a = fabs(y);
+  if (a < 1e-8) return y;
+  if (a > 22.) return y > 0. ? 1. : -1.;     // 1 - tanh(22) < 2^-62
+  int k;
+  dd::D2 E = dd::exp_dd2({2. * a, 0.}, &k);
+  dd::D2 Ek = {ldexp(E.hi, k), ldexp(E.lo, k)};
+  dd::D2 t = dd::div(dd::add(Ek, {-1., 0.}), dd::add(Ek, {1., 0.}));
+  return y > 0. ? t.hi : -t.hi;
+}
+
+// log10 x = log x / log 10 in double-double, for positive normal x (0 gives
+// -inf, negative or NaN gives NaN, +inf gives +inf)
+GH_HD double gh_log10(double x) {
+  if (!(x > 0.)) return x == 0. ? -HUGE_VAL : (x - x) / (x - x);
+  if (x == HUGE_VAL) return x;
+  dd::D2 l = dd::log_dd(x);
+  dd::D2 r = dd::div(l, {0x1.26bb1bbb55516p+1, -0x1.f48ad494ea3e9p-53});   // log 10
+  return r.hi;
 }
 
 }  // namespace gh

@@ -58,6 +58,7 @@ struct Dev {
   int nd, nq, nqs, ngst, nj, nax, jL, jR, fcap, shock, sync, solver, guess, maxit;
   int euler;               // TIMESTEP_ Euler (one update per step) instead of RK3
   int pc;                  // piecewise-constant reconstruction instead of PLM
+  int vpspec;              // VARIABLE_PSPEC_: spectral index from the shock's four-velocity
   int rg_mode;             // bit set of the tracks' regrid rules (1 << RG_*)
   long long cs;            // nj * nax
   long long fs1;           // nj * fcap
@@ -299,7 +300,7 @@ GH_HD void body_faces_s(const Dev& d, int j, int i) {
   store_v<ND, NQ>(d, d.fflux, k, fl);
   if (SHK) {
     double ps = -1.e300;
-    double Sd = shock_sd<ND, NQ>(pL, pR, 0, false, G, &ps);
+    double Sd = shock_sd<ND, NQ>(pL, pR, 0, false, G, &ps, d.vpspec);
     d.sd[k] = fmax(d.sd[k], Sd);
     if (ps > d.pspec[k]) d.pspec[k] = ps;
   }
@@ -550,8 +551,8 @@ GH_HD void f1face_at(const Dev& d, int j, int i, int s, const double* pc, const 
     for (int q = 0; q < NQ; ++q) d.ff1dn[q * d.fs1 + fd] = fl[q];
   if (SHK) {
     double psf = -1.e300, psr = -1.e300;
-    double sdf = shock_sd<2, NQ>(pL, pR, 1, false, G, &psf);
-    double sdr = shock_sd<2, NQ>(pR, pL, 1, true, G, &psr);
+    double sdf = shock_sd<2, NQ>(pL, pR, 1, false, G, &psf, d.vpspec);
+    double sdr = shock_sd<2, NQ>(pR, pL, 1, true, G, &psr, d.vpspec);
     d.ff1sd[0 * d.fs1 + fk] = sdf; d.ff1sd[1 * d.fs1 + fk] = psf;
     d.ff1sd[2 * d.fs1 + fk] = sdr; d.ff1sd[3 * d.fs1 + fk] = psr;
     if (fd >= 0) { d.ff1sddn[fd] = sdr; d.ff1sddn[d.fs1 + fd] = psr; }
@@ -1669,6 +1670,12 @@ __global__ void k_pow_batch(const double* __restrict__ x, const double* __restri
   if (i < n) out[i] = gh_pow(x[i], y[i]);
 }
 
+// tanh / log10 as the kernels evaluate them, for numerics checks
+__global__ void k_fn_batch(int fn, const double* __restrict__ x, double* __restrict__ out, long long n) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = fn == 0 ? gh_tanh(x[i]) : gh_log10(x[i]);
+}
+
 // lane-layout probe of v_mfma_f32_32x32x16_bf16: D = A B for row-major A (32x16), B (16x32)
 __global__ void __launch_bounds__(64) k_mfma_probe(const float* A, const float* B, float* Dm) {
   int l = threadIdx.x;
@@ -2119,7 +2126,7 @@ int gammahip_create(const gammahip_config* cfg, gammahip_ctx** out) {
   d.fcap = 4 * d.nax;
   d.shock = cfg->shock; d.sync = cfg->sync;
   d.solver = cfg->c2p_solver; d.guess = cfg->c2p_guess;
-  d.euler = cfg->timestep == 1; d.pc = cfg->recon == 1;
+  d.euler = cfg->timestep == 1; d.pc = cfg->recon == 1; d.vpspec = cfg->vpspec != 0;
   d.mw = nullptr; d.mw2b = nullptr; d.mlp_thr = 1e-3;
   d.maxit = cfg->c2p_maxiter > 0 ? cfg->c2p_maxiter : 1000;
   d.rg_mode = 0;            // set from the tracks' rules by gammahip_load
@@ -2781,6 +2788,23 @@ int gammahip_pow_batch(const double* x, const double* y, double* out, long long 
   CK(hipMemcpy(dx, x, sizeof(double) * n, hipMemcpyHostToDevice));
   CK(hipMemcpy(dy, y, sizeof(double) * n, hipMemcpyHostToDevice));
   hipLaunchKernelGGL(k_pow_batch, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, 0, dx, dy, dz, n);
+  CK(hipGetLastError());
+  CK(hipMemcpy(out, dz, sizeof(double) * n, hipMemcpyDeviceToHost));
+  return 0;
+#endif
+}
+
+int gammahip_fn_batch(int fn, const double* x, double* out, long long n) {
+  if (n < 1 || (fn != 0 && fn != 1)) return -1;
+#ifdef GH_EMU
+  for (long long i = 0; i < n; ++i) out[i] = fn == 0 ? gh_tanh(x[i]) : gh_log10(x[i]);
+  return 0;
+#else
+  DevBuf B;
+  double *dx = B.get<double>(n), *dz = B.get<double>(n);
+  if (!dx || !dz) return plain_fail("allocation failed");
+  CK(hipMemcpy(dx, x, sizeof(double) * n, hipMemcpyHostToDevice));
+  hipLaunchKernelGGL(k_fn_batch, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, 0, fn, dx, dz, n);
   CK(hipGetLastError());
   CK(hipMemcpy(out, dz, sizeof(double) * n, hipMemcpyDeviceToHost));
   return 0;

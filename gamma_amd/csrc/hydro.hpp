@@ -593,9 +593,16 @@ GH_HD void hllc_flux_lazy(const double* pL, const double* cL, const double* pR, 
 
 // ---- shock detector: radiation_sph.cpp:37-158 (Rezzolla & Zanotti 2013) ----
 // Returns -1 without touching *pspec when no shock is possible (:56-57).
+// spectral index of the electrons accelerated at a shock of four-velocity
+// fvelu: spectral_p, radiation_sph.cpp:18-35 (VARIABLE_PSPEC_ or the constant)
+GH_HD double spectral_p(double fvelu, bool varp) {
+  if (!varp) return kPspec;
+  return 2.11 + 0.11 * gh_tanh((gh_log10(fabs(fvelu)) - .5));
+}
+
 template <int ND, int NQ>
 GH_HD double shock_sd(const double* S1, const double* S2, int dim, bool reverse, double G,
-                      double* pspec) {
+                      double* pspec, bool varp = false) {
   int uux = UU1 + dim, uut = UU2 - dim;
   const int n_evals = 10;
   double chi = 0.5;
@@ -623,7 +630,7 @@ GH_HD double shock_sd(const double* S1, const double* S2, int dim, bool reverse,
     double dI = sqrt(h * h + A1 * A1 * (1. - cs * cs)) / ((h * h + A1 * A1) * rho * cs) * dp;
     I += dI;
   }
-  double vSR = tanh(I);
+  double vSR = gh_tanh(I);
   double rho22 = rho2 * rho2, lfac22 = lfac2 * lfac2, vx22 = vx2 * vx2;
   double g = G, gm1 = (g - 1.), gm12 = gm1 * gm1;
   double Da = 4. * g * p1 * ((g - 1) * p2 + p1) / (gm12 * (p1 - p2) * (p1 - p2));
@@ -641,7 +648,10 @@ GH_HD double shock_sd(const double* S1, const double* S2, int dim, bool reverse,
   double v2Sb = (Vs - vx2) * (h2 * rho2 * lfac22 * (1. - vx22) + p1 - p2);
   double v2S = v2Sa / v2Sb;
   double vlim = vSR + chi * (v2S - vSR);
-  *pspec = kPspec;
+  // shock Lorentz factor setup (:146-155): the shock runs into gas at rest
+  double betau = Vs;
+  double lfacu = 1. / sqrt(1. - betau * betau);
+  *pspec = spectral_p(lfacu * betau, varp);
   return v12 - vlim;
 }
 

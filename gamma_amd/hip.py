@@ -37,7 +37,7 @@ class Config(C.Structure):
         ("Nalpha", C.c_double), ("Nmp", C.c_double), ("Nme", C.c_double),
         ("rg_mode", C.c_int), ("rg", C.c_double * 6),
         ("c2p_solver", C.c_int), ("c2p_guess", C.c_int), ("c2p_maxiter", C.c_int),
-        ("timestep", C.c_int), ("recon", C.c_int),
+        ("timestep", C.c_int), ("recon", C.c_int), ("vpspec", C.c_int),
     ]
 
 
@@ -88,6 +88,7 @@ def load_library(path: str = LIB_PATH):
         "gammahip_c2p_batch": (I, [P, P, P, C.c_longlong, D, I, I, P, P, P, P]),
         "gammahip_mlp_load": (I, [P, P]),
         "gammahip_pow_batch": (I, [P, P, P, C.c_longlong]),
+        "gammahip_fn_batch": (I, [I, P, P, C.c_longlong]),
         "gammahip_state_size": (I, [P, C.POINTER(C.c_ulonglong)]),
         "gammahip_state_save": (I, [P, P, C.c_ulonglong]),
         "gammahip_state_restore": (I, [P, P, C.c_ulonglong]),
@@ -122,7 +123,7 @@ def make_config(rc: S.RunConfig, ntracks_active: int, solver=C2P_BRENT, guess=0)
     for k in range(6):
         c.rg[k] = rc.rg[k]
     c.c2p_solver, c.c2p_guess, c.c2p_maxiter = solver, guess, rc.c2p_maxiter
-    c.timestep, c.recon = rc.timestep, rc.recon
+    c.timestep, c.recon, c.vpspec = rc.timestep, rc.recon, rc.vpspec
     return c
 
 

@@ -78,6 +78,7 @@ class RunConfig:
     c2p_maxiter: int = 1000
     timestep: int = TS_RK3        # TIMESTEP_
     recon: int = RECON_PLM        # SPATIAL_RECONSTRUCTION_
+    vpspec: int = 0               # VARIABLE_PSPEC_
 
     @property
     def nq(self) -> int:          # NUM_Q = NUM_C(4) + NUM_TR(2) + NUM_S

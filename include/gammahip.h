@@ -52,6 +52,8 @@ typedef struct gammahip_config {
   int    timestep;    /* TIMESTEP_: 0 RK3 (rk3.cpp:12-195), 1 Euler (euler.cpp:12-19) */
   int    recon;       /* SPATIAL_RECONSTRUCTION_: 0 piecewise linear, 1 piecewise      */
                       /* constant (1d.cpp:378-381, 2d.cpp:439-447)                    */
+  int    vpspec;      /* VARIABLE_PSPEC_: spectral index from the shock's four-velocity */
+                      /* (radiation_sph.cpp:18-35, 146-155); 0: the constant p_       */
 } gammahip_config;
 
 /* Per-track hook parameters (userKinematics / userBoundaries /
@@ -197,6 +199,10 @@ int  gammahip_mfma_probe(const float* A /* 32x16 */, const float* B /* 16x32 */,
  * IEEE-exact operations, identical bits on host and device; no counterpart
  * in the reference, which calls the C library's pow), for numerics checks. */
 int  gammahip_pow_batch(const double* x, const double* y, double* out, long long n);
+/* tanh (fn 0) and log10 (fn 1) as the kernels evaluate them (the shock
+ * detector's, radiation_sph.cpp:18-35, 90): double-double, identical on host
+ * and device.  No counterpart in the reference; for numerics checks. */
+int  gammahip_fn_batch(int fn, const double* x, double* out, long long n);
 /* Seed the context's Newton cons2prim with the network (params = null
  * switches it off).  The previous pressure stays the judge: each solve starts
  * from whichever start has the smaller residual; stats.mlp_used counts the

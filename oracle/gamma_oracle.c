@@ -508,7 +508,7 @@ static double compute_sd(const og_config *c, const FS *S1, const FS *S2, int dim
     double dI = sqrt(h * h + A1 * A1 * (1. - cs * cs)) / ((h * h + A1 * A1) * rho * cs) * dp;
     I += dI;
   }
-  double vSR = tanh(I);
+  double vSR = og_tanh(I);
   double rho22 = rho2 * rho2, lfac22 = lfac2 * lfac2, vx22 = vx2 * vx2;
   double g = G, gm1 = (g - 1.), gm12 = gm1 * gm1;
   double Da = 4. * g * p1 * ((g - 1) * p2 + p1) / (gm12 * (p1 - p2) * (p1 - p2));
@@ -527,7 +527,12 @@ static double compute_sd(const og_config *c, const FS *S1, const FS *S2, int dim
   double v2S = v2Sa / v2Sb;
   double vlim = vSR + chi * (v2S - vSR);
   double Sd = v12 - vlim;
-  *pspec = PSPEC_CONST;     /* spectral_p :18-35 with VARIABLE_PSPEC_ disabled */
+  /* spectral_p :18-35 of the shock's four-velocity, the shock running into
+   * gas at rest (:146-155); the constant p_ without VARIABLE_PSPEC_ */
+  double betau = Vs;
+  double lfacu = 1. / sqrt(1. - betau * betau);
+  double fvelu = lfacu * betau;
+  *pspec = c->vpspec ? 2.11 + 0.11 * og_tanh((og_log10(fabs(fvelu)) - .5)) : PSPEC_CONST;
   return (Sd);
 }
 
@@ -1546,4 +1551,7 @@ void og_inject(const og_config *c, double *prim, int shocked, double pspec) {
 /* x^y as every kernel of the oracle evaluates it (og_pow.h), for tests */
 void og_pow_batch(const double *x, const double *y, double *out, long n) {
   for (long i = 0; i < n; ++i) out[i] = og_pow(x[i], y[i]);
+}
+void og_fn_batch(int fn, const double *x, double *out, long n) {
+  for (long i = 0; i < n; ++i) out[i] = fn == 0 ? og_tanh(x[i]) : og_log10(x[i]);
 }

@@ -53,6 +53,7 @@ typedef struct {
   int    c2p_maxiter;                          /* 1000 in the reference */
   int    timestep;    /* TIMESTEP_: 0 RK3, 1 Euler (euler.cpp:12-19) */
   int    recon;       /* SPATIAL_RECONSTRUCTION_: 0 piecewise linear, 1 piecewise constant */
+  int    vpspec;      /* VARIABLE_PSPEC_ (radiation_sph.cpp:18-35) */
   double c2p_tol;     /* Brent interval tolerance, 1e-13 in the reference (rel_sph.cpp:281);
                          other values only measure how much the reference's own
                          tolerance moves its trajectories */
@@ -76,6 +77,7 @@ void     og_destroy(og_grid *g);
 int      og_set_track_cells(og_grid *g, const int *ncells);
 int      og_set_track_bm(og_grid *g, const double *bm);
 void     og_pow_batch(const double *x, const double *y, double *out, long n);
+void     og_fn_batch(int fn, const double *x, double *out, long n);   /* 0 tanh, 1 log10 */
 int      og_load(og_grid *g, const double *x, const double *dx,
                  const double *y, const double *dy, const double *prim);
 /* prepForRun + radiation_initGammas + the first updateGhosts of Simu::run */

@@ -68,9 +68,7 @@ static inline og_d2 og_log_dd(double x) {
   og_d2 ln2 = og_mk(0x1.62e42fefa39efp-1, 0x1.abc9e3b39803fp-56);
   return og_dd_add(og_dd_mul_d(ln2, (double)e), lm);
 }
-static inline double og_exp_dd(og_d2 z) {
-  if (z.hi > 709.8) return HUGE_VAL;
-  if (z.hi < -745.2) return 0.;
+static inline og_d2 og_exp_dd2(og_d2 z, int *k2) {
   og_d2 ln2 = og_mk(0x1.62e42fefa39efp-1, 0x1.abc9e3b39803fp-56);
   double k = rint(z.hi * 0x1.71547652b82fep+0);
   og_d2 r = og_dd_add(z, og_dd_mul_d(og_mk(-ln2.hi, -ln2.lo), k));
@@ -83,7 +81,15 @@ static inline double og_exp_dd(og_d2 z) {
   E = og_dd_add(og_mk(1., 0.), og_dd_mul(r, E));
   E = og_dd_add(og_mk(1., 0.), og_dd_mul(r, E));
   for (int n = 0; n < 8; ++n) E = og_dd_mul(E, E);
-  return ldexp(E.hi, (int)k);
+  *k2 = (int)k;
+  return E;
+}
+static inline double og_exp_dd(og_d2 z) {
+  if (z.hi > 709.8) return HUGE_VAL;
+  if (z.hi < -745.2) return 0.;
+  int k;
+  og_d2 E = og_exp_dd2(z, &k);
+  return ldexp(E.hi, k);
 }
 static inline double og_pow(double x, double y) {
   if (y == 0.) return 1.;
@@ -91,5 +97,25 @@ static inline double og_pow(double x, double y) {
   if (x == 0.) return y > 0. ? 0. : HUGE_VAL;
   if (x == HUGE_VAL) return y > 0. ? HUGE_VAL : 0.;
   return og_exp_dd(og_dd_mul_d(og_log_dd(x), y));
+}
+/* tanh and log10 the same way (the reference calls the C library's; these
+ * agree with it to one ulp and are what the kernels evaluate) */
+static inline double og_tanh(double y) {
+  if (!(y == y)) return y;
+  double a = fabs(y);
+  if (a < 1e-8) return y;
+  if (a > 22.) return y > 0. ? 1. : -1.;
+  int k;
+  og_d2 E = og_exp_dd2(og_mk(2. * a, 0.), &k);
+  og_d2 Ek = og_mk(ldexp(E.hi, k), ldexp(E.lo, k));
+  og_d2 t = og_dd_div(og_dd_add(Ek, og_mk(-1., 0.)), og_dd_add(Ek, og_mk(1., 0.)));
+  return y > 0. ? t.hi : -t.hi;
+}
+static inline double og_log10(double x) {
+  if (!(x > 0.)) return x == 0. ? -HUGE_VAL : (x - x) / (x - x);
+  if (x == HUGE_VAL) return x;
+  og_d2 l = og_log_dd(x);
+  og_d2 r = og_dd_div(l, og_mk(0x1.26bb1bbb55516p+1, -0x1.f48ad494ea3e9p-53));
+  return r.hi;
 }
 #endif

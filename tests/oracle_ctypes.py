@@ -28,7 +28,7 @@ class OgConfig(C.Structure):
         ("kin_mode", C.c_int), ("v_in", C.c_double), ("v_out", C.c_double),
         ("bnd_mode", C.c_int), ("bm", C.c_double * 6),
         ("rg_mode", C.c_int), ("rg", C.c_double * 6),
-        ("c2p_maxiter", C.c_int), ("timestep", C.c_int), ("recon", C.c_int),
+        ("c2p_maxiter", C.c_int), ("timestep", C.c_int), ("recon", C.c_int), ("vpspec", C.c_int),
         ("c2p_tol", C.c_double), ("c2p_form", C.c_int),
     ]
 
@@ -47,7 +47,7 @@ def make_config(rc, c2p_tol=1e-13, c2p_form=0) -> OgConfig:
         c.rg[k] = rc.rg[k]
     c.rg_mode = rc.rg_mode
     c.c2p_maxiter = rc.c2p_maxiter
-    c.timestep, c.recon = rc.timestep, rc.recon
+    c.timestep, c.recon, c.vpspec = rc.timestep, rc.recon, rc.vpspec
     c.c2p_tol = c2p_tol
     c.c2p_form = c2p_form
     return c
@@ -68,7 +68,7 @@ def lib():
             "og_create": (P, [cfgp]), "og_destroy": (V, [P]),
             "og_load": (I, [P, P, P, P, P, P]), "og_init": (I, [P, D]),
             "og_set_track_cells": (I, [P, P]), "og_set_track_bm": (I, [P, P]),
-            "og_pow_batch": (V, [P, P, P, C.c_long]),
+            "og_pow_batch": (V, [P, P, P, C.c_long]), "og_fn_batch": (V, [I, P, P, C.c_long]),
             "og_run": (I, [P, I]), "og_ntracks": (I, [P]), "og_nax": (I, [P]),
             "og_track_info": (V, [P, P, P, P, P]), "og_time": (D, [P]),
             "og_iter": (C.c_long, [P]), "og_last_dt": (D, [P]),

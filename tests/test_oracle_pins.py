@@ -487,3 +487,73 @@ def test_pow_kernel_equals_oracle_bitwise(emu_lib):
     p = lambda a: a.ctypes.data_as(C.c_void_p)
     assert L.gammahip_pow_batch(p(x), p(y), p(out), len(x)) == 0
     assert np.array_equal(out, og_pow(x, y))
+
+
+def _fn(fn, x):
+    import ctypes as C
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    out = np.zeros_like(x)
+    oc.lib().og_fn_batch(fn, x.ctypes.data_as(C.c_void_p), out.ctypes.data_as(C.c_void_p), x.size)
+    return out
+
+
+def test_tanh_log10_against_libm_and_correct_rounding():
+    """The detector's tanh and log10 (double-double) are the doubles nearest
+    the 60-digit values; the C library's tanh is not (up to 2 ulp off, its
+    documented bound, on 17 % of these arguments), its log10 nearly always is."""
+    from decimal import Decimal, getcontext
+    getcontext().prec = 60
+    rng = np.random.default_rng(8)
+    y = np.concatenate([rng.uniform(-25, 25, 3000), np.exp(rng.uniform(np.log(1e-9), np.log(3.), 3000)),
+                        [0., -0., 1e-8, 2e-8, 0.5, 21.9, 22.1, -30.]])
+    t = _fn(0, y)
+    libm = np.array([math.tanh(v) for v in y])
+    assert np.all(np.abs(t - libm) <= 2 * np.spacing(np.abs(libm)))
+    assert np.mean(t == libm) > 0.8
+    for v, got in zip(y[:600], t[:600]):
+        d = Decimal(float(v))
+        e2 = (2 * d).exp()
+        exact = (e2 - 1) / (e2 + 1)
+        assert abs(Decimal(float(got)) - exact) <= Decimal(math.ulp(abs(float(exact)))) * Decimal("0.5000001"), v
+    x = np.concatenate([np.exp(rng.uniform(np.log(1e-300), np.log(1e300), 3000)), [1., 10., 1e-5, 3.1622776601683795]])
+    l = _fn(1, x)
+    libm = np.array([math.log10(v) for v in x])
+    assert np.all(np.abs(l - libm) <= np.spacing(np.abs(libm)) + 0.)
+    assert np.mean(l == libm) > 0.99
+    for v, got in zip(x[:600], l[:600]):
+        exact = Decimal(float(v)).log10()
+        assert abs(Decimal(float(got)) - exact) <= Decimal(math.ulp(abs(float(exact)) or 1e-300)) * Decimal("0.5000001"), v
+    assert _fn(1, np.array([10., 1000.]))[0] == 1. and _fn(1, np.array([1000.]))[0] == 3.
+
+
+def test_tanh_log10_kernel_equals_oracle_bitwise(emu_lib):
+    import ctypes as C
+    from gamma_amd import hip
+    L = hip.load_library(emu_lib)
+    rng = np.random.default_rng(9)
+    x = np.concatenate([rng.uniform(-30, 30, 5000), np.exp(rng.uniform(-700, 700, 5000))])
+    for fn in (0, 1):
+        out = np.zeros_like(x)
+        assert L.gammahip_fn_batch(fn, x.ctypes.data_as(C.c_void_p), out.ctypes.data_as(C.c_void_p), x.size) == 0
+        ref = _fn(fn, x)
+        assert np.array_equal(out, ref, equal_nan=True)
+
+
+def test_variable_spectral_index_of_an_exact_shock():
+    """VARIABLE_PSPEC_ (radiation_sph.cpp:18-35, 146-155): p = 2.11 + 0.11
+    tanh(log10(u_s) - 1/2) of the shock four-velocity u_s = W_s V_s, the shock
+    taken as running into gas at rest.  On the exact Marti-Mueller problem-1
+    shock, whose upstream gas is at rest, V_s is the exact shock speed."""
+    rc = setups.RunConfig(nd=1, gamma=G53, vpspec=1)
+    ps, vs, rL, rR, Vs, _ = er.star(10., 13.33, 0., 1., 1e-6, 0., G53)
+    shocked = np.zeros(rc.nq); shocked[0], shocked[1], shocked[2] = rR, ps, vs / math.sqrt(1 - vs * vs)
+    ahead = np.zeros(rc.nq); ahead[0], ahead[1] = 1., 1e-6
+    sd, pspec = oc.compute_sd(rc, shocked, ahead)
+    us = Vs / math.sqrt(1 - Vs * Vs)
+    want = 2.11 + 0.11 * math.tanh(math.log10(us) - 0.5)
+    assert sd > 0.01
+    assert pspec == pytest.approx(want, abs=2e-4)
+    assert 2.0 < pspec < 2.22
+    # the constant without the switch
+    _, p0 = oc.compute_sd(setups.RunConfig(nd=1, gamma=G53), shocked, ahead)
+    assert p0 == 2.22

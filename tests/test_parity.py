@@ -44,6 +44,9 @@ CASES = {
     "bm_jet_2d_euler": (lambda: _variant(setups.bm_jet_2d(12, 0.3, 40, 160), timestep=setups.TS_EULER), 60),
     "blandford_mckee_euler_pc": (lambda: _variant(setups.blandford_mckee_1d(400), timestep=setups.TS_EULER,
                                                   recon=setups.RECON_PC), 80),
+    # VARIABLE_PSPEC_ (radiation_sph.cpp:18-35): injected spectral index from the shock
+    "blandford_mckee_vpspec": (lambda: _variant(setups.blandford_mckee_1d(400), vpspec=1), 60),
+    "bm_jet_2d_vpspec": (lambda: _variant(setups.bm_jet_2d(12, 0.3, 40, 160), vpspec=1), 40),
 }
 
 
