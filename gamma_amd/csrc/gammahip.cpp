@@ -147,7 +147,7 @@ GH_HD void cell_state(const Dev& d, int j, int i, int dim, double* p, double* c,
   load_v<ND, NQ>(d, d.prim, k, p);
   Geo g = geo_of<ND>(d, j, k);
   double r = sweep_r<ND>(d, k, g);
-  prim2cons<ND, NQ>(p, c, r, d.tgeom[j], d.P.gamma);
+  prim2cons<ND, NQ>(p, c, r, d.tgeom[j], d.P.gamma, d.P.synge);
   state2flux<ND, NQ>(p, c, f, dim, r, d.tgeom[j]);
 }
 
@@ -159,7 +159,7 @@ GH_HD double cell_pc(const Dev& d, int j, int i, double* p, double* c) {
   load_v<ND, NQ>(d, d.prim, k, p);
   Geo g = geo_of<ND>(d, j, k);
   double r = sweep_r<ND>(d, k, g);
-  prim2cons<ND, NQ>(p, c, r, d.tgeom[j], d.P.gamma);
+  prim2cons<ND, NQ>(p, c, r, d.tgeom[j], d.P.gamma, d.P.synge);
   return r;
 }
 
@@ -275,8 +275,8 @@ GH_HD void body_faces_s(const Dev& d, int j, int i) {
       pL[q] = pL[q] + gLs[q] * (xf - x1);
       pR[q] = pR[q] + gRs[q] * (xf - x2);
     }
-    prim2cons<ND, NQ>(pL, cL, xf, geom, G);
-    prim2cons<ND, NQ>(pR, cR, xf, geom, G);
+    prim2cons<ND, NQ>(pL, cL, xf, geom, G, d.P.synge);
+    prim2cons<ND, NQ>(pR, cR, xf, geom, G, d.P.synge);
   }
   Waves w = lambdas<ND, NQ>(pL, cL, pR, cR, 0, xf, geom, G);
   double v = d.P.vi * w.lS;
@@ -529,8 +529,8 @@ GH_HD void f1face_at(const Dev& d, int j, int i, int s, const double* pc, const 
       pL[q] = qL + fc[q] * dxfL;
       pR[q] = qR + fn[q] * dxfR;
     }
-    prim2cons<2, NQ>(pL, cL, xm, geom, G);
-    prim2cons<2, NQ>(pR, cR, xm, geom, G);
+    prim2cons<2, NQ>(pL, cL, xm, geom, G, d.P.synge);
+    prim2cons<2, NQ>(pR, cR, xm, geom, G, d.P.synge);
   }
   Waves w = lambdas<2, NQ>(pL, cL, pR, cR, 1, xm, geom, G);
   double fl[NQ];
@@ -753,7 +753,7 @@ GH_HD void body_update_g(const Dev& d, int j, int i, int stage, bool valid, cons
     xnR = d.fx[k] + dt * d.fv[k];
     load_v<ND, NQ>(d, d.prim, k, p);
     Geo g = geo_of<ND>(d, j, k);
-    prim2cons<ND, NQ>(p, U, sweep_r<ND>(d, k, g), geom, G);
+    prim2cons<ND, NQ>(p, U, sweep_r<ND>(d, k, g), geom, G, d.P.synge);
     if (stage == 1) {
       store_v<ND, NQ>(d, d.U0, k, U);
       d.dV0[k] = g.dV;
@@ -771,7 +771,7 @@ GH_HD void body_update_g(const Dev& d, int j, int i, int stage, bool valid, cons
       else for (int q = 0; q < NQ; ++q) f1[q] = d.f1net[q * d.cs + k];
       for (int q = 0; q < NQ; ++q) U[q] += f1[q] * dt;
     }
-    sources<ND, NQ>(p, U, d.cx[k], d.cdx[k], trig_at<ND>(d, j), g.dV, dt, geom, G);
+    sources<ND, NQ>(p, U, d.cx[k], d.cdx[k], trig_at<ND>(d, j), g.dV, dt, geom, G, d.P.synge);
     if (NQ == 10 && d.sync) radiative(p, U, g.dV, dt, d.P);
     cxn = (xnR + xnL) / 2.;
     cdxn = (xnR - xnL);
@@ -784,7 +784,7 @@ GH_HD void body_update_g(const Dev& d, int j, int i, int stage, bool valid, cons
     double D = 1., S = 0., tau = 1.;
     if (cell) c2p_inputs<ND, NQ>(U, r1, geom, &D, &S, &tau);
     double gs = guess(cell, D, S, tau, p[PPP], geom);
-    if (cell) cons2prim<ND, NQ>(p, U, r1, geom, G, SV, gs, d.maxit, &st);
+    if (cell) cons2prim<ND, NQ>(p, U, r1, geom, G, SV, gs, d.maxit, &st, d.P.synge);
   }
   int ncalls = 1;
   if (stage > 1) {
@@ -803,7 +803,7 @@ GH_HD void body_update_g(const Dev& d, int j, int i, int stage, bool valid, cons
     double D = 1., S = 0., tau = 1.;
     if (cell) c2p_inputs<ND, NQ>(U, cxn, geom, &D, &S, &tau);
     double gs = guess(cell, D, S, tau, p[PPP], geom);
-    if (cell) cons2prim<ND, NQ>(p, U, cxn, geom, G, SV, gs, d.maxit, &st);
+    if (cell) cons2prim<ND, NQ>(p, U, cxn, geom, G, SV, gs, d.maxit, &st, d.P.synge);
     ncalls = 2;
   }
   if (!cell) return;
@@ -1009,8 +1009,8 @@ GH_HD bool merge_fill(const Dev& d, int j, int i) {
   load_v<ND, NQ>(d, d.prim, b + i, pc);
   load_v<ND, NQ>(d, d.prim, b + v, pv);
   Geo gc = geo_of<ND>(d, j, b + i), gv = geo_of<ND>(d, j, b + v);
-  prim2cons<ND, NQ>(pc, cc, sweep_r<ND>(d, b + i, gc), geom, G);
-  prim2cons<ND, NQ>(pv, cv, sweep_r<ND>(d, b + v, gv), geom, G);
+  prim2cons<ND, NQ>(pc, cc, sweep_r<ND>(d, b + i, gc), geom, G, d.P.synge);
+  prim2cons<ND, NQ>(pv, cv, sweep_r<ND>(d, b + v, gv), geom, G, d.P.synge);
   double dVl = gc.dV, dVv = gv.dV;
   for (int q = 0; q < NQ; ++q) cc[q] = (cc[q] * dVl + cv[q] * dVv) / (dVl + dVv);
   double xc = d.cx[b + i], dxc = d.cdx[b + i], xv = d.cx[b + v], dxv = d.cdx[b + v];
@@ -1020,7 +1020,7 @@ GH_HD bool merge_fill(const Dev& d, int j, int i) {
   d.cdx[b + i] = dxnew;
   Geo g2 = cell_geo<ND>(xnew, dxnew, d.ty[j], d.tdy[j], geom, trig_at<ND>(d, j));
   C2PStat st = {0, 0, 0, 0};
-  cons2prim<ND, NQ>(pc, cc, ND == 1 ? g2.cen0 : xnew, geom, G, d.solver, -1., d.maxit, &st);
+  cons2prim<ND, NQ>(pc, cc, ND == 1 ? g2.cen0 : xnew, geom, G, d.solver, -1., d.maxit, &st, d.P.synge);
   store_v<ND, NQ>(d, d.prim, b + i, pc);
   return left;
 }
@@ -2098,6 +2098,9 @@ int gammahip_create(const gammahip_config* cfg, gammahip_ctx** out) {
   if (cfg->rg_mode < RG_NONE || cfg->rg_mode > RG_BM1D) return fail(c, "unknown rg_mode");
   if (cfg->timestep != 0 && cfg->timestep != 1) return fail(c, "timestep must be 0 (RK3) or 1 (Euler)");
   if (cfg->recon != 0 && cfg->recon != 1) return fail(c, "recon must be 0 (piecewise linear) or 1 (piecewise constant)");
+  if (cfg->eos != 0 && cfg->eos != 1) return fail(c, "eos must be 0 (ideal) or 1 (Synge)");
+  if (cfg->eos == 1 && cfg->c2p_solver != 0)
+    return fail(c, "the Synge equation of state runs with the Brent cons2prim (the Newton residual assumes a constant index)");
   if (cfg->c2p_guess != 0) return fail(c, "c2p_guess must be 0 at creation; gammahip_mlp_load switches the learned guess on");
 #ifndef GH_EMU
   {
@@ -2132,7 +2135,7 @@ int gammahip_create(const gammahip_config* cfg, gammahip_ctx** out) {
   d.rg_mode = 0;            // set from the tracks' rules by gammahip_load
   for (int k = 0; k < 6; ++k) d.rg[k] = cfg->rg[k];
   d.P = Phys{cfg->gamma, cfg->cfl, cfg->vi, cfg->shock_thr, cfg->gamma_max_init,
-             cfg->Nalpha, cfg->Nmp, cfg->Nme};
+             cfg->Nalpha, cfg->Nmp, cfg->Nme, cfg->eos == 1};
   d.cs = (long long)d.nj * d.nax;
   d.fs1 = (long long)d.nj * d.fcap;
   c->ntr = d.nj;
@@ -2355,6 +2358,7 @@ int gammahip_set_option(gammahip_ctx* c, int key, long long value) {
     case GAMMAHIP_OPT_SOLVER:
       if (value != gh::C2P_BRENT && value != gh::C2P_NEWTON) return fail(c, "solver must be 0 (Brent) or 1 (Newton)");
       if (value == gh::C2P_BRENT && c->d.mw) return fail(c, "the learned guess is loaded: it seeds the Newton solver");
+      if (value == gh::C2P_NEWTON && c->d.P.synge) return fail(c, "the Synge equation of state runs with the Brent cons2prim");
       c->d.solver = (int)value; return 0;
     case GAMMAHIP_OPT_REMAP: c->remap = value ? 1 : 0; return 0;
     case GAMMAHIP_OPT_GRAPH: c->use_graph = value ? 1 : 0; return 0;
@@ -2558,11 +2562,11 @@ int gammahip_download(gammahip_ctx* c, int field, double* out) {
         for (int qq = 0; qq < nq; ++qq) pv[qq] = (nd == 1 && qq == 3) ? 0. : p[(size_t)sidx(qq) * cs + k];
         if (nd == 1) {
           Geo g = cell_geo<1>(x[k], dx[k], ty[j], tdy[j], tg[j], Trig{});
-          if (nq == 6) prim2cons<1, 6>(pv, cv, g.cen0, tg[j], d.P.gamma);
-          else prim2cons<1, 10>(pv, cv, g.cen0, tg[j], d.P.gamma);
+          if (nq == 6) prim2cons<1, 6>(pv, cv, g.cen0, tg[j], d.P.gamma, d.P.synge);
+          else prim2cons<1, 10>(pv, cv, g.cen0, tg[j], d.P.gamma, d.P.synge);
         } else {
-          if (nq == 6) prim2cons<2, 6>(pv, cv, x[k], tg[j], d.P.gamma);
-          else prim2cons<2, 10>(pv, cv, x[k], tg[j], d.P.gamma);
+          if (nq == 6) prim2cons<2, 6>(pv, cv, x[k], tg[j], d.P.gamma, d.P.synge);
+          else prim2cons<2, 10>(pv, cv, x[k], tg[j], d.P.gamma, d.P.synge);
         }
         out[k] = cv[q];
       }

@@ -42,7 +42,18 @@ constexpr double kBig = 1.e15;           // dt_loc reset value (cell.h:32)
 struct Phys {          // run-time constants shared by all kernels
   double gamma, cfl, vi, shock_thr, gamma_max_init;
   double Nalpha, Nmp, Nme;
+  int synge;           // EOS_ == SYNGE_EOS_ (spherical module)
 };
+
+// FluidState::gamma  rel_sph.cpp:37-52: the adiabatic index; with the Synge
+// equation of state (spherical module only) an effective index from p/rho,
+// between G for a cold gas and (G + 1)/2 for a hot one
+GH_HD double eos_gamma(double rho, double p, double G, bool synge) {
+  if (!synge) return G;
+  double a = p / rho / (G - 1.);
+  double e_ratio = a + sqrt(a * a + 1.);
+  return G - (G - 1.) / 2. * (1. - 1. / (e_ratio * e_ratio));
+}
 
 template <int NQ>
 struct State {
@@ -61,13 +72,14 @@ GH_HD double lfac(const double* prim) {
 
 // FluidState::prim2cons  rel_sph.cpp:55-82 / rel_cart.cpp:37-64
 template <int ND, int NQ>
-GH_HD void prim2cons(const double* prim, double* cons, double r, int geom, double G) {
+GH_HD void prim2cons(const double* prim, double* cons, double r, int geom, double G, bool synge = false) {
   double rho = prim[RHO], p = prim[PPP];
   double u = 0;
   for (int i = 0; i < ND; ++i) u += prim[UU1 + i] * prim[UU1 + i];
   u = sqrt(u);
   double lf = sqrt(1 + u * u);
-  double h = 1. + p * G / (G - 1.) / rho;
+  double gma = eos_gamma(rho, p, G, synge && geom == SPH);
+  double h = 1. + p * gma / (gma - 1.) / rho;
   double D = lf * rho;
   cons[DEN] = D;
   cons[TAU] = D * h * lf - p - D;
@@ -102,10 +114,11 @@ GH_HD void state2flux(const double* prim, const double* cons, double* flux, int 
 }
 
 // cons2prim residual  rel_sph.cpp:191-210 (fabs) / rel_cart.cpp:108-123
-GH_HD double c2p_f(double p, double D, double S, double E, double G, int geom) {
+GH_HD double c2p_f(double p, double D, double S, double E, double G, int geom, bool synge = false) {
   double w = 1. - (S * S) / ((E + p) * (E + p));
   double lf = 1. / sqrt(geom == SPH ? fabs(w) : w);
-  return (E + p - D * lf - (G * p * lf * lf) / (G - 1.));
+  double gma = eos_gamma(D / lf, p, G, synge && geom == SPH);   // rel_sph.cpp:199-203
+  return (E + p - D * lf - (gma * p * lf * lf) / (gma - 1.));
 }
 
 // Newton's residual in double-double arithmetic, scaled by a = 1/W^2 > 0
@@ -185,9 +198,9 @@ struct C2PStat { int iters; int fail; int evals; int used; };
 // rel_sph.cpp:273-285: root = best iterate b, bracket = [min(b,c), max(b,c)],
 // stop on |hi-lo| < 1e-13 * min(|lo|,|hi|) (0 if the bracket straddles 0).
 GH_HD double brent(double D, double S, double E, double G, int geom, double lo, double hi,
-                   int maxit, C2PStat* st) {
+                   int maxit, C2PStat* st, bool synge = false) {
   double a = lo, b = hi;
-  double fa = c2p_f(lo, D, S, E, G, geom), fb = c2p_f(hi, D, S, E, G, geom);
+  double fa = c2p_f(lo, D, S, E, G, geom, synge), fb = c2p_f(hi, D, S, E, G, geom, synge);
   double c = b, fc = fb, d = b - a, e = b - a;
   double root = 0.5 * (lo + hi), xl = lo, xu = hi;
   int it = 0;
@@ -226,7 +239,7 @@ GH_HD double brent(double D, double S, double E, double G, int geom, double lo, 
       }
       a = b; fa = fb;
       if (fabs(d) > tol) b += d; else b += (m > 0 ? +tol : -tol);
-      fb = c2p_f(b, D, S, E, G, geom);
+      fb = c2p_f(b, D, S, E, G, geom, synge);
       root = b;
       if ((fb < 0 && fc < 0) || (fb > 0 && fc > 0)) c = a;
       if (b < c) { xl = b; xu = c; } else { xl = c; xu = b; }
@@ -388,7 +401,7 @@ GH_HD void c2p_inputs(const double* cons, double r, int geom, double* D, double*
 // cons on exit = prim2cons(prim) (rel_sph.cpp:318).
 template <int ND, int NQ>
 GH_HD void cons2prim(double* prim, double* cons, double r, int geom, double G, int solver,
-                     double guess_override, int maxit, C2PStat* st) {
+                     double guess_override, int maxit, C2PStat* st, bool synge = false) {
   double D = cons[DEN], tau = cons[TAU], E = D + tau;
   double ss[2] = {cons[SS1], ND == 2 ? cons[SS2] : 0.};
   if (ND == 2 && geom == SPH) ss[1] /= r;
@@ -415,24 +428,24 @@ GH_HD void cons2prim(double* prim, double* cons, double r, int geom, double G, i
     if (st->fail & 1) pCand = pprev;                 // no bracket: keep the previous pressure
     st->fail |= f0;
   } else {
-    double f_init = c2p_f(pCand, D, S, E, G, geom);
+    double f_init = c2p_f(pCand, D, S, E, G, geom, synge);
     if (fabs(f_init) > 0.) {
       double p_lo = fmax(0., (1. + 1e-13) * S - E), p_hi;
       bool find_p = true;
       st->evals += 2;
-      if (c2p_f(p_lo, D, S, E, G, geom) < 0) find_p = false;
+      if (c2p_f(p_lo, D, S, E, G, geom, synge) < 0) find_p = false;
       if (f_init < 0) p_hi = 1. * pCand;
       else {
         p_hi = pCand;
         int guard = 0;
-        while (c2p_f(p_hi, D, S, E, G, geom) > 0) {
+        while (c2p_f(p_hi, D, S, E, G, geom, synge) > 0) {
           st->evals++;
           p_hi *= 10;
           if (++guard > 400) { find_p = false; st->fail |= 4; break; }
         }
       }
       if (p_hi < p_lo) find_p = false;
-      if (find_p) pCand = brent(D, S, E, G, geom, p_lo, p_hi, maxit, st);
+      if (find_p) pCand = brent(D, S, E, G, geom, p_lo, p_hi, maxit, st, synge);
       else {
         st->fail |= 1;
         pCand = pprev;
@@ -454,7 +467,7 @@ GH_HD void cons2prim(double* prim, double* cons, double r, int geom, double G, i
   prim[PPP] = p;
   prim[RHO] = rho;
   for (int t = TR1; t < NQ; ++t) prim[t] = cons[t] / D;
-  prim2cons<ND, NQ>(prim, cons, r, geom, G);
+  prim2cons<ND, NQ>(prim, cons, r, geom, G, synge);
 }
 
 // ---- HLLC (Mignone & Bodo 2006) through a moving face ----
@@ -725,7 +738,7 @@ GH_HD double minmod(double a, double b) {      // 1d.cpp:250-260
 // Cell::sourceTerms  rel_sph.cpp:452-496 (cartesian: none)
 template <int ND, int NQ>
 GH_HD void sources(const double* prim, double* U, double x, double dx, const Trig& T, double dV,
-                   double dt, int geom, double G) {
+                   double dt, int geom, double G, bool synge = false) {
   if (geom != SPH) return;
   double r = x, dr = dx;
   double r1 = r - dr / 2., r2 = r + dr / 2.;
@@ -734,7 +747,8 @@ GH_HD void sources(const double* prim, double* U, double x, double dx, const Tri
     U[SS1] += (2. * prim[PPP]) * (r / rsq) * dV * dt;
   } else {
     double rho = prim[RHO], ut = prim[UU2], p = prim[PPP];
-    double h = 1. + p * G / (G - 1.) / rho;
+    double gma = eos_gamma(rho, p, G, synge);
+    double h = 1. + p * gma / (gma - 1.) / rho;
     U[SS1] += (rho * h * ut * ut + 2. * p) * (r / rsq) * dV * dt;
     U[SS2] += p * T.c_y / fabs(T.s_y) * dV * dt;
   }
@@ -742,7 +756,7 @@ GH_HD void sources(const double* prim, double* U, double x, double dx, const Tri
 
 // Cell::radiativeSourceTerms  radiation_sph.cpp:285-300
 GH_HD void radiative(const double* prim, double* U, double dV, double dt, const Phys& P) {
-  double rho = prim[RHO], p = prim[PPP], gma = P.gamma;
+  double rho = prim[RHO], p = prim[PPP], gma = eos_gamma(rho, p, P.gamma, P.synge);
   double h = 1 + p * gma / (gma - 1.) / rho;
   double eps = rho * (h - 1.) / gma;
   double eB = kEpsB * eps;
@@ -762,7 +776,7 @@ GH_HD void inject(double* prim, bool shocked, double pspec, const Phys& P) {
     if (pspec > prim[PSN] || isnan(prim[PSN])) prim[PSN] = pspec;
     double gmax = P.gamma_max_init;
     prim[GMX] = (gmax != 0. ? lf * r43 / gmax : 0.) / (lf * rho);
-    double gma = P.gamma, p = prim[PPP], psyn = prim[PSN];
+    double p = prim[PPP], psyn = prim[PSN], gma = eos_gamma(rho, p, P.gamma, P.synge);
     double h = 1. + p * gma / (gma - 1.) / rho;
     double eps = rho * (h - 1.) / gma;
     double ee = kEpsE * eps;

@@ -37,7 +37,7 @@ class Config(C.Structure):
         ("Nalpha", C.c_double), ("Nmp", C.c_double), ("Nme", C.c_double),
         ("rg_mode", C.c_int), ("rg", C.c_double * 6),
         ("c2p_solver", C.c_int), ("c2p_guess", C.c_int), ("c2p_maxiter", C.c_int),
-        ("timestep", C.c_int), ("recon", C.c_int), ("vpspec", C.c_int),
+        ("timestep", C.c_int), ("recon", C.c_int), ("vpspec", C.c_int), ("eos", C.c_int),
     ]
 
 
@@ -123,7 +123,7 @@ def make_config(rc: S.RunConfig, ntracks_active: int, solver=C2P_BRENT, guess=0)
     for k in range(6):
         c.rg[k] = rc.rg[k]
     c.c2p_solver, c.c2p_guess, c.c2p_maxiter = solver, guess, rc.c2p_maxiter
-    c.timestep, c.recon, c.vpspec = rc.timestep, rc.recon, rc.vpspec
+    c.timestep, c.recon, c.vpspec, c.eos = rc.timestep, rc.recon, rc.vpspec, rc.eos
     return c
 
 

@@ -30,6 +30,7 @@ TR1, GMN, GMX, GMD, PSN = 4, 6, 7, 8, 9
 GEOM_CART, GEOM_SPH = 0, 1
 TS_RK3, TS_EULER = 0, 1
 RECON_PLM, RECON_PC = 0, 1
+EOS_IDEAL, EOS_SYNGE = 0, 1
 KIN_NONE, KIN_FIXED = 0, 1
 BND_NONE, BND_BM = 0, 1
 RG_NONE, RG_AR2D, RG_BM1D = 0, 1, 2
@@ -79,6 +80,7 @@ class RunConfig:
     timestep: int = TS_RK3        # TIMESTEP_
     recon: int = RECON_PLM        # SPATIAL_RECONSTRUCTION_
     vpspec: int = 0               # VARIABLE_PSPEC_
+    eos: int = EOS_IDEAL          # EOS_
 
     @property
     def nq(self) -> int:          # NUM_Q = NUM_C(4) + NUM_TR(2) + NUM_S

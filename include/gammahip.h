@@ -54,6 +54,8 @@ typedef struct gammahip_config {
                       /* constant (1d.cpp:378-381, 2d.cpp:439-447)                    */
   int    vpspec;      /* VARIABLE_PSPEC_: spectral index from the shock's four-velocity */
                       /* (radiation_sph.cpp:18-35, 146-155); 0: the constant p_       */
+  int    eos;         /* EOS_: 0 ideal (GAMMA_), 1 Synge (rel_sph.cpp:42-50, spherical */
+                      /* module; Brent cons2prim only)                                */
 } gammahip_config;
 
 /* Per-track hook parameters (userKinematics / userBoundaries /

@@ -102,15 +102,25 @@ static double fs_lfac(const og_config *c, const FS *s) {
 }
 
 /* FluidState::prim2cons  rel_sph.cpp:55-82, rel_cart.cpp:37-64
- * (gamma() is the ideal-EOS branch, rel_sph.cpp:39-40).  The 1D spherical
+ * (the index from og_gamma: GAMMA_ or the Synge branch).  The 1D spherical
  * build scales the dead slot cons[SS2] by r (rel_sph.cpp:78); kept. */
+/* FluidState::gamma  rel_sph.cpp:37-52: GAMMA_ (ideal EOS) or the Synge
+ * effective index (SYNGE_EOS_, a function of p/rho); sph: the spherical
+ * module, the only one whose gamma() has the Synge branch */
+static double og_gamma(const og_config *c, double rho, double p, int sph) {
+  if (c->eos != 1 || !sph) return c->gamma;
+  double a = p / rho / (c->gamma - 1.);
+  double e_ratio = a + sqrt(a * a + 1.);
+  return c->gamma - (c->gamma - 1.) / 2. * (1. - 1. / (e_ratio * e_ratio));
+}
+
 static void fs_prim2cons(const og_config *c, FS *s, double r) {
   double rho = s->prim[OG_RHO], p = s->prim[OG_PPP];
   double u = 0, uu[DM];
   for (int i = 0; i < c->nd; ++i) { uu[i] = s->prim[OG_UU1 + i]; u += uu[i] * uu[i]; }
   u = sqrt(u);
   double lfac = sqrt(1 + u * u);
-  double gma = c->gamma;
+  double gma = og_gamma(c, rho, p, c->geom == OG_GEOM_SPH);
   double h = 1. + p * gma / (gma - 1.) / rho;
   double D = lfac * rho;
   double tau = D * h * lfac - p - D;
@@ -165,7 +175,8 @@ static double c2p_f(const og_config *c, double p, double D, double S, double E, 
   }
   if (c->geom == OG_GEOM_SPH) lfac = 1. / sqrt(fabs(1. - (S * S) / ((E + p) * (E + p))));
   else                        lfac = 1. / sqrt(1. - (S * S) / ((E + p) * (E + p)));
-  double gma = c->gamma;
+  /* rel_sph.cpp:199-203: the index of the state (D / lfac, p) */
+  double gma = og_gamma(c, D / lfac, p, c->geom == OG_GEOM_SPH);
   return (E + p - D * lfac - (gma * p * lfac * lfac) / (gma - 1.));
 }
 
@@ -561,7 +572,7 @@ static double gammae2trac(const og_config *c, double gme, const FS *S) {   /* :1
 
 static double gamma_min_init(const og_config *c, const FS *S) {           /* :217-232 */
   double rho = S->prim[OG_RHO], p = S->prim[OG_PPP], psyn = S->prim[OG_PSN];
-  double gma = c->gamma;
+  double gma = og_gamma(c, rho, p, 1);
   double h = 1. + p * gma / (gma - 1.) / rho;
   double eps = rho * (h - 1.) / gma;
   double ee = EPS_E * eps;
@@ -597,7 +608,7 @@ static void cell_inject(const og_config *c, Cell *cl) {                    /* :2
 
 static void cell_radiative(const og_config *c, Cell *cl, double dt) {      /* :285-300 */
   FS *S = &cl->S;
-  double rho = S->prim[OG_RHO], p = S->prim[OG_PPP], gma = c->gamma;
+  double rho = S->prim[OG_RHO], p = S->prim[OG_PPP], gma = og_gamma(c, rho, p, 1);
   double h = 1 + p * gma / (gma - 1.) / rho;
   double eps = rho * (h - 1.) / gma;
   double eB = EPS_B * eps;
@@ -635,7 +646,7 @@ static void cell_sources(const og_config *c, Cell *cl, double dt) {
     cl->S.cons[OG_SS1] += (2. * p) * (r / rsq) * dV * dt;
   } else {
     double rho = cl->S.prim[OG_RHO], ut = cl->S.prim[OG_UU2], p = cl->S.prim[OG_PPP];
-    double gma = c->gamma;
+    double gma = og_gamma(c, rho, p, 1);
     double h = 1. + p * gma / (gma - 1.) / rho;
     double r = cl->G.x[0], dr = cl->G.dx[0], th = cl->G.x[1], dV = cl->G.dV;
     double r1 = r - dr / 2., r2 = r + dr / 2.;

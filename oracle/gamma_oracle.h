@@ -54,6 +54,7 @@ typedef struct {
   int    timestep;    /* TIMESTEP_: 0 RK3, 1 Euler (euler.cpp:12-19) */
   int    recon;       /* SPATIAL_RECONSTRUCTION_: 0 piecewise linear, 1 piecewise constant */
   int    vpspec;      /* VARIABLE_PSPEC_ (radiation_sph.cpp:18-35) */
+  int    eos;         /* EOS_: 0 ideal, 1 Synge (rel_sph.cpp:42-50) */
   double c2p_tol;     /* Brent interval tolerance, 1e-13 in the reference (rel_sph.cpp:281);
                          other values only measure how much the reference's own
                          tolerance moves its trajectories */

@@ -557,3 +557,35 @@ def test_variable_spectral_index_of_an_exact_shock():
     # the constant without the switch
     _, p0 = oc.compute_sd(setups.RunConfig(nd=1, gamma=G53), shocked, ahead)
     assert p0 == 2.22
+
+
+def test_synge_index_limits_and_round_trip():
+    """Synge EOS (rel_sph.cpp:42-50, spherical module): the effective index
+    falls from 5/3 for a cold gas to 4/3 for a hot one, and cons2prim, whose
+    residual takes the index of the state (D/W, p) (rel_sph.cpp:199-203),
+    recovers the pressure prim2cons started from."""
+    rc = setups.RunConfig(nd=1, geom=setups.GEOM_SPH, gamma=G53, eos=setups.EOS_SYNGE)
+    ideal = setups.RunConfig(nd=1, geom=setups.GEOM_SPH, gamma=G53)
+
+    def h_of(cfg, rho, p):
+        pr = np.zeros(cfg.nq); pr[0], pr[1] = rho, p
+        D, tau = oc.prim2cons(cfg, pr)[[0, 1]]
+        return (tau + D + p) / D                     # h = (tau + D + p) / D for a gas at rest
+
+    for theta, g_expect in ((1e-9, 5. / 3.), (1e9, 4. / 3.)):
+        h = h_of(rc, 1., theta)
+        g_eff = (h - 1.) / (h - 1. - theta)          # h = 1 + theta g / (g - 1)
+        assert g_eff == pytest.approx(g_expect, rel=1e-6), theta
+    hs = [h_of(rc, 1., t) for t in np.logspace(-3, 3, 25)]
+    gs = [(h - 1.) / (h - 1. - t) for h, t in zip(hs, np.logspace(-3, 3, 25))]
+    assert all(a > b for a, b in zip(gs, gs[1:]))     # monotone between the limits
+    # the ideal gas differs: the switch is live
+    assert h_of(ideal, 1., 1.) != pytest.approx(h_of(rc, 1., 1.), rel=1e-3)
+    rng = np.random.default_rng(12)
+    for _ in range(200):
+        rho, p, u = 10 ** rng.uniform(-3, 3), 10 ** rng.uniform(-4, 3), rng.uniform(-5, 5)
+        pr = np.zeros(rc.nq); pr[0], pr[1], pr[2] = rho, p, u
+        cons = oc.prim2cons(rc, pr)
+        guess = pr.copy(); guess[1] = p * 1.3
+        out, _, _ = oc.cons2prim(rc, cons, guess)
+        assert out[1] == pytest.approx(p, rel=1e-9), (rho, p, u)

@@ -47,6 +47,12 @@ CASES = {
     # VARIABLE_PSPEC_ (radiation_sph.cpp:18-35): injected spectral index from the shock
     "blandford_mckee_vpspec": (lambda: _variant(setups.blandford_mckee_1d(400), vpspec=1), 60),
     "bm_jet_2d_vpspec": (lambda: _variant(setups.bm_jet_2d(12, 0.3, 40, 160), vpspec=1), 40),
+    # EOS_ == SYNGE_EOS_ (rel_sph.cpp:42-50), with GAMMA_ = 5/3 as the reference requires
+    "shock_tube_sph_synge": (lambda: _variant(setups.shock_tube_1d(200, xmin=0.2, xmax=1.2, xmid=0.7,
+                                                                   geom=setups.GEOM_SPH, shock=1),
+                                              eos=setups.EOS_SYNGE), 80),
+    "bm_jet_2d_synge": (lambda: _variant(setups.bm_jet_2d(12, 0.3, 40, 160, gamma=5. / 3.),
+                                         eos=setups.EOS_SYNGE), 40),
 }
 
 
