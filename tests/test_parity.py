@@ -33,6 +33,9 @@ CASES = {
     "riemann_2d": (lambda: setups.riemann_2d(64, 16), 60),
     # edge cells grow, split next to the ghosts, which must follow their faces
     "expanding_tube": (lambda: setups.expanding_tube_1d(200), 300),
+    # the same with six spare slots: tracks fill their storage and splits wait
+    # for merges (the split guard ntrack < nax - 1, 1d.cpp:527)
+    "expanding_tube_full": (lambda: _variant(setups.expanding_tube_1d(200), nmax=206), 300),
     # 2D spherical blast wave with an angular Lorentz-factor profile: uneven
     # seeded track lengths, per-track BM inflow, shock detection, synchrotron
     "bm_jet_2d": (lambda: setups.bm_jet_2d(12, 0.3, 40, 160), 40),
@@ -294,6 +297,18 @@ def test_emu_parity_f1_sums_in_update(case, emu_lib):
     g.set_options(f1fuse=1)
     g.step(n)
     check_pair(o, g, shock=bool(s.cfg.shock), exact_expected=True)
+
+
+def test_emu_tracks_fill_their_storage(emu_lib):
+    """The capacity case does reach the split guard: the track sits at
+    nax - 1 cells with splits and merges still happening."""
+    mk, n = CASES["expanding_tube_full"]
+    s = mk()
+    g = hip.Grid(s, lib_path=emu_lib)
+    g.step(n)
+    assert g.tracks()["ntrack"][0] == g.nax - 1
+    sp, mg = g.regrid_counts()
+    assert sp[0] > 10 and mg[0] > 10
 
 
 @pytest.mark.parametrize("case", ["riemann_2d", "bm_jet_2d"])
