@@ -132,6 +132,21 @@ GH_HD Trig trig_at(const Dev& d, int j) {
   }
   return t;
 }
+// the same with the geometry given (a template constant where the grid has
+// one geometry: 2D grids), so that the kernel carries only that geometry's code
+template <int ND>
+GH_HD Trig trig_at_g(const Dev& d, int j, int geom) {
+  Trig t = {0., 0., 0., 0., 0., 0., 0.};
+  if (ND == 2 && geom == SPH) {
+    const double* a = d.ttrig + 7 * (long long)j;
+    t.s_hdy = a[0]; t.s_y = a[1]; t.c_y = a[2]; t.s_1 = a[3]; t.s_2 = a[4]; t.c_1 = a[5]; t.c_2 = a[6];
+  }
+  return t;
+}
+template <int ND>
+GH_HD Geo geo_of_g(const Dev& d, int j, long long k, int geom) {
+  return cell_geo<ND>(d.cx[k], d.cdx[k], d.ty[j], d.tdy[j], geom, trig_at_g<ND>(d, j, geom));
+}
 template <int ND>
 GH_HD Geo geo_of(const Dev& d, int j, long long k) {
   return cell_geo<ND>(d.cx[k], d.cdx[k], d.ty[j], d.tdy[j], d.tgeom[j], trig_at<ND>(d, j));
@@ -160,6 +175,16 @@ GH_HD double cell_pc(const Dev& d, int j, int i, double* p, double* c) {
   Geo g = geo_of<ND>(d, j, k);
   double r = sweep_r<ND>(d, k, g);
   prim2cons<ND, NQ>(p, c, r, d.tgeom[j], d.P.gamma, d.P.synge);
+  return r;
+}
+
+template <int ND, int NQ>
+GH_HD double cell_pc_g(const Dev& d, int j, int i, double* p, double* c, int geom) {
+  long long k = CI(d, j, i);
+  load_v<ND, NQ>(d, d.prim, k, p);
+  Geo g = geo_of_g<ND>(d, j, k, geom);
+  double r = sweep_r<ND>(d, k, g);
+  prim2cons<ND, NQ>(p, c, r, geom, d.P.gamma, d.P.synge);
   return r;
 }
 
@@ -405,11 +430,12 @@ GH_HD void body_nb(const Dev& d, int j, int i) {
 // gradients of the first KC neighbours from cache (KC x NQ doubles with
 // stride cs, thread-private) and recomputes the rest, so that a neighbour's
 // state is read once instead of twice.
-template <int NQ, int KC>
+// GEO: the grid's geometry when fixed at compile time (-1: the track's)
+template <int NQ, int KC, int GEO = -1>
 GH_HD void trans_grad(const Dev& d, int j, long long k, const double* p0, const double* m0,
                       const Geo& g0, double* gf, double* cache, int cs) {
-  int geom = d.tgeom[j];
-  Trig Tj = trig_at<2>(d, j);
+  int geom = GEO >= 0 ? GEO : d.tgeom[j];
+  Trig Tj = trig_at_g<2>(d, j, geom);
   double x0 = d.cx[k];
   double xL0 = x0 - d.cdx[k] / 2., xR0 = x0 + d.cdx[k] / 2.;
   double gavg[NQ];
@@ -434,7 +460,7 @@ GH_HD void trans_grad(const Dev& d, int j, long long k, const double* p0, const 
         double xm = (fmax(xL0, xLn) + fmin(xR0, xRn)) / 2.;
         double ov = fmax(0, fmin(xR0, xRn) - fmax(xL0, xLn));
         double dA = face_area<2>(1, xm, ov, geom, side == 0 ? Tj.s_1 : Tj.s_y, 0.);
-        Geo gn = geo_of<2>(d, tj, kn);
+        Geo gn = geo_of_g<2>(d, tj, kn, geom);     // 2D: one geometry for all tracks
         double dxmL = xm - g0.cen0, dxmR = xm - gn.cen0;
         double pn[NQ], mn[NQ];
         load_v<2, NQ>(d, d.prim, kn, pn);
@@ -465,17 +491,17 @@ GH_HD void trans_grad(const Dev& d, int j, long long k, const double* p0, const 
 // KC x NQ doubles per thread)
 template <int NQ> constexpr int f1_kc() { return NQ <= 6 ? 4 : 2; }
 
-template <int NQ>
+template <int NQ, int GEO = -1>
 GH_HD void body_f1grad_c(const Dev& d, int j, int i, double* cache, int cs) {
   if (j > d.nj - 2) return;
   int nt = d.ntrack[j];
   if (i >= nt) return;
   long long k = CI(d, j, i);
-  Geo g0 = geo_of<2>(d, j, k);
+  Geo g0 = geo_of_g<2>(d, j, k, GEO >= 0 ? GEO : d.tgeom[j]);
   double pc[NQ], mc[NQ], fc[NQ];
   load_v<2, NQ>(d, d.prim, k, pc);
   load_v<2, NQ>(d, d.gmv, k, mc);
-  trans_grad<NQ, f1_kc<NQ>()>(d, j, k, pc, mc, g0, fc, cache, cs);
+  trans_grad<NQ, f1_kc<NQ>(), GEO>(d, j, k, pc, mc, g0, fc, cache, cs);
   store_v<2, NQ>(d, d.gf1n, k, fc);
 }
 template <int NQ>
@@ -492,11 +518,11 @@ GH_HD void body_f1grad(const Dev& d, int j, int i) {
 // loops j upward, so track j+1 has not been recomputed yet); stale lists of
 // the top ghost track pointing into their own track read the previous values
 // too (their faces only feed ghost cells).
-template <int NQ, bool SHK>
+template <int NQ, bool SHK, int GEO = -1>
 GH_HD void f1face_at(const Dev& d, int j, int i, int s, const double* pc, const double* mc, const double* fc,
                      const Geo& g0) {
-  int geom = d.tgeom[j];
-  Trig Tj = trig_at<2>(d, j);
+  int geom = GEO >= 0 ? GEO : d.tgeom[j];
+  Trig Tj = trig_at_g<2>(d, j, geom);
   double G = d.P.gamma;
   long long k = CI(d, j, i);
   int tj = d.nbuj[k], lo = d.nbulo[k], n = s - d.f1off[k];
@@ -513,10 +539,10 @@ GH_HD void f1face_at(const Dev& d, int j, int i, int s, const double* pc, const 
   double pL[NQ], cL[NQ], pR[NQ], cR[NQ];
   double rL = xm, rR = xm;
   if (pc0 || d.nbucnt[kn] == 0) {
-    rL = cell_pc<2, NQ>(d, j, i, pL, cL);
-    rR = cell_pc<2, NQ>(d, tj, lo + n, pR, cR);
+    rL = cell_pc_g<2, NQ>(d, j, i, pL, cL, geom);
+    rR = cell_pc_g<2, NQ>(d, tj, lo + n, pR, cR, geom);
   } else {
-    Geo gn = geo_of<2>(d, tj, kn);
+    Geo gn = geo_of_g<2>(d, tj, kn, geom);
     double dxmL = xm - g0.cen0, dxmR = xm - gn.cen0;
     double dxfL = jpos - g0.cen1, dxfR = jpos - gn.cen1;
     double pn[NQ], mn[NQ], fn[NQ];
@@ -560,7 +586,7 @@ GH_HD void f1face_at(const Dev& d, int j, int i, int s, const double* pc, const 
 }
 
 // Face pass: one F1 face slot s of track j.
-template <int NQ, bool SHK>
+template <int NQ, bool SHK, int GEO = -1>
 GH_HD void body_f1face_s(const Dev& d, int j, int s) {
   if (j > d.nj - 2) return;
   int tot = d.f1tot[j];
@@ -571,7 +597,7 @@ GH_HD void body_f1face_s(const Dev& d, int j, int s) {
   load_v<2, NQ>(d, d.prim, k, pc);
   load_v<2, NQ>(d, d.gmv, k, mc);
   load_v<2, NQ>(d, d.gf1n, k, fc);
-  f1face_at<NQ, SHK>(d, j, i, s, pc, mc, fc, geo_of<2>(d, j, k));
+  f1face_at<NQ, SHK, GEO>(d, j, i, s, pc, mc, fc, geo_of_g<2>(d, j, k, GEO >= 0 ? GEO : d.tgeom[j]));
 }
 template <int NQ>
 GH_HD void body_f1face(const Dev& d, int j, int s) {
@@ -1176,11 +1202,11 @@ template <int ND, int NQ, bool SHK> GH_KERNEL2P(k_faces, (body_faces_s<ND, NQ, S
 // cell pass with the gradient cache in LDS: thread t's entry m at
 // cache[m * blockDim + t] (consecutive lanes, consecutive banks); the launch
 // passes blockDim * f1_kc * NQ doubles of dynamic LDS
-template <int NQ>
+template <int NQ, int GEO>
 __global__ void __launch_bounds__(KB) k_f1grad(Dev d, int ni, int remap) {
   extern __shared__ double f1cache[];
   double* cache = f1cache + threadIdx.x;
-  GH_TILE_LOOP(if (i < ni && j < d.nj) body_f1grad_c<NQ>(d, j, i, cache, (int)blockDim.x))
+  GH_TILE_LOOP(if (i < ni && j < d.nj) body_f1grad_c<NQ, GEO>(d, j, i, cache, (int)blockDim.x))
 }
 // fused gradient + own-face pass (same LDS cache as k_f1grad)
 template <int NQ>
@@ -1196,13 +1222,13 @@ template <int NQ> GH_KERNEL2P(k_f1sum, body_f1sum<NQ>)
 // accordingly, spilling if it must): 1 leaves the choice to the compiler, 4
 // trades spills for latency hiding.  The launch option picks one.
 // (XCD-aware with remap, as the cell passes: the face reads the upper track)
-template <int NQ, int W, bool SHK>
+template <int NQ, int W, bool SHK, int GEO>
 __global__ void __launch_bounds__(KB, W) k_f1faces(Dev d, int remap) {
   int j, s0;
   tile_of(remap, j, s0);
   int stride = gridDim.x * blockDim.x;
   int tot = min(d.f1tot[j], d.fcap);
-  for (int s = s0; s < tot; s += stride) body_f1face_s<NQ, SHK>(d, j, s);
+  for (int s = s0; s < tot; s += stride) body_f1face_s<NQ, SHK, GEO>(d, j, s);
 }
 // W: minimum wavefronts per SIMD (1 = the compiler's choice; 5 caps the
 // registers at 96 and spills the rest), picked by the launch option
@@ -1918,7 +1944,8 @@ struct Pipe {
           int bs = c->block;
           dim3 g = d.packed ? dim3(c->persist * c->ncu) : dim3((d.nax + bs - 1) / bs, d.nj);
           size_t lds = sizeof(double) * bs * f1_kc<NQ>() * NQ;
-          hipLaunchKernelGGL(k_f1grad<NQ>, g, dim3(bs), lds, c->stream, d, d.nax, c->remap);
+          if (c->cfg.geom == SPH) hipLaunchKernelGGL((k_f1grad<NQ, SPH>), g, dim3(bs), lds, c->stream, d, d.nax, c->remap);
+          else hipLaunchKernelGGL((k_f1grad<NQ, CART>), g, dim3(bs), lds, c->stream, d, d.nax, c->remap);
         }
 #endif
         tend(c, 3);
@@ -1930,12 +1957,15 @@ struct Pipe {
         {
           int bs = c->block;
           dim3 g((d.nax + bs - 1) / bs, d.nj);
+          // 2D grids have one geometry: the kernel is built for it
+          bool sph = c->cfg.geom == SPH, cap = (c->occ & 1) != 0;
+          auto launch = [&](auto kern) { hipLaunchKernelGGL(kern, g, dim3(bs), 0, c->stream, d, c->remap); };
           if (d.shock) {
-            if (c->occ & 1) hipLaunchKernelGGL((k_f1faces<NQ, 4, true>), g, dim3(bs), 0, c->stream, d, c->remap);
-            else hipLaunchKernelGGL((k_f1faces<NQ, 1, true>), g, dim3(bs), 0, c->stream, d, c->remap);
+            if (sph) { if (cap) launch(k_f1faces<NQ, 4, true, SPH>); else launch(k_f1faces<NQ, 1, true, SPH>); }
+            else { if (cap) launch(k_f1faces<NQ, 4, true, CART>); else launch(k_f1faces<NQ, 1, true, CART>); }
           } else {
-            if (c->occ & 1) hipLaunchKernelGGL((k_f1faces<NQ, 4, false>), g, dim3(bs), 0, c->stream, d, c->remap);
-            else hipLaunchKernelGGL((k_f1faces<NQ, 1, false>), g, dim3(bs), 0, c->stream, d, c->remap);
+            if (sph) { if (cap) launch(k_f1faces<NQ, 4, false, SPH>); else launch(k_f1faces<NQ, 1, false, SPH>); }
+            else { if (cap) launch(k_f1faces<NQ, 4, false, CART>); else launch(k_f1faces<NQ, 1, false, CART>); }
           }
         }
 #endif
