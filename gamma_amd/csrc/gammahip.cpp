@@ -1747,8 +1747,13 @@ struct gammahip_ctx {
   int ncu = 256;           // compute units of the device
 #ifndef GH_EMU
   hipStream_t stream = nullptr;
-  hipEvent_t ev0[16], ev1[16];
-  bool ev_pending[16];
+  // event pairs of timed launches, read back when the times are asked for
+  // (or the pool is full): no synchronisation between the timed kernels, so
+  // each starts behind its predecessor as in an untimed step
+  static constexpr int NEV = 128;
+  hipEvent_t ev0[NEV], ev1[NEV];
+  int ev_id[NEV];
+  int nev = 0;
   hipGraphExec_t graph = nullptr;   // two steps (an even number of buffer swaps)
   void* mlp_w = nullptr;            // learned-guess parameters (gammahip_mlp_load)
   void* mlp_w2b = nullptr;
@@ -1857,9 +1862,28 @@ void build_tiles(gammahip_ctx* c) {
 #endif
 }
 
+// add the recorded launch times to the per-kernel sums
+void tcollect(gammahip_ctx* c) {
+#ifndef GH_EMU
+  if (c->nev == 0) return;
+  hipEventSynchronize(c->ev1[c->nev - 1]);
+  for (int e = 0; e < c->nev; ++e) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, c->ev0[e], c->ev1[e]) != hipSuccess) continue;
+    c->kt[c->ev_id[e]].ms += ms;
+    c->kt[c->ev_id[e]].n++;
+  }
+  c->nev = 0;
+#else
+  (void)c;
+#endif
+}
 void tbeg(gammahip_ctx* c, int id) {
 #ifndef GH_EMU
-  if (c->timing) hipEventRecord(c->ev0[id], c->stream);
+  if (!c->timing) return;
+  if (c->nev == gammahip_ctx::NEV) tcollect(c);
+  c->ev_id[c->nev] = id;
+  hipEventRecord(c->ev0[c->nev], c->stream);
 #else
   (void)c; (void)id;
 #endif
@@ -1867,13 +1891,9 @@ void tbeg(gammahip_ctx* c, int id) {
 void tend(gammahip_ctx* c, int id) {
 #ifndef GH_EMU
   if (!c->timing) return;
-  hipEventRecord(c->ev1[id], c->stream);
-  // collect the previous measurement lazily at the next sync point
-  hipEventSynchronize(c->ev1[id]);
-  float ms = 0.f;
-  hipEventElapsedTime(&ms, c->ev0[id], c->ev1[id]);
-  c->kt[id].ms += ms;
-  c->kt[id].n++;
+  (void)id;
+  hipEventRecord(c->ev1[c->nev], c->stream);
+  c->nev++;
 #else
   (void)c; (void)id;
 #endif
@@ -2142,7 +2162,7 @@ int gammahip_create(const gammahip_config* cfg, gammahip_ctx** out) {
     if (hipGetDevice(&dev) == hipSuccess &&
         hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && ncu > 0)
       c->ncu = (ncu + 7) / 8 * 8;     // persistent grids are multiples of 8 (one band per XCD)
-    for (int k = 0; k < 16; ++k) {
+    for (int k = 0; k < gammahip_ctx::NEV; ++k) {
       hipEventCreate(&c->ev0[k]);
       hipEventCreate(&c->ev1[k]);
     }
@@ -2213,7 +2233,7 @@ int gammahip_destroy(gammahip_ctx* c) {
   if (c->mlp_w) (void)hipFree(c->mlp_w);
   if (c->mlp_w2b) (void)hipFree(c->mlp_w2b);
   if (c->stream) {
-    for (int k = 0; k < 16; ++k) { hipEventDestroy(c->ev0[k]); hipEventDestroy(c->ev1[k]); }
+    for (int k = 0; k < gammahip_ctx::NEV; ++k) { hipEventDestroy(c->ev0[k]); hipEventDestroy(c->ev1[k]); }
     hipStreamDestroy(c->stream);
   }
 #endif
@@ -2864,6 +2884,7 @@ int gammahip_mfma_probe(const float* A, const float* Bm, float* Dm) {
 
 int gammahip_kernel_times(gammahip_ctx* c, double* ms, long long* n) {
   c->timing = true;     // the first call switches event timing on
+  tcollect(c);
   for (int k = 0; k < 16; ++k) {
     ms[k] = c->kt[k].n ? c->kt[k].ms / c->kt[k].n : 0.;
     n[k] = c->kt[k].n;
