@@ -40,7 +40,7 @@ using namespace gh;
 namespace {
 
 enum { CNT_C2P_CALLS = 0, CNT_C2P_ITERS, CNT_NOBRACKET, CNT_MAXITER, CNT_PHICAP, CNT_NEGDT,
-       CNT_NANFLUX, CNT_F1OVF, CNT_SPLITS, CNT_MERGES, CNT_MLP, CNT_N = 16 };
+       CNT_NANFLUX, CNT_F1OVF, CNT_SPLITS, CNT_MERGES, CNT_MLP, CNT_RGCLASH, CNT_N = 16 };
 // Counters live in CNT_PARTS copies of one 128-byte line each; a workgroup adds
 // into copy (linear workgroup id mod CNT_PARTS) and the host sums the copies.
 // Device-wide atomics to one address serialise at the memory side (the XCDs
@@ -60,6 +60,7 @@ struct Dev {
   int pc;                  // piecewise-constant reconstruction instead of PLM
   int vpspec;              // VARIABLE_PSPEC_: spectral index from the shock's four-velocity
   int rg_mode;             // bit set of the tracks' regrid rules (1 << RG_*)
+  int circ;                // CIRC_REGRID_: splits and merges in pairs with the track's victims
   long long cs;            // nj * nax
   long long fs1;           // nj * fcap
   Phys P;
@@ -921,6 +922,39 @@ GH_HD int rg_check(const Dev& d, int j, int i, double iS) {
   return RG_SKIP;
 }
 
+// Cell::regridVal with the hook set's user_regridVal: the geometry's value
+// (cartesian.cpp:58-63 dx; spherical1D.cpp:79-86 dr/r; spherical.cpp:115-123
+// dr/(r dth), dth the track's width), replaced under the BM1D hooks by
+// dr/(r dth) lfac^(3/2) with the rule's dth (BM1D.cpp:322-331); the 2DST hooks
+// keep the geometry's value (2DST.cpp:171-178)
+template <int ND, int NQ>
+GH_HD double rg_val(const Dev& d, int j, long long k) {
+  double r = d.cx[k], dr = d.cdx[k];
+  if (d.trg[j] == RG_BM1D) {
+    double p[NQ];
+    load_v<ND, NQ>(d, d.prim, k, p);
+    return dr / (r * d.rg[2]) * gh_pow(lfac<ND, NQ>(p), 3. / 2.);
+  }
+  if (d.tgeom[j] != SPH) return dr;
+  return ND == 1 ? dr / r : dr / (r * d.tdy[j]);
+}
+// Grid::targetRegridVictims (1d.cpp:498-520, 2d.cpp:625-646): first cell of the
+// smallest value below 1e20 and first of the largest above 0 among cells
+// iLbnd+3 .. iRbnd-3, negative values skipped; -1 where none qualifies
+template <int ND, int NQ>
+GH_HD void rg_victims(const Dev& d, int j, int* is, int* ib) {
+  int nt = d.ntrack[j], iL = d.ngst - 1, iR = nt - d.ngst;
+  double mn = 1.e20, mx = 0.;
+  *is = -1;
+  *ib = -1;
+  for (int i = iL + 3; i <= iR - 3; ++i) {
+    double v = rg_val<ND, NQ>(d, j, CI(d, j, i));
+    if (v < 0) continue;
+    if (v < mn) { mn = v; *is = i; }
+    if (v > mx) { mx = v; *ib = i; }
+  }
+}
+
 // shock position of BM1D.cpp:292-301: last active cell with p > 1.5 eta, else iLbnd+1
 GH_HD double rg_shock_index(const Dev& d, int j) {
   int nt = d.ntrack[j], iL = d.ngst - 1, iR = nt - d.ngst;
@@ -1073,7 +1107,12 @@ GH_HD void body_rg_screen(const Dev& d, int j, int i) {
   bool ok = (a == RG_SPLIT && nt < d.nax - 1) || (a == RG_MERGE && nt > 2 * d.ngst + 2);
   if (ok) GH_ATOMIC_MIN(&d.tfirst[j], i);
 }
-// sequential applyRegrid loop of one track from its first flagged cell
+// sequential applyRegrid loop of one track from its first flagged cell (the
+// cells before it take no action, so the victims found at the start of the
+// track's loop are the same); with CIRC_REGRID_ a split of i first merges the
+// smallest cell is (then splits i-1 if is < i), a merge of i is followed by a
+// split of the biggest cell ib (ib-1 if ib > i), and a victim equal to i -- the
+// reference's exit(10) -- is counted and ends the track's regrid
 template <int ND, int NQ>
 GH_HD void body_rg_apply(const Dev& d, int j, int i0) {
   if (i0 != 0 || !active_track(d, j)) return;
@@ -1081,18 +1120,37 @@ GH_HD void body_rg_apply(const Dev& d, int j, int i0) {
   if (i == INT_MAX) return;
   bool bm = d.trg[j] == RG_BM1D;
   double iS = bm ? rg_shock_index(d, j) : 0.;
+  int is = -1, ib = -1;
+  if (d.circ) rg_victims<ND, NQ>(d, j, &is, &ib);
   for (; i <= d.ntrack[j] - d.ngst - 1; ++i) {
     int a = rg_check<ND, NQ>(d, j, i, iS);
     int nt = d.ntrack[j];
+    bool acted = false;
     if (a == RG_SPLIT && nt < d.nax - 1) {
-      rg_split<ND, NQ>(d, j, i);
+      int isplit = i;
+      if (d.circ) {
+        if (is < 0 || is == i) { GH_ATOMIC_ADD(cnt_at(d, CNT_RGCLASH), 1ull); return; }
+        rg_merge<ND, NQ>(d, j, is);
+        d.nmerge[j]++;
+        if (i > is) isplit--;
+      }
+      rg_split<ND, NQ>(d, j, isplit);
       d.nsplit[j]++;
-      if (bm) iS = rg_shock_index(d, j);
+      acted = true;
     }
     if (a == RG_MERGE && nt > 2 * d.ngst + 2) {
+      if (d.circ && (ib < 0 || ib == i)) { GH_ATOMIC_ADD(cnt_at(d, CNT_RGCLASH), 1ull); return; }
       rg_merge<ND, NQ>(d, j, i);
       d.nmerge[j]++;
+      if (d.circ) {
+        rg_split<ND, NQ>(d, j, ib > i ? ib - 1 : ib);
+        d.nsplit[j]++;
+      }
+      acted = true;
+    }
+    if (acted) {
       if (bm) iS = rg_shock_index(d, j);
+      if (d.circ) rg_victims<ND, NQ>(d, j, &is, &ib);
     }
   }
 }
@@ -1427,10 +1485,72 @@ __device__ double wg_shock_index(const Dev& d, int j, int* red) {
   __syncthreads();
   return r;
 }
+// rg_victims over the workgroup: each lane keeps the first minimum and the
+// first maximum of its cells (taken in increasing order), and the reduction
+// prefers the smaller value, then the smaller index -- the first cell of the
+// extreme value, as the sequential scan finds it
+template <int ND, int NQ>
+__device__ void wg_victims(const Dev& d, int j, double* rv, int* ri, int* is, int* ib) {
+  int nt = d.ntrack[j], iL = d.ngst - 1, iR = nt - d.ngst;
+  int t = threadIdx.x;
+  double mn = 1.e20, mx = 0.;
+  int imn = -1, imx = -1;
+  for (int ii = iL + 3 + t; ii <= iR - 3; ii += RGB) {
+    double v = rg_val<ND, NQ>(d, j, CI(d, j, ii));
+    if (v < 0) continue;
+    if (v < mn) { mn = v; imn = ii; }
+    if (v > mx) { mx = v; imx = ii; }
+  }
+  for (int side = 0; side < 2; ++side) {
+    rv[t] = side == 0 ? mn : mx;
+    ri[t] = side == 0 ? imn : imx;
+    __syncthreads();
+    for (int w = RGB / 2; w > 0; w >>= 1) {
+      if (t < w) {
+        double o = rv[t + w];
+        int oi = ri[t + w];
+        bool better = side == 0 ? o < rv[t] : o > rv[t];
+        if (oi >= 0 && (ri[t] < 0 || better || (o == rv[t] && oi < ri[t]))) { rv[t] = o; ri[t] = oi; }
+      }
+      __syncthreads();
+    }
+    if (side == 0) *is = ri[0]; else *ib = ri[0];
+    __syncthreads();
+  }
+}
+// Grid::split and Grid::merge of cell i over the workgroup: the cells after
+// the action move one slot (wg_shift), its arithmetic runs on lane 0
+template <int ND, int NQ>
+__device__ void wg_split(const Dev& d, int j, long long b, int i) {
+  int nt = d.ntrack[j];
+  __syncthreads();
+  wg_shift<ND, NQ>(d, b, i + 1, nt, i, nt - 1, +1);
+  if (threadIdx.x == 0) {
+    d.ntrack[j] = nt + 1;
+    split_fill<ND, NQ>(d, j, i);
+    d.nsplit[j]++;
+  }
+  __syncthreads();
+}
+template <int ND, int NQ>
+__device__ void wg_merge(const Dev& d, int j, long long b, int i, int* s_left) {
+  int nt = d.ntrack[j];
+  __syncthreads();
+  if (threadIdx.x == 0) *s_left = merge_fill<ND, NQ>(d, j, i) ? 1 : 0;
+  __syncthreads();
+  if (*s_left) wg_shift<ND, NQ>(d, b, i, nt, i, nt - 1, -1);
+  else wg_shift<ND, NQ>(d, b, i + 2, nt, i + 1, nt - 1, -1);
+  if (threadIdx.x == 0) {
+    d.ntrack[j] = nt - 1;
+    d.nmerge[j]++;
+  }
+  __syncthreads();
+}
 template <int ND, int NQ>
 __global__ void __launch_bounds__(RGB) k_rg_apply(Dev d) {
   __shared__ int red[RGB];
-  __shared__ int s_act, s_pos;
+  __shared__ double rv[RGB];
+  __shared__ int s_act;
   int j = blockIdx.x;
   if (!active_track(d, j)) return;
   int i = d.tfirst[j];
@@ -1438,6 +1558,8 @@ __global__ void __launch_bounds__(RGB) k_rg_apply(Dev d) {
   bool bm = d.trg[j] == RG_BM1D;
   long long b = CI(d, j, 0);
   double iS = bm ? wg_shock_index(d, j, red) : 0.;
+  int is = -1, ib = -1;
+  if (d.circ) wg_victims<ND, NQ>(d, j, rv, red, &is, &ib);
   for (;;) {
     int nt = d.ntrack[j];
     int last = nt - d.ngst - 1;
@@ -1465,28 +1587,28 @@ __global__ void __launch_bounds__(RGB) k_rg_apply(Dev d) {
     i += first;
     int act = s_act;
     __syncthreads();
+    // is, ib and i are the same on every lane: a clash returns the whole workgroup
     if (act == 1) {
-      wg_shift<ND, NQ>(d, b, i + 1, nt, i, nt - 1, +1);
-      if (threadIdx.x == 0) {
-        d.ntrack[j] = nt + 1;
-        split_fill<ND, NQ>(d, j, i);
-        d.nsplit[j]++;
+      int isplit = i;
+      if (d.circ) {
+        if (is < 0 || is == i) {
+          if (threadIdx.x == 0) GH_ATOMIC_ADD(cnt_at(d, CNT_RGCLASH), 1ull);
+          return;
+        }
+        wg_merge<ND, NQ>(d, j, b, is, &s_act);
+        if (i > is) isplit--;
       }
-      __syncthreads();
-      if (bm) iS = wg_shock_index(d, j, red);
+      wg_split<ND, NQ>(d, j, b, isplit);
     } else {
-      if (threadIdx.x == 0) s_act = merge_fill<ND, NQ>(d, j, i) ? 1 : 0;
-      __syncthreads();
-      bool left = s_act == 1;
-      if (left) wg_shift<ND, NQ>(d, b, i, nt, i, nt - 1, -1);
-      else wg_shift<ND, NQ>(d, b, i + 2, nt, i + 1, nt - 1, -1);
-      if (threadIdx.x == 0) {
-        d.ntrack[j] = nt - 1;
-        d.nmerge[j]++;
+      if (d.circ && (ib < 0 || ib == i)) {
+        if (threadIdx.x == 0) GH_ATOMIC_ADD(cnt_at(d, CNT_RGCLASH), 1ull);
+        return;
       }
-      __syncthreads();
-      if (bm) iS = wg_shock_index(d, j, red);
+      wg_merge<ND, NQ>(d, j, b, i, &s_act);
+      if (d.circ) wg_split<ND, NQ>(d, j, b, ib > i ? ib - 1 : ib);
     }
+    if (bm) iS = wg_shock_index(d, j, red);
+    if (d.circ) wg_victims<ND, NQ>(d, j, rv, red, &is, &ib);
     ++i;
   }
 }
@@ -2149,6 +2271,7 @@ int gammahip_create(const gammahip_config* cfg, gammahip_ctx** out) {
   if (cfg->timestep != 0 && cfg->timestep != 1) return fail(c, "timestep must be 0 (RK3) or 1 (Euler)");
   if (cfg->recon != 0 && cfg->recon != 1) return fail(c, "recon must be 0 (piecewise linear) or 1 (piecewise constant)");
   if (cfg->eos != 0 && cfg->eos != 1) return fail(c, "eos must be 0 (ideal) or 1 (Synge)");
+  if (cfg->circ_regrid != 0 && cfg->circ_regrid != 1) return fail(c, "circ_regrid must be 0 or 1");
   if (cfg->eos == 1 && cfg->c2p_solver != 0)
     return fail(c, "the Synge equation of state runs with the Brent cons2prim (the Newton residual assumes a constant index)");
   if (cfg->c2p_guess != 0) return fail(c, "c2p_guess must be 0 at creation; gammahip_mlp_load switches the learned guess on");
@@ -2180,6 +2303,7 @@ int gammahip_create(const gammahip_config* cfg, gammahip_ctx** out) {
   d.shock = cfg->shock; d.sync = cfg->sync;
   d.solver = cfg->c2p_solver; d.guess = cfg->c2p_guess;
   d.euler = cfg->timestep == 1; d.pc = cfg->recon == 1; d.vpspec = cfg->vpspec != 0;
+  d.circ = cfg->circ_regrid != 0;
   d.mw = nullptr; d.mw2b = nullptr; d.mlp_thr = 1e-3;
   d.maxit = cfg->c2p_maxiter > 0 ? cfg->c2p_maxiter : 1000;
   d.rg_mode = 0;            // set from the tracks' rules by gammahip_load
@@ -2703,6 +2827,7 @@ int gammahip_stats_get(gammahip_ctx* c, gammahip_stats* st) {
   if (d2h(c, m.data(), c->d.nmerge, sizeof(long long) * c->d.nj)) return -1;
   for (int j = 0; j < c->d.nj; ++j) { st->splits += s[j]; st->merges += m[j]; }
   st->mlp_used = h[CNT_MLP];
+  st->regrid_clash = h[CNT_RGCLASH];
   return 0;
 }
 

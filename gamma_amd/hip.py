@@ -38,6 +38,7 @@ class Config(C.Structure):
         ("rg_mode", C.c_int), ("rg", C.c_double * 6),
         ("c2p_solver", C.c_int), ("c2p_guess", C.c_int), ("c2p_maxiter", C.c_int),
         ("timestep", C.c_int), ("recon", C.c_int), ("vpspec", C.c_int), ("eos", C.c_int),
+        ("circ_regrid", C.c_int),
     ]
 
 
@@ -51,8 +52,8 @@ class TrackHooks(C.Structure):
 class Stats(C.Structure):
     _fields_ = [(n, C.c_ulonglong) for n in (
         "c2p_calls", "c2p_iters", "c2p_nobracket", "c2p_maxiter", "c2p_phicap",
-        "negative_dt", "nan_flux", "f1_overflow", "splits", "merges", "mlp_used")] + [
-        ("reserved", C.c_ulonglong * 5)]
+        "negative_dt", "nan_flux", "f1_overflow", "splits", "merges", "mlp_used", "regrid_clash")] + [
+        ("reserved", C.c_ulonglong * 4)]
 
 
 _libs: dict = {}
@@ -124,6 +125,7 @@ def make_config(rc: S.RunConfig, ntracks_active: int, solver=C2P_BRENT, guess=0)
         c.rg[k] = rc.rg[k]
     c.c2p_solver, c.c2p_guess, c.c2p_maxiter = solver, guess, rc.c2p_maxiter
     c.timestep, c.recon, c.vpspec, c.eos = rc.timestep, rc.recon, rc.vpspec, rc.eos
+    c.circ_regrid = rc.circ_regrid
     return c
 
 

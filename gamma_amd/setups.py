@@ -81,6 +81,7 @@ class RunConfig:
     recon: int = RECON_PLM        # SPATIAL_RECONSTRUCTION_
     vpspec: int = 0               # VARIABLE_PSPEC_
     eos: int = EOS_IDEAL          # EOS_
+    circ_regrid: int = 0          # CIRC_REGRID_
 
     @property
     def nq(self) -> int:          # NUM_Q = NUM_C(4) + NUM_TR(2) + NUM_S

@@ -56,6 +56,12 @@ typedef struct gammahip_config {
                       /* (radiation_sph.cpp:18-35, 146-155); 0: the constant p_       */
   int    eos;         /* EOS_: 0 ideal (GAMMA_), 1 Synge (rel_sph.cpp:42-50, spherical */
                       /* module; Brent cons2prim only)                                */
+  int    circ_regrid; /* CIRC_REGRID_ (1d.cpp:532-551, 2d.cpp:658-681): each split     */
+                      /* pairs with a merge of the track's smallest cell, each merge  */
+                      /* with a split of its biggest (Grid::targetRegridVictims,      */
+                      /* 1d.cpp:498-520); a victim equal to the acting cell, the      */
+                      /* reference's exit(10), is counted in gammahip_stats.          */
+                      /* regrid_clash and ends that track's regrid                    */
 } gammahip_config;
 
 /* Per-track hook parameters (userKinematics / userBoundaries /
@@ -75,7 +81,8 @@ typedef struct gammahip_track_hooks {
 typedef struct gammahip_stats {
   unsigned long long c2p_calls, c2p_iters, c2p_nobracket, c2p_maxiter, c2p_phicap;
   unsigned long long negative_dt, nan_flux, f1_overflow, splits, merges, mlp_used;
-  unsigned long long reserved[5];
+  unsigned long long regrid_clash;   /* CIRC_REGRID_ victim equal to the acting cell */
+  unsigned long long reserved[4];
 } gammahip_stats;
 
 /* Grid::Grid + Grid::initialise (src/Dimensions/1d.cpp:53-81, 2d.cpp:59-96):

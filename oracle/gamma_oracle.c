@@ -1244,22 +1244,74 @@ static int regrid_check(og_grid *g, int j, int i, double *iS_cache, int *iS_vali
   return SKIP;
 }
 
-/* Grid::regrid + applyRegrid  1d.cpp:484-557, 2d.cpp:609-683 (CIRC_REGRID_ disabled) */
-static void grid_regrid(og_grid *g) {
-  if (g->c.rg_mode == OG_RG_NONE) return;
+/* Cell::regridVal with the setups' user_regridVal: the geometry's value
+ * (cartesian.cpp:58-63: dx; spherical1D.cpp:79-86: dr/r; spherical.cpp:115-123:
+ * dr/(r dth) with the cell's F1 width), replaced under the BM1D hooks by
+ * dr/(r dth) lfac^(3/2) with the setup's dth (BM1D.cpp:322-331); the 2DST
+ * hooks leave the geometry's value (2DST.cpp:171-178). */
+static double regrid_val(og_grid *g, int j, int i) {
+  const og_config *c = &g->c;
+  Cell *cl = CELL(g, j, i);
+  double r = cl->G.x[MV], dr = cl->G.dx[MV];
+  if (c->rg_mode == OG_RG_BM1D) return dr / (r * c->rg[2]) * og_pow(fs_lfac(c, &cl->S), 3. / 2.);
+  if (c->geom != OG_GEOM_SPH) return dr;
+  return c->nd == 1 ? dr / r : dr / (r * cl->G.dx[F1]);
+}
+
+/* Grid::targetRegridVictims  1d.cpp:498-520, 2d.cpp:625-646: the first cell
+ * of the smallest value (below 1e20) and the first of the largest (above 0)
+ * among cells iLbnd+3 .. iRbnd-3, negative values skipped.  -1 where no cell
+ * qualifies (the reference keeps its previous victim there; this restatement
+ * reports it as a clash instead). */
+static void target_victims(og_grid *g, int j, int *is, int *ib) {
+  double mn = 1.e20, mx = 0.;
+  *is = -1; *ib = -1;
+  for (int i = g->iLbnd[j] + 3; i <= g->iRbnd[j] - 3; ++i) {
+    double v = regrid_val(g, j, i);
+    if (v < 0) continue;
+    if (v < mn) { mn = v; *is = i; }
+    if (v > mx) { mx = v; *ib = i; }
+  }
+}
+
+/* Grid::regrid + applyRegrid  1d.cpp:484-557, 2d.cpp:609-683.  With
+ * circ_regrid (CIRC_REGRID_) a split of cell i first merges the smallest
+ * cell is (and splits i-1 if is < i), and a merge of cell i is followed by a
+ * split of the biggest cell ib (ib-1 if ib > i); a victim equal to i is the
+ * reference's exit(10), returned here as -10. */
+static int grid_regrid(og_grid *g) {
+  if (g->c.rg_mode == OG_RG_NONE) return 0;
+  int circ = g->c.circ_regrid;
   for (int j = 0; j < g->nj; ++j) {
     if (!active_track(g, j)) continue;
     double iS = 0; int iS_valid = 0;
+    int is = -1, ib = -1;
+    if (circ) target_victims(g, j, &is, &ib);
     for (int i = g->iLbnd[j] + 1; i <= g->iRbnd[j] - 1; ++i) {
       int a = regrid_check(g, j, i, &iS, &iS_valid);
+      int acted = 0;
       if (a == SPLIT && g->ntrack[j] < g->nax - 1) {
-        grid_split(g, j, i); g->nsplit[j]++; g->cnt[4]++; iS_valid = 0;
+        int isplit = i;
+        if (circ) {
+          if (is < 0 || is == i) return -10;
+          grid_merge(g, j, is); g->nmerge[j]++; g->cnt[5]++;
+          if (i > is) isplit--;
+        }
+        grid_split(g, j, isplit); g->nsplit[j]++; g->cnt[4]++; iS_valid = 0;
+        acted = 1;
       }
       if (a == MERGE && g->ntrack[j] > 2 * g->c.ngst + 2) {
+        if (circ && (ib < 0 || ib == i)) return -10;
         grid_merge(g, j, i); g->nmerge[j]++; g->cnt[5]++; iS_valid = 0;
+        if (circ) {
+          grid_split(g, j, ib > i ? ib - 1 : ib); g->nsplit[j]++; g->cnt[4]++;
+        }
+        acted = 1;
       }
+      if (acted && circ) target_victims(g, j, &is, &ib);
     }
   }
+  return 0;
 }
 
 /* ---------------- apply() sweeps over every allocated cell ---------------- */
@@ -1408,7 +1460,7 @@ int og_init(og_grid *g, double t0) {
 /* one iteration of Simu::run  simu.cpp:49-77 (dumps and hooks without effect omitted) */
 static int og_step(og_grid *g) {
   int e;
-  grid_regrid(g);
+  if ((e = grid_regrid(g))) return e;
   if (g->c.shock) grid_reset_shock(g);
   grid_update_ghosts(g);
   if ((e = grid_prep(g))) return e;

@@ -55,6 +55,9 @@ typedef struct {
   int    recon;       /* SPATIAL_RECONSTRUCTION_: 0 piecewise linear, 1 piecewise constant */
   int    vpspec;      /* VARIABLE_PSPEC_ (radiation_sph.cpp:18-35) */
   int    eos;         /* EOS_: 0 ideal, 1 Synge (rel_sph.cpp:42-50) */
+  int    circ_regrid; /* CIRC_REGRID_ (1d.cpp:532-551, 2d.cpp:658-681): every split
+                         paired with a merge of the track's smallest cell, every
+                         merge with a split of its biggest */
   double c2p_tol;     /* Brent interval tolerance, 1e-13 in the reference (rel_sph.cpp:281);
                          other values only measure how much the reference's own
                          tolerance moves its trajectories */

@@ -29,7 +29,7 @@ class OgConfig(C.Structure):
         ("bnd_mode", C.c_int), ("bm", C.c_double * 6),
         ("rg_mode", C.c_int), ("rg", C.c_double * 6),
         ("c2p_maxiter", C.c_int), ("timestep", C.c_int), ("recon", C.c_int), ("vpspec", C.c_int),
-        ("eos", C.c_int),
+        ("eos", C.c_int), ("circ_regrid", C.c_int),
         ("c2p_tol", C.c_double), ("c2p_form", C.c_int),
     ]
 
@@ -49,6 +49,7 @@ def make_config(rc, c2p_tol=1e-13, c2p_form=0) -> OgConfig:
     c.rg_mode = rc.rg_mode
     c.c2p_maxiter = rc.c2p_maxiter
     c.timestep, c.recon, c.vpspec, c.eos = rc.timestep, rc.recon, rc.vpspec, rc.eos
+    c.circ_regrid = rc.circ_regrid
     c.c2p_tol = c2p_tol
     c.c2p_form = c2p_form
     return c

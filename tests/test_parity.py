@@ -56,6 +56,10 @@ CASES = {
                                               eos=setups.EOS_SYNGE), 80),
     "bm_jet_2d_synge": (lambda: _variant(setups.bm_jet_2d(12, 0.3, 40, 160, gamma=5. / 3.),
                                          eos=setups.EOS_SYNGE), 40),
+    # CIRC_REGRID_ (1d.cpp:532-551, 2d.cpp:658-681): splits and merges in pairs
+    # with the track's smallest / biggest cell
+    "riemann_2d_circ": (lambda: _variant(setups.riemann_2d(64, 16), circ_regrid=1), 60),
+    "expanding_tube_circ": (lambda: _variant(setups.expanding_tube_1d(200), circ_regrid=1), 300),
 }
 
 
