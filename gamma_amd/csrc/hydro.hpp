@@ -197,10 +197,13 @@ struct C2PStat { int iters; int fail; int evals; int used; };
 // Brent's method (Brent 1973, ch. 4) with the GSL fsolver contract used by
 // rel_sph.cpp:273-285: root = best iterate b, bracket = [min(b,c), max(b,c)],
 // stop on |hi-lo| < 1e-13 * min(|lo|,|hi|) (0 if the bracket straddles 0).
+// flo, fhi: the residual at lo and hi, which the caller has already evaluated
+// while bracketing (the fsolver's set call evaluates them again; the residual
+// is a pure function, so the values are the same bits)
 GH_HD double brent(double D, double S, double E, double G, int geom, double lo, double hi,
-                   int maxit, C2PStat* st, bool synge = false) {
+                   double flo, double fhi, int maxit, C2PStat* st, bool synge = false) {
   double a = lo, b = hi;
-  double fa = c2p_f(lo, D, S, E, G, geom, synge), fb = c2p_f(hi, D, S, E, G, geom, synge);
+  double fa = flo, fb = fhi;
   double c = b, fc = fb, d = b - a, e = b - a;
   double root = 0.5 * (lo + hi), xl = lo, xu = hi;
   int it = 0;
@@ -433,19 +436,21 @@ GH_HD void cons2prim(double* prim, double* cons, double r, int geom, double G, i
       double p_lo = fmax(0., (1. + 1e-13) * S - E), p_hi;
       bool find_p = true;
       st->evals += 2;
-      if (c2p_f(p_lo, D, S, E, G, geom, synge) < 0) find_p = false;
-      if (f_init < 0) p_hi = 1. * pCand;
-      else {
-        p_hi = pCand;
-        int guard = 0;
-        while (c2p_f(p_hi, D, S, E, G, geom, synge) > 0) {
-          st->evals++;
-          p_hi *= 10;
-          if (++guard > 400) { find_p = false; st->fail |= 4; break; }
-        }
+      double f_lo = c2p_f(p_lo, D, S, E, G, geom, synge);
+      if (f_lo < 0) find_p = false;
+      // p_hi grows by decades from the previous pressure until f(p_hi) <= 0;
+      // f at the previous pressure is f_init
+      p_hi = pCand;
+      double f_hi = f_init;
+      int guard = 0;
+      while (f_hi > 0) {
+        p_hi *= 10;
+        if (++guard > 400) { find_p = false; st->fail |= 4; break; }
+        st->evals++;
+        f_hi = c2p_f(p_hi, D, S, E, G, geom, synge);
       }
       if (p_hi < p_lo) find_p = false;
-      if (find_p) pCand = brent(D, S, E, G, geom, p_lo, p_hi, maxit, st, synge);
+      if (find_p) pCand = brent(D, S, E, G, geom, p_lo, p_hi, f_lo, f_hi, maxit, st, synge);
       else {
         st->fail |= 1;
         pCand = pprev;
