@@ -735,13 +735,15 @@ struct NoGuess {
   GH_HD double operator()(bool, double, double, double, double, int) const { return -1.; }
 };
 
-template <int ND, int NQ, int SV, class GuessT>
+// GEO: the grid's geometry when fixed at compile time (2D grids have one;
+// -1: the track's, as 1D ensembles mix them)
+template <int ND, int NQ, int SV, class GuessT, int GEO = -1>
 GH_HD void body_update_g(const Dev& d, int j, int i, int stage, bool valid, const GuessT& guess) {
   bool run = valid && active_track(d, j);
   int nt = run ? d.ntrack[j] : 0;
   run = run && i < nt;
   int iL = d.ngst - 1, iR = nt - d.ngst;
-  int geom = run ? d.tgeom[j] : 0;
+  int geom = GEO >= 0 ? GEO : run ? d.tgeom[j] : 0;
   double G = d.P.gamma;
   double dt = run ? d.tdt[j] : 0.;
   double w0 = stage == 2 ? 3. / 4. : 1. / 3.;
@@ -779,7 +781,7 @@ GH_HD void body_update_g(const Dev& d, int j, int i, int stage, bool valid, cons
     xnL = d.fx[k - 1] + dt * d.fv[k - 1];
     xnR = d.fx[k] + dt * d.fv[k];
     load_v<ND, NQ>(d, d.prim, k, p);
-    Geo g = geo_of<ND>(d, j, k);
+    Geo g = geo_of_g<ND>(d, j, k, geom);
     prim2cons<ND, NQ>(p, U, sweep_r<ND>(d, k, g), geom, G, d.P.synge);
     if (stage == 1) {
       store_v<ND, NQ>(d, d.U0, k, U);
@@ -798,11 +800,11 @@ GH_HD void body_update_g(const Dev& d, int j, int i, int stage, bool valid, cons
       else for (int q = 0; q < NQ; ++q) f1[q] = d.f1net[q * d.cs + k];
       for (int q = 0; q < NQ; ++q) U[q] += f1[q] * dt;
     }
-    sources<ND, NQ>(p, U, d.cx[k], d.cdx[k], trig_at<ND>(d, j), g.dV, dt, geom, G, d.P.synge);
+    sources<ND, NQ>(p, U, d.cx[k], d.cdx[k], trig_at_g<ND>(d, j, geom), g.dV, dt, geom, G, d.P.synge);
     if (NQ == 10 && d.sync) radiative(p, U, g.dV, dt, d.P);
     cxn = (xnR + xnL) / 2.;
     cdxn = (xnR - xnL);
-    g2 = cell_geo<ND>(cxn, cdxn, d.ty[j], d.tdy[j], geom, trig_at<ND>(d, j));
+    g2 = cell_geo<ND>(cxn, cdxn, d.ty[j], d.tdy[j], geom, trig_at_g<ND>(d, j, geom));
     for (int q = 0; q < NQ; ++q) U[q] /= g2.dV;
     r1 = g2.cen0;
   }
@@ -824,7 +826,7 @@ GH_HD void body_update_g(const Dev& d, int j, int i, int stage, bool valid, cons
       double xaR = w0 * d.fx0[k] + w1 * xnR;
       cxn = (xaR + xaL) / 2.;
       cdxn = (xaR - xaL);
-      Geo g3 = cell_geo<ND>(cxn, cdxn, d.ty[j], d.tdy[j], geom, trig_at<ND>(d, j));
+      Geo g3 = cell_geo<ND>(cxn, cdxn, d.ty[j], d.tdy[j], geom, trig_at_g<ND>(d, j, geom));
       for (int q = 0; q < NQ; ++q) U[q] /= g3.dV;
     }
     double D = 1., S = 0., tau = 1.;
@@ -849,9 +851,9 @@ GH_HD void body_update_g(const Dev& d, int j, int i, int stage, bool valid, cons
 // cell.cpp:102-127, rk3.cpp:12-195).  One thread owns face i and cell i.
 // The solver is a template argument so that each kernel holds one root
 // finder only (smaller code, fewer instruction-cache misses).
-template <int ND, int NQ, int SV>
+template <int ND, int NQ, int SV, int GEO = -1>
 GH_HD void body_update_s(const Dev& d, int j, int i, int stage) {
-  body_update_g<ND, NQ, SV>(d, j, i, stage, true, NoGuess{});
+  body_update_g<ND, NQ, SV, NoGuess, GEO>(d, j, i, stage, true, NoGuess{});
 }
 template <int ND, int NQ>
 GH_HD void body_update(const Dev& d, int j, int i, int stage) {
@@ -1204,76 +1206,77 @@ __device__ __forceinline__ void tile_of(int remap, int& j, int& i) {
 // workgroups that the dispatcher places on XCD x (linear id % 8), so adjacent
 // tracks -- F1 neighbours -- share an L2.  Grid sizes are multiples of 8.
 // The trip count depends on blockIdx only, so every lane of a wavefront runs
-// the same iterations.
-// Packed form: a persistent grid walks the tile list built by k_tiles.  With
-// remap the list is cut into 8 contiguous bands, band x walked by the
-// workgroups that the dispatcher places on XCD x (linear id % 8), so adjacent
-// tracks -- F1 neighbours -- share an L2.  Grid sizes are multiples of 8.
-// The plain form is the same loop with one trip, so the body is emitted once.
-// The trip count depends on blockIdx only, so every lane of a wavefront runs
-// the same iterations.
-#define GH_TILE_LOOP(...)                                                        \
-  {                                                                              \
-    int t_ = 0, end_ = 1, st_ = 1;                                               \
-    if (d.packed) {                                                              \
-      int nt_ = *d.ntiles, G_ = gridDim.x;                                       \
-      if (remap) {                                                               \
-        int x_ = blockIdx.x & 7;                                                 \
-        t_ = (int)((long long)nt_ * x_ / 8) + (int)(blockIdx.x >> 3);            \
-        end_ = (int)((long long)nt_ * (x_ + 1) / 8);                             \
-        st_ = G_ >> 3;                                                           \
-      } else {                                                                   \
-        t_ = blockIdx.x; end_ = nt_; st_ = G_;                                   \
-      }                                                                          \
+// the same iterations.  PK (a template argument of every kernel built on this
+// loop) selects the form at compile time: the plain form (one workgroup per
+// tile) is then straight-line code holding none of the loop's state in
+// registers (in the update kernel those scalars spilled into vector lanes).
+#define GH_TILE_LOOP(PK, ...)                                                    \
+  if constexpr (!(PK)) {                                                         \
+    int i, j;                                                                    \
+    tile_of(remap, j, i);                                                        \
+    __VA_ARGS__;                                                                 \
+  } else {                                                                       \
+    int nt_ = *d.ntiles, G_ = gridDim.x;                                         \
+    int t_ = blockIdx.x, end_ = nt_, st_ = G_;                                   \
+    if (remap) {                                                                 \
+      int x_ = blockIdx.x & 7;                                                   \
+      t_ = (int)((long long)nt_ * x_ / 8) + (int)(blockIdx.x >> 3);              \
+      end_ = (int)((long long)nt_ * (x_ + 1) / 8);                               \
+      st_ = G_ >> 3;                                                             \
     }                                                                            \
     for (; t_ < end_; t_ += st_) {                                               \
-      int i, j;                                                                  \
-      if (d.packed) {                                                            \
-        j = d.tiles[2 * t_];                                                     \
-        i = d.tiles[2 * t_ + 1] * (int)blockDim.x + (int)threadIdx.x;            \
-      } else {                                                                   \
-        tile_of(remap, j, i);                                                    \
-      }                                                                          \
+      int j = d.tiles[2 * t_];                                                   \
+      int i = d.tiles[2 * t_ + 1] * (int)blockDim.x + (int)threadIdx.x;          \
       __VA_ARGS__;                                                               \
     }                                                                            \
   }
-#define GH_KERNEL2P(NAME, BODY)                                                 \
+// TPL: the kernel's other template parameters, each preceded by a comma, in
+// parentheses: () or (, int ND, int NQ)
+#define GH_UNPAREN(...) __VA_ARGS__
+#define GH_KERNEL2P(TPL, NAME, BODY)                                            \
+  template <bool PK GH_UNPAREN TPL>                                              \
   __global__ void __launch_bounds__(KB) NAME(Dev d, int ni, int remap) {        \
-    GH_TILE_LOOP(if (i < ni && j < d.nj) BODY(d, j, i))                          \
-  }
-#define GH_KERNEL2SP(NAME, BODY)                                                \
-  __global__ void __launch_bounds__(KB) NAME(Dev d, int ni, int remap, int stage) { \
-    GH_TILE_LOOP(if (i < ni && j < d.nj) BODY(d, j, i, stage))                   \
+    GH_TILE_LOOP(PK, if (i < ni && j < d.nj) BODY(d, j, i))                      \
   }
 namespace {
 GH_KERNEL2(k_ghost_tracks, body_ghost_tracks)
-GH_KERNEL2P(k_dt, body_dt)
-GH_KERNEL2P(k_nb, body_nb)
-GH_KERNEL2P(k_f1_shock, body_f1_shock)
+GH_KERNEL2P((), k_dt, body_dt)
+GH_KERNEL2P((), k_nb, body_nb)
+GH_KERNEL2P((), k_f1_shock, body_f1_shock)
 GH_KERNEL2(k_shock_reset, body_shock_reset)
 GH_KERNEL2(k_shock_detect, body_shock_detect)
 GH_KERNEL2(k_rg_init, body_rg_init)
-GH_KERNEL2P(k_rg_shockpos, body_rg_shockpos)
+GH_KERNEL2P((), k_rg_shockpos, body_rg_shockpos)
 GH_KERNEL2(k_advance_time, body_advance_time)
 template <int ND, int NQ> GH_KERNEL2(k_ghost_mv, (body_ghost_mv<ND, NQ>))
-template <int ND, int NQ, bool SHK> GH_KERNEL2P(k_faces, (body_faces_s<ND, NQ, SHK>))
+// MV face pass: built for 5 wavefronts per SIMD with six fields and no shock
+// detector (the body then fits 96 registers without spilling; left to itself
+// the compiler hoists loads into more and runs 4)
+template <bool PK, int ND, int NQ, bool SHK>
+__global__ void __launch_bounds__(KB, (SHK || NQ > 6) ? 1 : 5) k_faces(Dev d, int ni, int remap) {
+  GH_TILE_LOOP(PK, if (i < ni && j < d.nj) body_faces_s<ND, NQ, SHK>(d, j, i))
+}
 // cell pass with the gradient cache in LDS: thread t's entry m at
 // cache[m * blockDim + t] (consecutive lanes, consecutive banks); the launch
-// passes blockDim * f1_kc * NQ doubles of dynamic LDS
-template <int NQ, int GEO>
-__global__ void __launch_bounds__(KB) k_f1grad(Dev d, int ni, int remap) {
+// passes blockDim * f1_kc * NQ doubles of dynamic LDS.  The pass waits on
+// chains of dependent neighbour loads: scheduled for 3 wavefronts per SIMD it
+// keeps more of them in flight per wavefront, which beats a fourth wavefront
+// (0.336 against 0.363 ms on the 2048^2 grid)
+template <bool PK, int NQ, int GEO>
+__global__ void __launch_bounds__(KB) __attribute__((amdgpu_waves_per_eu(1, 3)))
+k_f1grad(Dev d, int ni, int remap) {
   extern __shared__ double f1cache[];
   double* cache = f1cache + threadIdx.x;
-  GH_TILE_LOOP(if (i < ni && j < d.nj) body_f1grad_c<NQ, GEO>(d, j, i, cache, (int)blockDim.x))
+  GH_TILE_LOOP(PK, if (i < ni && j < d.nj) body_f1grad_c<NQ, GEO>(d, j, i, cache, (int)blockDim.x))
 }
 // fused gradient + own-face pass (same LDS cache as k_f1grad)
-template <int NQ>
+template <bool PK, int NQ>
 __global__ void __launch_bounds__(KB) k_f1cell(Dev d, int ni, int remap) {
   extern __shared__ double f1cache[];
   double* cache = f1cache + threadIdx.x;
-  GH_TILE_LOOP(if (i < ni && j < d.nj) body_f1cell_c<NQ>(d, j, i, cache, (int)blockDim.x))
+  GH_TILE_LOOP(PK, if (i < ni && j < d.nj) body_f1cell_c<NQ>(d, j, i, cache, (int)blockDim.x))
 }
-template <int NQ> GH_KERNEL2P(k_f1sum, body_f1sum<NQ>)
+GH_KERNEL2P((, int NQ), k_f1sum, body_f1sum<NQ>)
 // face pass over the F1 slots of each track: thread (j, c*bs + t) of a plain
 // (chunk, track) grid walks slots c*bs + t + m*(chunks*bs)
 // W = minimum wavefronts per SIMD the compiler must allow (registers capped
@@ -1289,14 +1292,15 @@ __global__ void __launch_bounds__(KB, W) k_f1faces(Dev d, int remap) {
   for (int s = s0; s < tot; s += stride) body_f1face_s<NQ, SHK, GEO>(d, j, s);
 }
 // W: minimum wavefronts per SIMD (1 = the compiler's choice; 5 caps the
-// registers at 96 and spills the rest), picked by the launch option
-template <int ND, int NQ, int SV, int W>
+// registers at 96 and spills the rest), picked by the launch option; GEO as
+// body_update_g (-1 in 1D)
+template <bool PK, int ND, int NQ, int SV, int W, int GEO>
 __global__ void __launch_bounds__(KB, W) k_update(Dev d, int ni, int remap, int stage) {
-  GH_TILE_LOOP(if (i < ni && j < d.nj) body_update_s<ND, NQ, SV>(d, j, i, stage))
+  GH_TILE_LOOP(PK, if (i < ni && j < d.nj) body_update_s<ND, NQ, SV, GEO>(d, j, i, stage))
 }
-template <int ND, int NQ> GH_KERNEL2P(k_inject, (body_inject<ND, NQ>))
+GH_KERNEL2P((, int ND, int NQ), k_inject, (body_inject<ND, NQ>))
 template <int ND, int NQ> GH_KERNEL2(k_init_gammas, (body_init_gammas<ND, NQ>))
-template <int ND, int NQ> GH_KERNEL2P(k_rg_screen, (body_rg_screen<ND, NQ>))
+GH_KERNEL2P((, int ND, int NQ), k_rg_screen, (body_rg_screen<ND, NQ>))
 
 // Tile list of the packed launch: ceil(ntrack / bs) chunks of every track, in
 // track order (one workgroup; block-wide scan of the per-thread counts).
@@ -1802,13 +1806,13 @@ struct MlpGuess {
   }
 };
 
-template <int ND, int NQ>
+template <bool PK, int ND, int NQ>
 __global__ void __launch_bounds__(KB) k_update_mlp(Dev d, int ni, int remap, int stage) {
   __shared__ __bf16 lds[KB / 64][64 * H];
   __shared__ float rows[KB / 64][64];
   int wv = threadIdx.x / 64;
   MlpGuess g{d.mw, (const bf16x8*)d.mw2b, lds[wv], rows[wv], d.mlp_thr, d.P.gamma};
-  GH_TILE_LOOP(body_update_g<ND, NQ, C2P_NEWTON>(d, j, i, stage, i < ni, g))
+  GH_TILE_LOOP(PK, body_update_g<ND, NQ, C2P_NEWTON>(d, j, i, stage, i < ni, g))
 }
 
 // x^y as the kernels evaluate it (gh_pow), for numerics checks
@@ -1939,8 +1943,8 @@ int d2h(gammahip_ctx* c, void* dst, const void* src, size_t bytes) {
 #define L2S(c, body, ntr, ni, st)                              \
   for (int j_ = 0; j_ < (ntr); ++j_)                           \
     for (int i_ = 0; i_ < (ni); ++i_) body((c)->d, j_, i_, st)
-#define L2P L2
-#define L2SP L2S
+#define L2P(c, body, kern, targs, ntr, ni) L2(c, body, ntr, ni)
+#define L2SP(c, body, kern, targs, ntr, ni, st) L2S(c, body, ntr, ni, st)
 #else
 #define L2(c, kern, ntr, ni)                                                         \
   do {                                                                               \
@@ -1954,18 +1958,28 @@ int d2h(gammahip_ctx* c, void* dst, const void* src, size_t bytes) {
     dim3 g_(((ni) + bs_ - 1) / bs_, (ntr));                                          \
     hipLaunchKernelGGL(kern, g_, dim3(bs_), 0, (c)->stream, (c)->d, (int)(ni), (c)->remap, (int)(st)); \
   } while (0)
-// kernels with a packed form: a persistent grid of c->persist workgroups per CU
-#define L2P(c, kern, ntr, ni)                                                        \
+// kernels with a packed form: kern<PK, targs...> (targs: its other template
+// arguments, each after a comma, in parentheses), the packed build on a
+// persistent grid of c->persist workgroups per CU, the plain one per tile
+#define L2P(c, body, kern, targs, ntr, ni)                                           \
   do {                                                                               \
     int bs_ = (c)->block;                                                            \
-    dim3 g_ = (c)->d.packed ? dim3((c)->persist * (c)->ncu) : dim3(((ni) + bs_ - 1) / bs_, (ntr)); \
-    hipLaunchKernelGGL(kern, g_, dim3(bs_), 0, (c)->stream, (c)->d, (int)(ni), (c)->remap); \
+    if ((c)->d.packed)                                                               \
+      hipLaunchKernelGGL((kern<true GH_UNPAREN targs>), dim3((c)->persist * (c)->ncu), dim3(bs_), 0, \
+                         (c)->stream, (c)->d, (int)(ni), (c)->remap);                \
+    else                                                                             \
+      hipLaunchKernelGGL((kern<false GH_UNPAREN targs>), dim3(((ni) + bs_ - 1) / bs_, (ntr)), dim3(bs_), 0, \
+                         (c)->stream, (c)->d, (int)(ni), (c)->remap);                \
   } while (0)
-#define L2SP(c, kern, ntr, ni, st)                                                   \
+#define L2SP(c, body, kern, targs, ntr, ni, st)                                      \
   do {                                                                               \
     int bs_ = (c)->block;                                                            \
-    dim3 g_ = (c)->d.packed ? dim3((c)->persist * (c)->ncu) : dim3(((ni) + bs_ - 1) / bs_, (ntr)); \
-    hipLaunchKernelGGL(kern, g_, dim3(bs_), 0, (c)->stream, (c)->d, (int)(ni), (c)->remap, (int)(st)); \
+    if ((c)->d.packed)                                                               \
+      hipLaunchKernelGGL((kern<true GH_UNPAREN targs>), dim3((c)->persist * (c)->ncu), dim3(bs_), 0, \
+                         (c)->stream, (c)->d, (int)(ni), (c)->remap, (int)(st));     \
+    else                                                                             \
+      hipLaunchKernelGGL((kern<false GH_UNPAREN targs>), dim3(((ni) + bs_ - 1) / bs_, (ntr)), dim3(bs_), 0, \
+                         (c)->stream, (c)->d, (int)(ni), (c)->remap, (int)(st));     \
   } while (0)
 #endif
 
@@ -2032,16 +2046,12 @@ struct Pipe {
   static void prep(gammahip_ctx* c) {
     Dev& d = c->d;
     tbeg(c, 1);
-#ifdef GH_EMU
-    L2P(c, (body_faces<ND, NQ>), d.nj, d.nax);
-#else
-    if (d.shock) L2P(c, (k_faces<ND, NQ, true>), d.nj, d.nax);
-    else L2P(c, (k_faces<ND, NQ, false>), d.nj, d.nax);
-#endif
+    if (d.shock) L2P(c, (body_faces<ND, NQ>), k_faces, (, ND, NQ, true), d.nj, d.nax);
+    else L2P(c, (body_faces<ND, NQ>), k_faces, (, ND, NQ, false), d.nj, d.nax);
     tend(c, 1);
-    L2P(c, KSEL(body_dt, k_dt), d.nj, d.nax);
+    L2P(c, body_dt, k_dt, (), d.nj, d.nax);
     if (ND == 2) {
-      L2P(c, KSEL(body_nb, k_nb), d.nj, d.nax);
+      L2P(c, body_nb, k_nb, (), d.nj, d.nax);
 #ifdef GH_EMU
       for (int j = 0; j < d.nj; ++j) {
         int base = 0, nt = d.ntrack[j];
@@ -2072,7 +2082,8 @@ struct Pipe {
         int bs = c->block;
         dim3 g = d.packed ? dim3(c->persist * c->ncu) : dim3((d.nax + bs - 1) / bs, d.nj);
         size_t lds = sizeof(double) * bs * f1_kc<NQ>() * NQ;
-        hipLaunchKernelGGL(k_f1cell<NQ>, g, dim3(bs), lds, c->stream, d, d.nax, c->remap);
+        if (d.packed) hipLaunchKernelGGL((k_f1cell<true, NQ>), g, dim3(bs), lds, c->stream, d, d.nax, c->remap);
+        else hipLaunchKernelGGL((k_f1cell<false, NQ>), g, dim3(bs), lds, c->stream, d, d.nax, c->remap);
 #endif
         tend(c, 5);
       } else {
@@ -2086,8 +2097,14 @@ struct Pipe {
           int bs = c->block;
           dim3 g = d.packed ? dim3(c->persist * c->ncu) : dim3((d.nax + bs - 1) / bs, d.nj);
           size_t lds = sizeof(double) * bs * f1_kc<NQ>() * NQ;
-          if (c->cfg.geom == SPH) hipLaunchKernelGGL((k_f1grad<NQ, SPH>), g, dim3(bs), lds, c->stream, d, d.nax, c->remap);
-          else hipLaunchKernelGGL((k_f1grad<NQ, CART>), g, dim3(bs), lds, c->stream, d, d.nax, c->remap);
+          bool sph = c->cfg.geom == SPH;
+          if (d.packed) {
+            if (sph) hipLaunchKernelGGL((k_f1grad<true, NQ, SPH>), g, dim3(bs), lds, c->stream, d, d.nax, c->remap);
+            else hipLaunchKernelGGL((k_f1grad<true, NQ, CART>), g, dim3(bs), lds, c->stream, d, d.nax, c->remap);
+          } else {
+            if (sph) hipLaunchKernelGGL((k_f1grad<false, NQ, SPH>), g, dim3(bs), lds, c->stream, d, d.nax, c->remap);
+            else hipLaunchKernelGGL((k_f1grad<false, NQ, CART>), g, dim3(bs), lds, c->stream, d, d.nax, c->remap);
+          }
         }
 #endif
         tend(c, 3);
@@ -2113,10 +2130,10 @@ struct Pipe {
 #endif
         tend(c, 2);
       }
-      if (d.shock) L2P(c, KSEL(body_f1_shock, k_f1_shock), d.nj, d.nax);
+      if (d.shock) L2P(c, body_f1_shock, k_f1_shock, (), d.nj, d.nax);
       if (!d.f1fused) {
         tbeg(c, 4);
-        L2P(c, KSEL(body_f1sum<NQ>, k_f1sum<NQ>), d.nj, d.nax);
+        L2P(c, body_f1sum<NQ>, k_f1sum, (, NQ), d.nj, d.nax);
         tend(c, 4);
       }
       std::swap(d.gf1, d.gf1n);
@@ -2148,13 +2165,26 @@ struct Pipe {
 #ifdef GH_EMU
     L2S(c, (body_update<ND, NQ>), d.nj, d.nax, stage);
 #else
-    if (d.mw) L2SP(c, (k_update_mlp<ND, NQ>), d.nj, d.nax, stage);
-    else if (d.solver == C2P_NEWTON) {
-      if (c->occ & 2) L2SP(c, (k_update<ND, NQ, C2P_NEWTON, 5>), d.nj, d.nax, stage);
-      else L2SP(c, (k_update<ND, NQ, C2P_NEWTON, 1>), d.nj, d.nax, stage);
-    } else {
-      if (c->occ & 2) L2SP(c, (k_update<ND, NQ, C2P_BRENT, 5>), d.nj, d.nax, stage);
-      else L2SP(c, (k_update<ND, NQ, C2P_BRENT, 1>), d.nj, d.nax, stage);
+    if (d.mw) L2SP(c, _, k_update_mlp, (, ND, NQ), d.nj, d.nax, stage);
+    else {
+      // 2D grids have one geometry: the kernel is built for it (1D ensembles
+      // mix geometries per track)
+      bool cap = (c->occ & 2) != 0;
+      auto upd = [&](auto sv) {
+        constexpr int SV = decltype(sv)::value;
+        if constexpr (ND == 1) {
+          if (cap) L2SP(c, _, k_update, (, ND, NQ, SV, 5, -1), d.nj, d.nax, stage);
+          else L2SP(c, _, k_update, (, ND, NQ, SV, 1, -1), d.nj, d.nax, stage);
+        } else if (c->cfg.geom == SPH) {
+          if (cap) L2SP(c, _, k_update, (, ND, NQ, SV, 5, SPH), d.nj, d.nax, stage);
+          else L2SP(c, _, k_update, (, ND, NQ, SV, 1, SPH), d.nj, d.nax, stage);
+        } else {
+          if (cap) L2SP(c, _, k_update, (, ND, NQ, SV, 5, CART), d.nj, d.nax, stage);
+          else L2SP(c, _, k_update, (, ND, NQ, SV, 1, CART), d.nj, d.nax, stage);
+        }
+      };
+      if (d.solver == C2P_NEWTON) upd(std::integral_constant<int, C2P_NEWTON>{});
+      else upd(std::integral_constant<int, C2P_BRENT>{});
     }
 #endif
     tend(c, 0);
@@ -2164,8 +2194,8 @@ struct Pipe {
     Dev& d = c->d;
     if (!d.rg_mode) return;
     L2(c, KSEL(body_rg_init, k_rg_init), d.nj, 1);
-    if (d.rg_mode & (1 << RG_BM1D)) L2P(c, KSEL(body_rg_shockpos, k_rg_shockpos), d.nj, d.nax);
-    L2P(c, KSEL((body_rg_screen<ND, NQ>), (k_rg_screen<ND, NQ>)), d.nj, d.nax);
+    if (d.rg_mode & (1 << RG_BM1D)) L2P(c, body_rg_shockpos, k_rg_shockpos, (), d.nj, d.nax);
+    L2P(c, (body_rg_screen<ND, NQ>), k_rg_screen, (, ND, NQ), d.nj, d.nax);
 #ifdef GH_EMU
     for (int j = 0; j < d.nj; ++j) body_rg_apply<ND, NQ>(d, j, 0);
 #else
@@ -2177,7 +2207,7 @@ struct Pipe {
     switch (what) {
       case GAMMAHIP_RESET_SHOCK: if (d.shock) L2(c, KSEL(body_shock_reset, k_shock_reset), d.nj, d.nax); break;
       case GAMMAHIP_DETECT_SHOCK: if (d.shock) L2(c, KSEL(body_shock_detect, k_shock_detect), d.nj, d.nax); break;
-      case GAMMAHIP_INJECT: if (NQ == 10 && d.sync) L2P(c, KSEL((body_inject<ND, NQ>), (k_inject<ND, NQ>)), d.nj, d.nax); break;
+      case GAMMAHIP_INJECT: if (NQ == 10 && d.sync) L2P(c, (body_inject<ND, NQ>), k_inject, (, ND, NQ), d.nj, d.nax); break;
       case GAMMAHIP_INIT_GAMMAS: if (NQ == 10 && d.sync) L2(c, KSEL((body_init_gammas<ND, NQ>), (k_init_gammas<ND, NQ>)), d.nj, d.nax); break;
       case GAMMAHIP_ADVANCE_TIME: L2(c, KSEL(body_advance_time, k_advance_time), d.nj, 1); break;
       default: break;
