@@ -1241,7 +1241,51 @@ __device__ __forceinline__ void tile_of(int remap, int& j, int& i) {
 namespace {
 GH_KERNEL2(k_ghost_tracks, body_ghost_tracks)
 GH_KERNEL2P((), k_dt, body_dt)
-GH_KERNEL2P((), k_nb, body_nb)
+// body_nb on a wavefront of 64 consecutive cells of one track: the lower
+// bound of the previous cell's right face (first search of nb_side) is the
+// previous lane's own bound of its right face (same face, same track, and the
+// search returns the bound whatever its starting hint), so each lane searches
+// once per side and takes the other from its neighbour; lane 0 searches both.
+// Every lane runs the shuffles (valid = false for lanes without a tile cell).
+__device__ void body_nb_lanes(const Dev& d, int j, int i, bool valid) {
+  int nt = valid ? d.ntrack[j] : 0;
+  bool cell = valid && i >= 1 && i <= nt - 2;
+  long long k = cell ? CI(d, j, i) : 0;
+  double xjL = 0., xjR = 0.;
+  if (cell) { xjL = d.cx[k] - d.cdx[k] / 2.; xjR = d.cx[k] + d.cdx[k] / 2.; }
+  int lane = (int)(threadIdx.x & 63);
+  int lo2[2] = {0, 0}, cnt2[2] = {0, 0};
+  for (int side = 0; side < 2; ++side) {
+    int ja = side == 0 ? j - 1 : j + 1;
+    bool has = cell && (side == 0 ? j != 0 : j != d.nj - 1);
+    int L = has ? lb_face(d, ja, xjR, i) : 0;
+    int lp = __shfl(L, lane == 0 ? 0 : lane - 1, 64);
+    if (has) {
+      int nta = d.ntrack[ja];
+      int fin = 0;
+      if (i > 1) {
+        if (lane == 0) lp = lb_face(d, ja, d.cx[k - 1] + d.cdx[k - 1] / 2., i - 1);
+        fin = lp < nta - 2 ? lp : nta - 2;
+      }
+      int s0 = fin;
+      if (d.fx[CI(d, ja, s0)] > xjL && s0 > 0) s0--;
+      if (L < s0) L = s0;
+      lo2[side] = s0 + 1;
+      cnt2[side] = L - s0;
+    }
+  }
+  if (!cell) return;
+  d.nbdj[k] = j - 1; d.nbdlo[k] = lo2[0]; d.nbdcnt[k] = cnt2[0];
+  d.nbuj[k] = j + 1; d.nbulo[k] = lo2[1]; d.nbucnt[k] = cnt2[1];
+  for (int t = lo2[1]; t < lo2[1] + cnt2[1]; ++t) {
+    GH_ATOMIC_MIN(&d.incf[CI(d, j + 1, t)], i);
+    GH_ATOMIC_ADD(&d.dncnt[CI(d, j + 1, t)], 1);
+  }
+}
+template <bool PK>
+__global__ void __launch_bounds__(KB) k_nb(Dev d, int ni, int remap) {
+  GH_TILE_LOOP(PK, body_nb_lanes(d, j, i, i < ni && j < d.nj))
+}
 GH_KERNEL2P((), k_f1_shock, body_f1_shock)
 GH_KERNEL2(k_shock_reset, body_shock_reset)
 GH_KERNEL2(k_shock_detect, body_shock_detect)
@@ -1363,53 +1407,59 @@ __global__ void __launch_bounds__(KB) k_dt_finish(Dev d) {
   for (int j = threadIdx.x; j < d.nj; j += KB) d.tdt[j] = d.P.cfl * (d.nd == 2 ? sm[0] : d.tmin[j]);
 }
 // exclusive prefix sum of upper-neighbour counts along each track (F1 face slots)
-__global__ void __launch_bounds__(KB) k_f1_scan(Dev d) {
-  __shared__ int sm[KB];
-  int j = blockIdx.x;
-  int nt = d.ntrack[j];
-  int base = 0;
-  for (int i0 = 0; i0 < nt; i0 += KB) {
-    int i = i0 + threadIdx.x;
-    int v = i < nt ? d.nbucnt[CI(d, j, i)] : 0;
-    if (v < 0) v = 0;
-    sm[threadIdx.x] = v;
-    __syncthreads();
-    for (int s = 1; s < KB; s <<= 1) {
-      int t = (int)threadIdx.x >= s ? sm[threadIdx.x - s] : 0;
-      __syncthreads();
-      sm[threadIdx.x] += t;
-      __syncthreads();
-    }
-    if (i < nt) {
-      int off = base + sm[threadIdx.x] - v;
-      d.f1off[CI(d, j, i)] = off;
-      for (int n = 0; n < v && off + n < d.fcap; ++n) d.f1own[(long long)j * d.fcap + off + n] = i;
-    }
-    int tot = sm[KB - 1];
-    __syncthreads();
-    base += tot;
+// inclusive prefix sum of one value per thread over the workgroup: shuffles
+// within each wavefront, the wavefront totals through LDS; *total = the sum
+// over the workgroup
+__device__ int wg_scan_incl(int v, int* wsum, int* total) {
+  int lane = (int)(threadIdx.x & 63), w = (int)(threadIdx.x >> 6);
+  for (int o = 1; o < 64; o <<= 1) {
+    int t = __shfl_up(v, o, 64);
+    if (lane >= o) v += t;
   }
-  if (threadIdx.x == 0) {
-    d.f1tot[j] = base;
-    if (base > d.fcap) atomicAdd(cnt_at(d, CNT_F1OVF), 1ull);
+  if (lane == 63) wsum[w] = v;
+  __syncthreads();
+  int add = 0, tot = 0;
+  for (int q = 0; q < KB / 64; ++q) {
+    if (q < w) add += wsum[q];
+    tot += wsum[q];
   }
   __syncthreads();
-  base = 0;                    // down slots: the same scan over the fresh-source counts
-  for (int i0 = 0; i0 < nt; i0 += KB) {
-    int i = i0 + threadIdx.x;
-    int v = i < nt ? d.dncnt[CI(d, j, i)] : 0;
-    sm[threadIdx.x] = v;
-    __syncthreads();
-    for (int s = 1; s < KB; s <<= 1) {
-      int t = (int)threadIdx.x >= s ? sm[threadIdx.x - s] : 0;
-      __syncthreads();
-      sm[threadIdx.x] += t;
-      __syncthreads();
+  *total = tot;
+  return v + add;
+}
+// Per track: exclusive scans of the F1 face counts (slot offsets, owner of
+// each slot, the track's total) and of the fresh-source counts (down-slot
+// offsets), KB cells at a time; the next chunk's counts are loaded while the
+// current chunk is scanned.
+__global__ void __launch_bounds__(KB) k_f1_scan(Dev d) {
+  __shared__ int wsum[KB / 64];
+  int j = blockIdx.x;
+  int nt = d.ntrack[j];
+  for (int pass = 0; pass < 2; ++pass) {
+    const int* cnt = pass == 0 ? d.nbucnt : d.dncnt;
+    int base = 0;
+    int vnext = (int)threadIdx.x < nt ? cnt[CI(d, j, threadIdx.x)] : 0;
+    for (int i0 = 0; i0 < nt; i0 += KB) {
+      int i = i0 + (int)threadIdx.x;
+      int v = vnext;
+      if (pass == 0 && v < 0) v = 0;
+      vnext = i + KB < nt ? cnt[CI(d, j, i + KB)] : 0;
+      int tot;
+      int off = base + wg_scan_incl(v, wsum, &tot) - v;
+      if (i < nt) {
+        if (pass == 0) {
+          d.f1off[CI(d, j, i)] = off;
+          for (int n = 0; n < v && off + n < d.fcap; ++n) d.f1own[(long long)j * d.fcap + off + n] = i;
+        } else {
+          d.dnoff[CI(d, j, i)] = off;
+        }
+      }
+      base += tot;
     }
-    if (i < nt) d.dnoff[CI(d, j, i)] = base + sm[threadIdx.x] - v;
-    int tot = sm[KB - 1];
-    __syncthreads();
-    base += tot;
+    if (pass == 0 && threadIdx.x == 0) {
+      d.f1tot[j] = base;
+      if (base > d.fcap) atomicAdd(cnt_at(d, CNT_F1OVF), 1ull);
+    }
   }
 }
 // The applyRegrid loop of one track on one workgroup, in the sequential
