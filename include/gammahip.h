@@ -169,8 +169,9 @@ int  gammahip_sync(gammahip_ctx* ctx);
  * tiles that covers only each track's live cells (uneven tracks, ensembles),
  * 0 launches one workgroup per chunk of the nmax-wide track slots; OCC bit 0
  * runs the F1 face kernel in a build capped at 128 registers (4 wavefronts
- * per SIMD), bit 1 the update kernel in one capped at 96 (5 wavefronts),
- * spilling what does not fit; F1FUSE 1 sums each cell's F1 face fluxes inside
+ * per SIMD) with six fields, at 168 (3) with ten, bit 1 the update kernel in
+ * one capped at 96 (5 wavefronts; the default build of 2D Cartesian six-field
+ * grids), spilling what does not fit; F1FUSE 1 sums each cell's F1 face fluxes inside
  * the update kernel instead of in a pass of its own (fewer bytes, a longer
  * dependent-load chain in the update); SOLVER switches cons2prim between
  * Brent (0, the reference's algorithm) and Newton (1; required while the
