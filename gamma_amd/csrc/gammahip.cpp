@@ -1175,7 +1175,23 @@ GH_HD void body_advance_time(const Dev& d, int j, int i) {
 // On the uniform 2048^2 grid this cuts the transverse-gradient pass's HBM
 // traffic from 1.87 to 0.81 GB per launch; on tracks of uneven length the
 // bands are uneven work, so it is a tuned option, off by default.
+// remap 2 (track groups; grid of ceil(ni/64) x ceil(ntr/W) workgroups of W
+// wavefronts, plain_launch): workgroup (x, y) takes the 64-cell chunk x of
+// tracks yW .. yW+W-1, one per wavefront, so the F1 neighbours of its inner
+// tracks are read by the workgroup itself, on one CU, at the same time.
+// Track indices past the launch's last track fail the kernels' j < nj test
+// (launches over fewer tracks than the grid holds use this order only when
+// their count is a multiple of W).  j is the same on every lane of a
+// wavefront; readfirstlane says so to the compiler, which otherwise treats it
+// as varying per lane: per-track values would then be loaded per lane and
+// same-address atomics (the shock-index maximum) issued per lane instead of
+// once per wavefront (0.025 -> 0.7 ms per launch on the 1D ensemble).
 __device__ __forceinline__ void tile_of(int remap, int& j, int& i) {
+  if (remap == 2) {
+    j = __builtin_amdgcn_readfirstlane((int)blockIdx.y * (int)(blockDim.x >> 6) + (int)(threadIdx.x >> 6));
+    i = (int)blockIdx.x * 64 + (int)(threadIdx.x & 63);
+    return;
+  }
   int gx = gridDim.x, gy = gridDim.y;
   int L = blockIdx.x + gx * blockIdx.y;
   if (remap) {
@@ -1331,7 +1347,8 @@ template <int NQ, int W, bool SHK, int GEO>
 __global__ void __launch_bounds__(KB, W) k_f1faces(Dev d, int remap) {
   int j, s0;
   tile_of(remap, j, s0);
-  int stride = gridDim.x * blockDim.x;
+  if (j >= d.nj) return;
+  int stride = remap == 2 ? (int)gridDim.x * 64 : (int)(gridDim.x * blockDim.x);
   int tot = min(d.f1tot[j], d.fcap);
   for (int s = s0; s < tot; s += stride) body_f1face_s<NQ, SHK, GEO>(d, j, s);
 }
@@ -1866,7 +1883,7 @@ __global__ void __launch_bounds__(KB) k_update_mlp(Dev d, int ni, int remap, int
   __shared__ float rows[KB / 64][64];
   int wv = threadIdx.x / 64;
   MlpGuess g{d.mw, (const bf16x8*)d.mw2b, lds[wv], rows[wv], d.mlp_thr, d.P.gamma};
-  GH_TILE_LOOP(PK, body_update_g<ND, NQ, C2P_NEWTON>(d, j, i, stage, i < ni, g))
+  GH_TILE_LOOP(PK, body_update_g<ND, NQ, C2P_NEWTON>(d, j, i, stage, i < ni && j < d.nj, g))
 }
 
 // x^y as the kernels evaluate it (gh_pow), for numerics checks
@@ -1915,11 +1932,13 @@ struct gammahip_ctx {
   bool timing = false;
   KTimer kt[16];
   // launch options (gammahip_set_option; the autotuner picks them)
-  // defaults: 64-thread workgroups, the fastest plain launch measured on the
-  // 2D grids; hardware round-robin order (the XCD bands help the uniform grid
-  // by a few percent and cost the uneven ones up to 2x)
-  int block = 64;          // threads per workgroup of the per-cell kernels (64..256)
-  int remap = 0;           // XCD-aware tile order
+  // defaults: 128-thread workgroups in track groups (two tracks per
+  // workgroup), the fastest plain launch measured on the 2D grids (2048^2:
+  // 4.99 against 5.07 ms per step for 64 threads in hardware order; 2D jet:
+  // 2 % faster); the XCD bands help the uniform grid a little more but cost
+  // the uneven ones up to 2x
+  int block = 128;         // threads per workgroup of the per-cell kernels (64..256)
+  int remap = 2;           // tile order: 0 hardware, 1 XCD bands, 2 track groups
   int use_graph = 0;       // replay steps from a captured hipGraph
   int persist = 0;         // packed launch: persistent workgroups per CU (0: off)
   int occ = 0;             // bit 0: F1 face kernel built for 4 wavefronts per SIMD; bit 1: update kernel for 5
@@ -2000,17 +2019,27 @@ int d2h(gammahip_ctx* c, void* dst, const void* src, size_t bytes) {
 #define L2P(c, body, kern, targs, ntr, ni) L2(c, body, ntr, ni)
 #define L2SP(c, body, kern, targs, ntr, ni, st) L2S(c, body, ntr, ni, st)
 #else
+// grid and tile order of a plain per-cell launch over ntr tracks x ni cells:
+// one workgroup per chunk of the block size, or with the track-group order
+// (remap 2) one per 64-cell chunk of block/64 tracks; a launch over fewer
+// tracks than the grid that is not a whole number of groups keeps the
+// hardware order
+struct PlainLaunch { dim3 g; int remap; };
+PlainLaunch plain_launch(const gammahip_ctx* c, int ntr, int ni) {
+  int bs = c->block, W = bs / 64;
+  if (c->remap == 2 && (ntr == c->d.nj || ntr % W == 0))
+    return {dim3((ni + 63) / 64, (ntr + W - 1) / W), 2};
+  return {dim3((ni + bs - 1) / bs, ntr), c->remap == 2 ? 0 : c->remap};
+}
 #define L2(c, kern, ntr, ni)                                                         \
   do {                                                                               \
-    int bs_ = (c)->block;                                                            \
-    dim3 g_(((ni) + bs_ - 1) / bs_, (ntr));                                          \
-    hipLaunchKernelGGL(kern, g_, dim3(bs_), 0, (c)->stream, (c)->d, (int)(ni), (c)->remap); \
+    PlainLaunch pl_ = plain_launch((c), (ntr), (ni));                                \
+    hipLaunchKernelGGL(kern, pl_.g, dim3((c)->block), 0, (c)->stream, (c)->d, (int)(ni), pl_.remap); \
   } while (0)
 #define L2S(c, kern, ntr, ni, st)                                                    \
   do {                                                                               \
-    int bs_ = (c)->block;                                                            \
-    dim3 g_(((ni) + bs_ - 1) / bs_, (ntr));                                          \
-    hipLaunchKernelGGL(kern, g_, dim3(bs_), 0, (c)->stream, (c)->d, (int)(ni), (c)->remap, (int)(st)); \
+    PlainLaunch pl_ = plain_launch((c), (ntr), (ni));                                \
+    hipLaunchKernelGGL(kern, pl_.g, dim3((c)->block), 0, (c)->stream, (c)->d, (int)(ni), pl_.remap, (int)(st)); \
   } while (0)
 // kernels with a packed form: kern<PK, targs...> (targs: its other template
 // arguments, each after a comma, in parentheses), the packed build on a
@@ -2018,22 +2047,26 @@ int d2h(gammahip_ctx* c, void* dst, const void* src, size_t bytes) {
 #define L2P(c, body, kern, targs, ntr, ni)                                           \
   do {                                                                               \
     int bs_ = (c)->block;                                                            \
-    if ((c)->d.packed)                                                               \
+    if ((c)->d.packed) {                                                             \
       hipLaunchKernelGGL((kern<true GH_UNPAREN targs>), dim3((c)->persist * (c)->ncu), dim3(bs_), 0, \
                          (c)->stream, (c)->d, (int)(ni), (c)->remap);                \
-    else                                                                             \
-      hipLaunchKernelGGL((kern<false GH_UNPAREN targs>), dim3(((ni) + bs_ - 1) / bs_, (ntr)), dim3(bs_), 0, \
-                         (c)->stream, (c)->d, (int)(ni), (c)->remap);                \
+    } else {                                                                         \
+      PlainLaunch pl_ = plain_launch((c), (ntr), (ni));                              \
+      hipLaunchKernelGGL((kern<false GH_UNPAREN targs>), pl_.g, dim3(bs_), 0,        \
+                         (c)->stream, (c)->d, (int)(ni), pl_.remap);                 \
+    }                                                                                \
   } while (0)
 #define L2SP(c, body, kern, targs, ntr, ni, st)                                      \
   do {                                                                               \
     int bs_ = (c)->block;                                                            \
-    if ((c)->d.packed)                                                               \
+    if ((c)->d.packed) {                                                             \
       hipLaunchKernelGGL((kern<true GH_UNPAREN targs>), dim3((c)->persist * (c)->ncu), dim3(bs_), 0, \
                          (c)->stream, (c)->d, (int)(ni), (c)->remap, (int)(st));     \
-    else                                                                             \
-      hipLaunchKernelGGL((kern<false GH_UNPAREN targs>), dim3(((ni) + bs_ - 1) / bs_, (ntr)), dim3(bs_), 0, \
-                         (c)->stream, (c)->d, (int)(ni), (c)->remap, (int)(st));     \
+    } else {                                                                         \
+      PlainLaunch pl_ = plain_launch((c), (ntr), (ni));                              \
+      hipLaunchKernelGGL((kern<false GH_UNPAREN targs>), pl_.g, dim3(bs_), 0,        \
+                         (c)->stream, (c)->d, (int)(ni), pl_.remap, (int)(st));      \
+    }                                                                                \
   } while (0)
 #endif
 
@@ -2134,10 +2167,10 @@ struct Pipe {
         L2(c, body_f1cell<NQ>, d.nj, d.nax);
 #else
         int bs = c->block;
-        dim3 g = d.packed ? dim3(c->persist * c->ncu) : dim3((d.nax + bs - 1) / bs, d.nj);
+        PlainLaunch pl = plain_launch(c, d.nj, d.nax);
         size_t lds = sizeof(double) * bs * f1_kc<NQ>() * NQ;
-        if (d.packed) hipLaunchKernelGGL((k_f1cell<true, NQ>), g, dim3(bs), lds, c->stream, d, d.nax, c->remap);
-        else hipLaunchKernelGGL((k_f1cell<false, NQ>), g, dim3(bs), lds, c->stream, d, d.nax, c->remap);
+        if (d.packed) hipLaunchKernelGGL((k_f1cell<true, NQ>), dim3(c->persist * c->ncu), dim3(bs), lds, c->stream, d, d.nax, c->remap);
+        else hipLaunchKernelGGL((k_f1cell<false, NQ>), pl.g, dim3(bs), lds, c->stream, d, d.nax, pl.remap);
 #endif
         tend(c, 5);
       } else {
@@ -2149,15 +2182,16 @@ struct Pipe {
 #else
         if (!d.pc) {
           int bs = c->block;
-          dim3 g = d.packed ? dim3(c->persist * c->ncu) : dim3((d.nax + bs - 1) / bs, d.nj);
+          dim3 g = dim3(c->persist * c->ncu);
+          PlainLaunch pl = plain_launch(c, d.nj, d.nax);
           size_t lds = sizeof(double) * bs * f1_kc<NQ>() * NQ;
           bool sph = c->cfg.geom == SPH;
           if (d.packed) {
             if (sph) hipLaunchKernelGGL((k_f1grad<true, NQ, SPH>), g, dim3(bs), lds, c->stream, d, d.nax, c->remap);
             else hipLaunchKernelGGL((k_f1grad<true, NQ, CART>), g, dim3(bs), lds, c->stream, d, d.nax, c->remap);
           } else {
-            if (sph) hipLaunchKernelGGL((k_f1grad<false, NQ, SPH>), g, dim3(bs), lds, c->stream, d, d.nax, c->remap);
-            else hipLaunchKernelGGL((k_f1grad<false, NQ, CART>), g, dim3(bs), lds, c->stream, d, d.nax, c->remap);
+            if (sph) hipLaunchKernelGGL((k_f1grad<false, NQ, SPH>), pl.g, dim3(bs), lds, c->stream, d, d.nax, pl.remap);
+            else hipLaunchKernelGGL((k_f1grad<false, NQ, CART>), pl.g, dim3(bs), lds, c->stream, d, d.nax, pl.remap);
           }
         }
 #endif
@@ -2169,13 +2203,13 @@ struct Pipe {
 #else
         {
           int bs = c->block;
-          dim3 g((d.nax + bs - 1) / bs, d.nj);
+          PlainLaunch pl = plain_launch(c, d.nj, d.nax);   // slots walked in strides of the grid's width
           // 2D grids have one geometry: the kernel is built for it
           // capped build: 4 wavefronts per SIMD with six fields, 3 with ten
           // (at 4 the ten-field states spill 300 bytes per thread)
           constexpr int WC = NQ > 6 ? 3 : 4;
           bool sph = c->cfg.geom == SPH, cap = (c->occ & 1) != 0;
-          auto launch = [&](auto kern) { hipLaunchKernelGGL(kern, g, dim3(bs), 0, c->stream, d, c->remap); };
+          auto launch = [&](auto kern) { hipLaunchKernelGGL(kern, pl.g, dim3(bs), 0, c->stream, d, pl.remap); };
           if (d.shock) {
             if (sph) { if (cap) launch(k_f1faces<NQ, WC, true, SPH>); else launch(k_f1faces<NQ, 1, true, SPH>); }
             else { if (cap) launch(k_f1faces<NQ, WC, true, CART>); else launch(k_f1faces<NQ, 1, true, CART>); }
@@ -2621,7 +2655,10 @@ int gammahip_set_option(gammahip_ctx* c, int key, long long value) {
       if (value == gh::C2P_BRENT && c->d.mw) return fail(c, "the learned guess is loaded: it seeds the Newton solver");
       if (value == gh::C2P_NEWTON && c->d.P.synge) return fail(c, "the Synge equation of state runs with the Brent cons2prim");
       c->d.solver = (int)value; return 0;
-    case GAMMAHIP_OPT_REMAP: c->remap = value ? 1 : 0; return 0;
+    case GAMMAHIP_OPT_REMAP:
+      if (value < 0 || value > 2) return fail(c, "remap must be 0 (hardware order), 1 (XCD bands) or 2 (track groups)");
+      c->remap = value;
+      return 0;
     case GAMMAHIP_OPT_GRAPH: c->use_graph = value ? 1 : 0; return 0;
     case GAMMAHIP_OPT_MLP_THR:
       if (value < 0 || value > 16) return fail(c, "network threshold exponent must be in 0..16");

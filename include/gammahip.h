@@ -159,8 +159,10 @@ int  gammahip_sync(gammahip_ctx* ctx);
 
 /* Launch options (no counterpart in the reference; chosen by the autotuner in
  * gamma_amd/autotune.py): threads per workgroup of the per-cell kernels (64,
- * 128, 256; default 64), XCD-aware ordering of the (track, chunk) tiles
- * (default off), replay of
+ * 128, 256; default 128), the order of the (track, chunk) tiles (REMAP 0
+ * hardware order; 1 XCD-aware bands of tracks; 2 track groups, the default:
+ * each workgroup takes one 64-cell chunk of block/64 consecutive tracks, one
+ * track per wavefront), replay of
  * two-step hipGraphs by gammahip_step, and n for the learned guess's trigger:
  * a wavefront runs the network when one of its cells' previous pressures
  * leaves a residual |f| above 10^-n times the magnitude of f's terms
