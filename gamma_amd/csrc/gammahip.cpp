@@ -1337,20 +1337,22 @@ __global__ void __launch_bounds__(KB) k_f1cell(Dev d, int ni, int remap) {
   GH_TILE_LOOP(PK, if (i < ni && j < d.nj) body_f1cell_c<NQ>(d, j, i, cache, (int)blockDim.x))
 }
 GH_KERNEL2P((, int NQ), k_f1sum, body_f1sum<NQ>)
-// face pass over the F1 slots of each track: thread (j, c*bs + t) of a plain
-// (chunk, track) grid walks slots c*bs + t + m*(chunks*bs)
+// face pass over the F1 slots of each track: one thread per slot of the
+// track's capacity (4 nax); threads past the track's slot count leave at
+// once.  (A grid-stride loop over the slots kept the shock detector's
+// constants live across it: 153 scalars spilled and 203 VGPRs in the ten-field
+// spherical build, against 4 and 169 without the loop.)
 // W = minimum wavefronts per SIMD the compiler must allow (registers capped
 // accordingly, spilling if it must): 1 leaves the choice to the compiler, 4
 // trades spills for latency hiding.  The launch option picks one.
 // (XCD-aware with remap, as the cell passes: the face reads the upper track)
 template <int NQ, int W, bool SHK, int GEO>
 __global__ void __launch_bounds__(KB, W) k_f1faces(Dev d, int remap) {
-  int j, s0;
-  tile_of(remap, j, s0);
+  int j, s;
+  tile_of(remap, j, s);
   if (j >= d.nj) return;
-  int stride = remap == 2 ? (int)gridDim.x * 64 : (int)(gridDim.x * blockDim.x);
   int tot = min(d.f1tot[j], d.fcap);
-  for (int s = s0; s < tot; s += stride) body_f1face_s<NQ, SHK, GEO>(d, j, s);
+  if (s < tot) body_f1face_s<NQ, SHK, GEO>(d, j, s);
 }
 // W: minimum wavefronts per SIMD (1 = the compiler's choice; 5 caps the
 // registers at 96 and spills the rest), picked by the launch option; GEO as
@@ -2203,7 +2205,7 @@ struct Pipe {
 #else
         {
           int bs = c->block;
-          PlainLaunch pl = plain_launch(c, d.nj, d.nax);   // slots walked in strides of the grid's width
+          PlainLaunch pl = plain_launch(c, d.nj, d.fcap);  // one thread per F1 slot of the capacity
           // 2D grids have one geometry: the kernel is built for it
           // capped build: 4 wavefronts per SIMD with six fields, 3 with ten
           // (at 4 the ten-field states spill 300 bytes per thread)
