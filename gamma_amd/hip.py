@@ -225,12 +225,13 @@ class Grid:
 
     def set_options(self, block=None, remap=None, graph=None, mlp_thr=None, persist=None, occ=None,
                     f1fuse=None, solver=None, f1cell=None):
-        """Launch options (threads per workgroup, XCD-aware tile order, hipGraph
-        replay, trigger exponent n of the learned guess: threshold 10^-n,
-        persistent packed launch with n workgroups per CU or 0, register-capped
-        builds: bit 0 the F1 face kernel, bit 1 the update kernel; F1 flux sums
-        inside the update kernel; cons2prim solver, C2P_BRENT or C2P_NEWTON;
-        fused transverse-gradient and own-face F1 pass)."""
+        """Launch options: threads per workgroup; tile order (0 hardware, 1 XCD
+        bands, 2 track groups); hipGraph replay; trigger exponent n of the
+        learned guess (threshold 10^-n); persistent packed launch with n
+        workgroups per CU or 0; register-capped builds (bit 0 the F1 face
+        kernel, bit 1 the update kernel); F1 flux sums inside the update
+        kernel; cons2prim solver (C2P_BRENT or C2P_NEWTON); fused
+        transverse-gradient and own-face F1 pass."""
         for key, val in ((OPT_SOLVER, solver), (OPT_BLOCK, block), (OPT_REMAP, remap), (OPT_GRAPH, graph),
                          (OPT_MLP_THR, mlp_thr), (OPT_PERSIST, persist), (OPT_OCC, occ),
                          (OPT_F1FUSE, f1fuse), (OPT_F1CELL, f1cell)):
