@@ -223,8 +223,9 @@ def main():
                     help="run only that configuration's plain form for --steps steps (for profilers)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU only; multi-GPU sharding is a "
-                                   "design (see DESIGN.md), not built", "n_gpus": args.gpus}))
+        print(json.dumps({"error": "this build runs on one GPU only; sharding the tracks across GPUs "
+                                   "(ghost-track exchange per stage, one dt reduction per step) is "
+                                   "designed, not built", "n_gpus": args.gpus}))
         sys.exit(2)
 
     from gamma_amd import autotune, hip, setups
