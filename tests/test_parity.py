@@ -2,7 +2,7 @@
 
 Each case runs the same setup through the oracle (CPU restatement) and the
 backend, then compares, over the active cells of every track:
-  * L1-relative error of rho, p and u  <= 1e-10  (north-star tolerance);
+  * L1-relative error of rho, p and u  <= 1e-10  (the required tolerance);
   * identical per-track cell counts and regrid split/merge counts;
   * identical shock flags when shock detection is on.
 The ``emu`` backend runs the kernel bodies on the CPU (index/arithmetic check
