@@ -156,18 +156,7 @@ GH_HD Geo geo_of(const Dev& d, int j, long long k) {
 template <int ND>
 GH_HD double sweep_r(const Dev& d, long long k, const Geo& g) { return ND == 1 ? g.cen0 : d.cx[k]; }
 
-// state of a cell as the prepForUpdate sweeps leave it: prim, cons, flux along dim
-template <int ND, int NQ>
-GH_HD void cell_state(const Dev& d, int j, int i, int dim, double* p, double* c, double* f) {
-  long long k = CI(d, j, i);
-  load_v<ND, NQ>(d, d.prim, k, p);
-  Geo g = geo_of<ND>(d, j, k);
-  double r = sweep_r<ND>(d, k, g);
-  prim2cons<ND, NQ>(p, c, r, d.tgeom[j], d.P.gamma, d.P.synge);
-  state2flux<ND, NQ>(p, c, f, dim, r, d.tgeom[j]);
-}
-
-// prim and cons of a cell as cell_state forms them; returns the radius its
+// prim and cons of a cell as the prepForUpdate sweeps leave them; returns the radius its
 // physical flux takes
 template <int ND, int NQ>
 GH_HD double cell_pc(const Dev& d, int j, int i, double* p, double* c) {
@@ -1210,12 +1199,6 @@ __device__ __forceinline__ void tile_of(int remap, int& j, int& i) {
     int i, j;                                                                    \
     tile_of(remap, j, i);                                                        \
     if (i < ni && j < d.nj) BODY(d, j, i);                                       \
-  }
-#define GH_KERNEL2S(NAME, BODY)                                                 \
-  __global__ void __launch_bounds__(KB) NAME(Dev d, int ni, int remap, int stage) { \
-    int i, j;                                                                    \
-    tile_of(remap, j, i);                                                        \
-    if (i < ni && j < d.nj) BODY(d, j, i, stage);                                \
   }
 // Packed form: a persistent grid walks the tile list built by k_tiles.  With
 // remap the list is cut into 8 contiguous bands, band x walked by the
