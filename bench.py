@@ -36,8 +36,10 @@ def algorithmic_bytes(kid: int, cells: float, f1faces: float, nq: int, stage_mix
     that are RK stages 2/3 (which also read the RK base)."""
     d, i4 = 8.0, 4.0
     if kid == 0:   # k_update: prim, x, dx, face x, v, flux in; prim, x, dx, face x out;
-        # RK base (U0, dV0, x0) written in stage 1 / read in stages 2-3; net F1 flux in (2D)
-        per = (nq + 2 + 2 + nq) * d + (nq + 3) * d + (nq + 2) * d
+        # RK base (U0, dV0, x0) written in stage 1 / read in stages 2-3; net F1 flux in (2D);
+        # in stages 2-3 also the dt candidates of the stage's faces: the faces' two
+        # signal speeds in, the cell's dt_loc read and written
+        per = (nq + 2 + 2 + nq) * d + (nq + 3) * d + (nq + 2) * d + stage_mix * 4 * d
         if f1faces > 0:
             per += nq * d
         return cells * per

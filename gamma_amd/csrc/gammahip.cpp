@@ -846,6 +846,7 @@ GH_HD void body_update_s(const Dev& d, int j, int i, int stage) {
 }
 template <int ND, int NQ>
 GH_HD void body_update(const Dev& d, int j, int i, int stage) {
+  if (stage > 1) body_dt(d, j, i);         // the faces' dt candidates (prep with dt = false)
   if (d.solver == C2P_NEWTON) body_update_s<ND, NQ, C2P_NEWTON>(d, j, i, stage);
   else body_update_s<ND, NQ, C2P_BRENT>(d, j, i, stage);
 }
@@ -1346,7 +1347,10 @@ template <int ND, int NQ, int W, int GEO>
 constexpr int update_waves() { return (W == 1 && ND == 2 && NQ == 6 && GEO == CART) ? 5 : W; }
 template <bool PK, int ND, int NQ, int SV, int W, int GEO>
 __global__ void __launch_bounds__(KB, (update_waves<ND, NQ, W, GEO>())) k_update(Dev d, int ni, int remap, int stage) {
-  GH_TILE_LOOP(PK, if (i < ni && j < d.nj) body_update_s<ND, NQ, SV, GEO>(d, j, i, stage))
+  GH_TILE_LOOP(PK, if (i < ni && j < d.nj) {
+    if (stage > 1) body_dt(d, j, i);
+    body_update_s<ND, NQ, SV, GEO>(d, j, i, stage);
+  })
 }
 GH_KERNEL2P((, int ND, int NQ), k_inject, (body_inject<ND, NQ>))
 template <int ND, int NQ> GH_KERNEL2(k_init_gammas, (body_init_gammas<ND, NQ>))
@@ -1868,7 +1872,10 @@ __global__ void __launch_bounds__(KB) k_update_mlp(Dev d, int ni, int remap, int
   __shared__ float rows[KB / 64][64];
   int wv = threadIdx.x / 64;
   MlpGuess g{d.mw, (const bf16x8*)d.mw2b, lds[wv], rows[wv], d.mlp_thr, d.P.gamma};
-  GH_TILE_LOOP(PK, body_update_g<ND, NQ, C2P_NEWTON>(d, j, i, stage, i < ni && j < d.nj, g))
+  GH_TILE_LOOP(PK, {
+    if (stage > 1 && i < ni && j < d.nj) body_dt(d, j, i);
+    body_update_g<ND, NQ, C2P_NEWTON>(d, j, i, stage, i < ni && j < d.nj, g);
+  })
 }
 
 // x^y as the kernels evaluate it (gh_pow), for numerics checks
@@ -2115,13 +2122,16 @@ struct Pipe {
     L2(c, KSEL((body_ghost_mv<ND, NQ>), (k_ghost_mv<ND, NQ>)), d.nj, 2 * d.ngst);
     if (retile) build_tiles(c);
   }
-  static void prep(gammahip_ctx* c) {
+  // dt = false: the dt candidates of these faces are formed by the update
+  // kernel that follows (stages 2 and 3, whose candidates are first read by
+  // the next step's collect_dt), which has the cell's faces at hand
+  static void prep(gammahip_ctx* c, bool dt = true) {
     Dev& d = c->d;
     tbeg(c, 1);
     if (d.shock) L2P(c, (body_faces<ND, NQ>), k_faces, (, ND, NQ, true), d.nj, d.nax);
     else L2P(c, (body_faces<ND, NQ>), k_faces, (, ND, NQ, false), d.nj, d.nax);
     tend(c, 1);
-    L2P(c, body_dt, k_dt, (), d.nj, d.nax);
+    if (dt) L2P(c, body_dt, k_dt, (), d.nj, d.nax);
     if (ND == 2) {
       L2P(c, body_nb, k_nb, (), d.nj, d.nax);
 #ifdef GH_EMU
@@ -2293,8 +2303,8 @@ struct Pipe {
   static void evolve(gammahip_ctx* c) {
     update(c, 1);
     if (c->d.euler) return;              // Grid::evolve of euler.cpp:12-19
-    ghosts(c); prep(c); update(c, 2);
-    ghosts(c); prep(c); update(c, 3);
+    ghosts(c); prep(c, false); update(c, 2);
+    ghosts(c); prep(c, false); update(c, 3);
   }
   // one iteration of Simu::run (simu.cpp:47-77)
   static void step(gammahip_ctx* c) {
