@@ -95,13 +95,13 @@ def run_form(setup, solver, options, steps, warmup, timing_steps, surrogate=None
         g.set_options(**options)
     g.step(warmup)
     g.sync()
-    n0 = int(g.tracks()["nact"][2:-2].sum())
+    n0 = track_cells(g)
     g.sync()
     t0 = time.perf_counter()
     g.step(steps)
     g.sync()
     el = time.perf_counter() - t0
-    n1 = int(g.tracks()["nact"][2:-2].sum())
+    n1 = track_cells(g)
     st = g.stats()
     nf1 = float(sum(a.sum() for a in g.active(hip.F_NF1))) if setup.cfg.nd == 2 else 0.
     g.kernel_times()                      # switches event timing on and clears

@@ -6,8 +6,6 @@ next to their sources, git-ignored but shipped with the tree).
   (GH_EMU); a test tool that checks indexing and arithmetic against the oracle
   without a GPU.  Never loaded by the package.
 * ``oracle/libgamma_oracle.so`` -- the CPU oracle (test infrastructure).
-* ``gamma_amd/copy_probe`` -- a plain copy kernel of known size, the
-  calibration point for the HBM traffic counters.
 * ``gamma_amd/host/shock_tube`` (and ``tests/shock_tube_emu``) -- a C++
   caller of the hot path through ``gamma_amd/host/gammahip_grid.hpp``, linked
   against the HIP library (and against the CPU build, for tests).
@@ -63,13 +61,6 @@ def build_oracle() -> str:
     return os.path.join(d, "libgamma_oracle.so")
 
 
-def build_copy_probe(out: str | None = None) -> str:
-    out = out or os.path.join(HERE, "copy_probe")
-    flags = [f for f in HIP_FLAGS if f not in ("-fPIC", "-shared")]
-    _run([HIPCC, *flags, os.path.join(HERE, "csrc", "copy_probe.cpp"), "-o", out])
-    return out
-
-
 def build_host() -> list:
     src = os.path.join(HERE, "host", "shock_tube.cpp")
     outs = []
@@ -86,7 +77,6 @@ def build_all() -> None:
     build_hip()
     build_emu()
     build_oracle()
-    build_copy_probe()
     build_host()
 
 

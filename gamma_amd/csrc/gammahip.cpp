@@ -2807,7 +2807,10 @@ int gammahip_state_restore(gammahip_ctx* c, const void* buf, unsigned long long 
     if (h2d(c, s.p, o, s.bytes)) return -1;
     o += s.pad;
   }
-  return 0;
+  // the restored track lengths replace the loaded ones: the packed launch's
+  // tile list must cover them before the next step's regrid screens the cells
+  build_tiles(c);
+  return post(c);
 }
 
 int gammahip_track_info(gammahip_ctx* c, int* ntrack, int* nact, int* iL, int* iR) {

@@ -20,7 +20,22 @@
  * version is unpinned by the reference Makefile, so Brent iterates are not
  * bit-pinned to GSL's; the converged root is pinned to 1e-13 relative, the
  * reference's own tolerance.
+ *
+ * Planted mistakes: built with -DOG_MUT=n (never by the Makefile) the oracle
+ * carries mistake n of the list below; the pin tests build these variants
+ * and check that the pin meant for each one fails on it.
+ *   1 RK3 stage-2 weights swapped          2 dt_loc reset before every stage
+ *   3 neighbour sweep without its step back 4 ghost cells copied from the
+ *                                             second active cell
+ *   5 merge ignores which neighbour is smaller 6 transverse gradient average
+ *                                             without its area weights
+ *   7 BM1D refinement window includes the shock cell 8 split with the
+ *                                             unlimited central gradient
+ *   9 AR2D rule reads dy/dx              10 RK3 stage-3 weights swapped
  */
+#ifndef OG_MUT
+#define OG_MUT 0
+#endif
 #include "gamma_oracle.h"
 #include "og_pow.h"
 #include <math.h>
@@ -75,6 +90,7 @@ struct og_grid {
   long *nsplit, *nmerge;
   double *tbm;   /* [nj][4] BM boundary parameters of each track (config bm by default) */
   int err;
+  double *rec;   /* og_record_stages: [3 stages][4 fields][nj * nax] */
 };
 
 #define CELL(g, j, i) (&(g)->C[(size_t)(j) * (g)->nax + (i)])
@@ -772,7 +788,7 @@ static void grid_update_ghosts(og_grid *g) {
   }
   for (int j = 0; j < g->nj; ++j) {
     int iL = g->iLbnd[j], iR = g->iRbnd[j], nt = g->ntrack[j];
-    Cell *eL = CELL(g, j, iL + 1);
+    Cell *eL = CELL(g, j, iL + (OG_MUT == 4 ? 2 : 1));
     for (int i = 0; i <= iL; ++i) {
       Cell *cl = CELL(g, j, i);
       *cl = *eL;
@@ -854,7 +870,7 @@ static void grid_neighbors(og_grid *g) {
       cl->neigh[1].j = j + 1; cl->neigh[1].lo = 0; cl->neigh[1].cnt = 0;
       if (j != 0) {
         double xm = FACE(g, j - 1, im)->x[MV];
-        if (xm > xjL && im > 0) { im--; xm = FACE(g, j - 1, im)->x[MV]; }
+        if (OG_MUT != 3 && xm > xjL && im > 0) { im--; xm = FACE(g, j - 1, im)->x[MV]; }
         while (xm < xjR) {
           if (cl->neigh[0].cnt == 0) cl->neigh[0].lo = im + 1;
           cl->neigh[0].cnt++;
@@ -864,7 +880,7 @@ static void grid_neighbors(og_grid *g) {
       }
       if (j != g->nj - 1) {
         double xp = FACE(g, j + 1, ip)->x[MV];
-        if (xp > xjL && ip > 0) { ip--; xp = FACE(g, j + 1, ip)->x[MV]; }
+        if (OG_MUT != 3 && xp > xjL && ip > 0) { ip--; xp = FACE(g, j + 1, ip)->x[MV]; }
         while (xp < xjR) {
           if (cl->neigh[1].cnt == 0) cl->neigh[1].lo = ip + 1;
           cl->neigh[1].cnt++;
@@ -918,6 +934,7 @@ static void trans_gradients(og_grid *g, int j, int i) {
       Int.x[MV] = (fmax(xL0, xLn) + fmin(xR0, xRn)) / 2.;
       Int.dx = fmax(0, fmin(xR0, xRn) - fmax(xL0, xLn));
       face_dA(c, &Int);
+      if (OG_MUT == 6) Int.dA = 1.;
       dAtot += Int.dA;
       double xm = Int.x[MV];
       double dxmL = xm - c0->G.cen[MV], dxmR = xm - cn->G.cen[MV];
@@ -1016,6 +1033,8 @@ static int grid_fluxes(og_grid *g) {
 
 /* Grid::prepForUpdate  grid.cpp:28-37 */
 static int grid_prep(og_grid *g) {
+  if (OG_MUT == 2)
+    for (size_t k = 0; k < (size_t)g->nj * g->nax; ++k) g->C[k].dt_loc = 1.e15;
   grid_prim2cons(g);
   grid_state2flux(g);
   grid_neighbors(g);
@@ -1110,6 +1129,22 @@ static void grid_rk_average(og_grid *g, double w0, double w1) {
   }
 }
 
+/* test hook: the faces' signal speeds and velocities and the cells' widths
+ * as the prepForUpdate of stage s left them (og_record_stages) */
+static void record_stage(og_grid *g, int s) {
+  if (!g->rec) return;
+  size_t n = (size_t)g->nj * g->nax;
+  double *r = g->rec + (size_t)(s - 1) * 4 * n;
+  for (int j = 0; j < g->nj; ++j) {
+    for (int i = 0; i < g->nax - 1; ++i) {
+      const Face *f = FACE(g, j, i);
+      size_t k = (size_t)j * (g->nax - 1) + i;
+      r[k] = f->v; r[n + k] = f->lL; r[2 * n + k] = f->lR;
+    }
+    for (int i = 0; i < g->nax; ++i) r[3 * n + (size_t)j * g->nax + i] = CELL(g, j, i)->G.dl[MV];
+  }
+}
+
 /* Grid::evolve (RK3)  rk3.cpp:12-96 (1D), 98-195 (2D); Euler: euler.cpp:12-19 */
 static int grid_evolve(og_grid *g, double dt) {
   int e;
@@ -1118,12 +1153,16 @@ static int grid_evolve(og_grid *g, double dt) {
   grid_update(g, dt);
   grid_update_ghosts(g);
   if ((e = grid_prep(g))) return e;
+  record_stage(g, 2);
   grid_update(g, dt);
-  grid_rk_average(g, 3. / 4., 1. / 4.);
+  if (OG_MUT == 1) grid_rk_average(g, 1. / 4., 3. / 4.);
+  else grid_rk_average(g, 3. / 4., 1. / 4.);
   grid_update_ghosts(g);
   if ((e = grid_prep(g))) return e;
+  record_stage(g, 3);
   grid_update(g, dt);
-  grid_rk_average(g, 1. / 3., 2. / 3.);
+  if (OG_MUT == 10) grid_rk_average(g, 2. / 3., 1. / 3.);
+  else grid_rk_average(g, 1. / 3., 2. / 3.);
   return 0;
 }
 
@@ -1157,6 +1196,12 @@ static void grid_split(og_grid *g, int j, int i) {
     grad_prim(c, cL, cc, MV, gL); grad_prim(c, cc, cRo, MV, gR);
     for (int q = 0; q < c->nq; ++q) gC[q] = minmod(gL[q], gR[q]);
   }
+  if (OG_MUT == 8)
+    for (int q = 0; q < c->nq; ++q) {
+      double a[QM], b[QM];
+      grad_prim(c, cL, cc, MV, a); grad_prim(c, cc, cRo, MV, b);
+      gC[q] = (a[q] + b[q]) / 2.;
+    }
   double x_old = cc->G.x[MV], dl_old = cc->G.dx[MV];
   cc->G.x[MV] = x_old - dl_old / 4.;
   cR->G.x[MV] = x_old + dl_old / 4.;
@@ -1187,6 +1232,7 @@ static void grid_merge(og_grid *g, int j, int i) {
   int ngst = c->ngst;
   Cell *cc = CELL(g, j, i), *cL = CELL(g, j, i - 1), *cR = CELL(g, j, i + 1), *cV;
   int left = ((cL->G.dx[MV] < cR->G.dx[MV] && i != ngst) || i == g->nact[j] + ngst - 1);
+  if (OG_MUT == 5) left = i != ngst;
   cV = left ? cL : cR;
   double rc = c->nd == 1 ? cc->G.cen[0] : cc->G.x[0];
   double rv = c->nd == 1 ? cV->G.cen[0] : cV->G.x[0];
@@ -1216,7 +1262,7 @@ static int regrid_check(og_grid *g, int j, int i, double *iS_cache, int *iS_vali
   const og_config *c = &g->c;
   Cell *cl = CELL(g, j, i);
   if (c->rg_mode == OG_RG_AR2D) {
-    double ar = cl->G.dx[MV] / cl->G.dx[F1];
+    double ar = OG_MUT == 9 ? cl->G.dx[F1] / cl->G.dx[MV] : cl->G.dx[MV] / cl->G.dx[F1];
     if (ar > c->rg[0]) return SPLIT;
     if (ar < c->rg[1]) return MERGE;
     return SKIP;
@@ -1235,7 +1281,7 @@ static int regrid_check(og_grid *g, int j, int i, double *iS_cache, int *iS_vali
       *iS_cache = iS; *iS_valid = 1;
     }
     double dist = i - *iS_cache;
-    if (0 < dist && dist < 10) ar *= 10;
+    if ((OG_MUT == 7 ? 0 <= dist : 0 < dist) && dist < 10) ar *= 10;
     double lfac = fs_lfac(c, &cl->S);
     if (ar > c->rg[0] * target / og_pow(lfac, 3. / 2.)) return SPLIT;
     if (ar < c->rg[1] * target / og_pow(lfac, 3. / 2.)) return MERGE;
@@ -1366,7 +1412,7 @@ og_grid *og_create(const og_config *cfg) {
 void og_destroy(og_grid *g) {
   if (!g) return;
   free(g->C); free(g->I); free(g->nact); free(g->ntrack); free(g->iLbnd); free(g->iRbnd);
-  free(g->nsplit); free(g->nmerge); free(g->tbm); free(g);
+  free(g->nsplit); free(g->nmerge); free(g->tbm); free(g->rec); free(g);
 }
 
 /* Initial active cells per track (Grid::prepForRun gives every track
@@ -1464,8 +1510,10 @@ static int og_step(og_grid *g) {
   if (g->c.shock) grid_reset_shock(g);
   grid_update_ghosts(g);
   if ((e = grid_prep(g))) return e;
+  record_stage(g, 1);
   if (g->c.shock) grid_detect_shock(g);
   double dt = g->c.cfl * grid_collect_dt(g);
+  if (g->c.dt_fixed > 0) dt = g->c.dt_fixed;
   g->dt = dt;
   if ((e = grid_evolve(g, dt))) return e;
   g->t += dt;
@@ -1617,4 +1665,40 @@ void og_pow_batch(const double *x, const double *y, double *out, long n) {
 }
 void og_fn_batch(int fn, const double *x, double *out, long n) {
   for (long i = 0; i < n; ++i) out[i] = fn == 0 ? og_tanh(x[i]) : og_log10(x[i]);
+}
+
+/* ---- pieces of a step (pin tests) ---- */
+int og_prep(og_grid *g) {
+  grid_update_ghosts(g);
+  return grid_prep(g);
+}
+int og_update_ghosts(og_grid *g) {
+  grid_update_ghosts(g);
+  return 0;
+}
+void og_get_neighbors(const og_grid *g, int j, int i, int *out) {
+  const Cell *cl = CELL(g, j, i);
+  out[0] = cl->neigh[0].j; out[1] = cl->neigh[0].lo; out[2] = cl->neigh[0].cnt;
+  out[3] = cl->neigh[1].j; out[4] = cl->neigh[1].lo; out[5] = cl->neigh[1].cnt;
+}
+void og_split(og_grid *g, int j, int i) { grid_split(g, j, i); g->nsplit[j]++; g->cnt[4]++; }
+void og_merge(og_grid *g, int j, int i) { grid_merge(g, j, i); g->nmerge[j]++; g->cnt[5]++; }
+int og_regrid_decision(og_grid *g, int j, int i) {
+  double iS = 0; int iS_valid = 0;
+  return regrid_check(g, j, i, &iS, &iS_valid);
+}
+int og_record_stages(og_grid *g, int on) {
+  free(g->rec);
+  g->rec = NULL;
+  if (!on) return 0;
+  g->rec = (double *)calloc((size_t)12 * g->nj * g->nax, sizeof(double));
+  return g->rec ? 0 : -1;
+}
+int og_get_stage(const og_grid *g, int stage, int field, double *out) {
+  if (!g->rec || stage < 1 || stage > 3 || field < 0 || field > 3) return -1;
+  size_t n = (size_t)g->nj * g->nax;
+  const double *r = g->rec + (size_t)(stage - 1) * 4 * n + (size_t)field * n;
+  size_t m = field == 3 ? n : (size_t)g->nj * (g->nax - 1);
+  memcpy(out, r, sizeof(double) * m);
+  return 0;
 }

@@ -68,6 +68,9 @@ typedef struct {
                          with c2p_tol 1e-16 the root to the last place of a double:
                          1 and 2 only measure how far rounding inside the residual
                          moves the reference's trajectories */
+  double dt_fixed;    /* > 0: every step advances by this dt instead of CFL_ * collect_dt
+                         (collect_dt still runs and resets dt_loc); pins the time
+                         integrator's order at a fixed step.  0: the reference's rule */
 } og_config;
 
 typedef struct og_grid og_grid;
@@ -107,6 +110,26 @@ void     og_get_faces(const og_grid *g, int field, double *out /* [nj][nax-1] */
 void     og_counters(const og_grid *g, long *out /* [8] */);
 /* per-track split/merge counts [nj] */
 void     og_regrid_counts(const og_grid *g, long *splits, long *merges);
+
+/* ---- pieces of a step, for the oracle's own pin tests ----
+ * og_prep: the step's first half as Simu::run runs it before collect_dt
+ * (updateGhosts, prepForUpdate: neighbours, MV faces, kinematics, fluxes).
+ * og_update_ghosts: Grid::updateGhosts alone.  Both return 0 or an error code. */
+int      og_prep(og_grid *g);
+int      og_update_ghosts(og_grid *g);
+/* F1 neighbour list of cell (j, i): out = {j_down, first, count, j_up, first, count};
+ * the list is the cells first .. first+count-1 of track j_down / j_up */
+void     og_get_neighbors(const og_grid *g, int j, int i, int *out /* [6] */);
+/* Grid::split / Grid::merge of cell (j, i) as applyRegrid calls them, and the
+ * setup's checkCellForRegrid decision for (j, i): 0 skip, 1 merge, 2 split */
+void     og_split(og_grid *g, int j, int i);
+void     og_merge(og_grid *g, int j, int i);
+int      og_regrid_decision(og_grid *g, int j, int i);
+/* Record, after each prepForUpdate of a step (stage 1, 2, 3), the faces'
+ * v, lL, lR and the cells' dl[MV]: og_get_stage(g, stage, field, out) with
+ * field 0 v, 1 lL, 2 lR ([nj][nax-1]), 3 dl ([nj][nax]). */
+int      og_record_stages(og_grid *g, int on);
+int      og_get_stage(const og_grid *g, int stage, int field, double *out);
 
 /* ---- pointwise entry points used by the oracle's pin tests ---- */
 void og_prim2cons(const og_config *c, const double *prim, double r, double *cons);

@@ -30,11 +30,11 @@ class OgConfig(C.Structure):
         ("rg_mode", C.c_int), ("rg", C.c_double * 6),
         ("c2p_maxiter", C.c_int), ("timestep", C.c_int), ("recon", C.c_int), ("vpspec", C.c_int),
         ("eos", C.c_int), ("circ_regrid", C.c_int),
-        ("c2p_tol", C.c_double), ("c2p_form", C.c_int),
+        ("c2p_tol", C.c_double), ("c2p_form", C.c_int), ("dt_fixed", C.c_double),
     ]
 
 
-def make_config(rc, c2p_tol=1e-13, c2p_form=0) -> OgConfig:
+def make_config(rc, c2p_tol=1e-13, c2p_form=0, dt_fixed=0.) -> OgConfig:
     c = OgConfig()
     c.nd, c.geom, c.nq, c.shock, c.sync = rc.nd, rc.geom, rc.nq, rc.shock, rc.sync
     c.ngst, c.nmax, c.ncell_mv, c.ncell_f1 = rc.ngst, rc.nmax, rc.ncell_mv, rc.ncell_f1
@@ -52,18 +52,21 @@ def make_config(rc, c2p_tol=1e-13, c2p_form=0) -> OgConfig:
     c.circ_regrid = rc.circ_regrid
     c.c2p_tol = c2p_tol
     c.c2p_form = c2p_form
+    c.dt_fixed = dt_fixed
     return c
 
 
-_lib = None
+_libs = {}
 
 
-def lib():
-    global _lib
-    if _lib is None:
-        if not os.path.exists(LIB):
+def lib(path: str | None = None):
+    """The oracle library (default build), or another build of it at `path`
+    (the pin tests load builds with planted mistakes)."""
+    path = path or LIB
+    if path not in _libs:
+        if path == LIB and not os.path.exists(LIB):
             subprocess.check_call(["make", "-s", "-C", os.path.join(ROOT, "oracle")])
-        L = C.CDLL(LIB)
+        L = C.CDLL(path)
         P, D, I, V = C.c_void_p, C.c_double, C.c_int, None
         cfgp = C.POINTER(OgConfig)
         sig = {
@@ -82,13 +85,16 @@ def lib():
             "og_compute_sd": (D, [cfgp, P, P, I, I, P]),
             "og_cell_geom": (V, [cfgp, D, D, D, D, P]), "og_face_area": (D, [cfgp, I, D, D, D]),
             "og_sync_source": (D, [cfgp, P, D, D]), "og_inject": (V, [cfgp, P, I, D]),
+            "og_prep": (I, [P]), "og_update_ghosts": (I, [P]), "og_get_neighbors": (V, [P, I, I, P]),
+            "og_split": (V, [P, I, I]), "og_merge": (V, [P, I, I]), "og_regrid_decision": (I, [P, I, I]),
+            "og_record_stages": (I, [P, I]), "og_get_stage": (I, [P, I, I, P]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
             fn.restype = res
             fn.argtypes = args
-        _lib = L
-    return _lib
+        _libs[path] = L
+    return _libs[path]
 
 
 def _p(a: np.ndarray):
@@ -104,11 +110,12 @@ I_X, I_V, I_LL, I_LR, I_LS, I_DA, I_FLUX0 = 0, 1, 2, 3, 4, 5, 10
 class OracleGrid:
     """Grid + Simu::run loop of the oracle for one setup."""
 
-    def __init__(self, setup, c2p_tol=1e-13, c2p_form=0):
+    def __init__(self, setup, c2p_tol=1e-13, c2p_form=0, dt_fixed=0., lib_path=None):
         self.setup = setup
         self.rc = setup.cfg
-        self.cfg = make_config(self.rc, c2p_tol, c2p_form)
-        self.h = lib().og_create(C.byref(self.cfg))
+        self.cfg = make_config(self.rc, c2p_tol, c2p_form, dt_fixed)
+        self.L = L = lib(lib_path)
+        self.h = L.og_create(C.byref(self.cfg))
         if not self.h:
             raise MemoryError("oracle allocation failed")
         x = np.ascontiguousarray(setup.x, dtype=np.float64)
@@ -118,56 +125,56 @@ class OracleGrid:
         prim = np.ascontiguousarray(setup.prim, dtype=np.float64)
         if setup.ncells is not None:
             nc = np.ascontiguousarray(setup.ncells, dtype=np.int32)
-            if lib().og_set_track_cells(self.h, _p(nc)) != 0:
+            if self.L.og_set_track_cells(self.h, _p(nc)) != 0:
                 raise ValueError("bad per-track cell counts")
         if setup.hooks is not None:
             bm = np.array([list(h.get("bm", self.rc.bm))[:4] for h in setup.hooks], dtype=np.float64)
-            lib().og_set_track_bm(self.h, _p(bm))
-        lib().og_load(self.h, _p(x), _p(dx), _p(y), _p(dy), _p(prim))
-        lib().og_init(self.h, setup.t0)
-        self.nj = lib().og_ntracks(self.h)
-        self.nax = lib().og_nax(self.h)
+            self.L.og_set_track_bm(self.h, _p(bm))
+        self.L.og_load(self.h, _p(x), _p(dx), _p(y), _p(dy), _p(prim))
+        self.L.og_init(self.h, setup.t0)
+        self.nj = self.L.og_ntracks(self.h)
+        self.nax = self.L.og_nax(self.h)
 
     def __del__(self):
         if getattr(self, "h", None):
-            lib().og_destroy(self.h)
+            self.L.og_destroy(self.h)
             self.h = None
 
     def run(self, nsteps: int) -> None:
-        e = lib().og_run(self.h, nsteps)
+        e = self.L.og_run(self.h, nsteps)
         if e:
             raise RuntimeError(f"oracle run failed with code {e}")
 
     @property
     def t(self) -> float:
-        return lib().og_time(self.h)
+        return self.L.og_time(self.h)
 
     @property
     def it(self) -> int:
-        return lib().og_iter(self.h)
+        return self.L.og_iter(self.h)
 
     @property
     def dt(self) -> float:
-        return lib().og_last_dt(self.h)
+        return self.L.og_last_dt(self.h)
 
     def tracks(self):
         n = [np.zeros(self.nj, dtype=np.int32) for _ in range(4)]
-        lib().og_track_info(self.h, *[_p(a) for a in n])
+        self.L.og_track_info(self.h, *[_p(a) for a in n])
         return dict(ntrack=n[0], nact=n[1], iLbnd=n[2], iRbnd=n[3])
 
     def cells(self, field: int) -> np.ndarray:
         out = np.zeros((self.nj, self.nax))
-        lib().og_get_cells(self.h, field, _p(out))
+        self.L.og_get_cells(self.h, field, _p(out))
         return out
 
     def faces(self, field: int) -> np.ndarray:
         out = np.zeros((self.nj, self.nax - 1))
-        lib().og_get_faces(self.h, field, _p(out))
+        self.L.og_get_faces(self.h, field, _p(out))
         return out
 
     def counters(self) -> dict:
         out = np.zeros(8, dtype=np.int64)
-        lib().og_counters(self.h, _p(out))
+        self.L.og_counters(self.h, _p(out))
         keys = ["c2p_calls", "brent_iters", "no_bracket", "max_iter", "splits", "merges",
                 "phi_cap", "tangled"]
         return dict(zip(keys, out.tolist()))
@@ -175,8 +182,46 @@ class OracleGrid:
     def regrid_counts(self):
         s = np.zeros(self.nj, dtype=np.int64)
         m = np.zeros(self.nj, dtype=np.int64)
-        lib().og_regrid_counts(self.h, _p(s), _p(m))
+        self.L.og_regrid_counts(self.h, _p(s), _p(m))
         return s, m
+
+    # ---- pieces of a step (pin tests) ----
+    def prep(self) -> None:
+        """updateGhosts + prepForUpdate, as the step runs them before collect_dt."""
+        e = self.L.og_prep(self.h)
+        if e:
+            raise RuntimeError(f"oracle prep failed with code {e}")
+
+    def update_ghosts(self) -> None:
+        self.L.og_update_ghosts(self.h)
+
+    def neighbors(self, j: int, i: int):
+        """((j_down, first, count), (j_up, first, count)) of cell (j, i)."""
+        out = np.zeros(6, dtype=np.int32)
+        self.L.og_get_neighbors(self.h, j, i, _p(out))
+        return tuple(out[:3].tolist()), tuple(out[3:].tolist())
+
+    def split(self, j: int, i: int) -> None:
+        self.L.og_split(self.h, j, i)
+
+    def merge(self, j: int, i: int) -> None:
+        self.L.og_merge(self.h, j, i)
+
+    def regrid_decision(self, j: int, i: int) -> int:
+        return self.L.og_regrid_decision(self.h, j, i)
+
+    def record_stages(self, on: bool = True) -> None:
+        if self.L.og_record_stages(self.h, int(on)):
+            raise MemoryError("oracle record allocation failed")
+
+    def stage(self, s: int, field: int) -> np.ndarray:
+        """Faces' v (0), lL (1), lR (2) [nj][nax-1] or cells' dl (3) [nj][nax]
+        as the prepForUpdate of stage s of the last step left them."""
+        shape = (self.nj, self.nax) if field == 3 else (self.nj, self.nax - 1)
+        out = np.zeros(shape)
+        if self.L.og_get_stage(self.h, s, field, _p(out)):
+            raise RuntimeError("stage recording is off")
+        return out
 
     def active(self, field: int):
         """List over active tracks of the active-cell values of a cell field."""
