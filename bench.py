@@ -63,13 +63,17 @@ def algorithmic_bytes(kid: int, cells: float, f1faces: float, nq: int, stage_mix
     return 0.0
 
 
-def cpu_baseline(sample_tracks: int = 64, nx: int = 2048, steps: int = 30) -> dict:
-    """Oracle (CPU restatement, one core) on a bounded sample of the same
-    workload: the bottom `sample_tracks` tracks of the 2048 x 2048 grid (same
-    cells, same seeded perturbation), `steps` RK3 steps after 2 warm-up steps."""
+def cpu_baseline(sample_tracks: int = 256, nx: int = 2048, steps: int = 20) -> dict:
+    """Oracle (the CPU restatement: track loops on all host cores, as the
+    reference's OpenMP loops run) on a bounded sample of the same workload:
+    the bottom `sample_tracks` tracks of the 2048 x 2048 grid (same cells,
+    same seeded perturbation), `steps` RK3 steps after 2 warm-up steps."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_ctypes as oc
     from gamma_amd import setups
+    L = oc.lib()
+    threads = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
+    L.og_set_threads(threads)
     s = setups.riemann_2d(nx, sample_tracks, ly=sample_tracks / nx)
     g = oc.OracleGrid(s)
     g.run(2)
@@ -79,9 +83,46 @@ def cpu_baseline(sample_tracks: int = 64, nx: int = 2048, steps: int = 30) -> di
     el = time.perf_counter() - t0
     n1 = sum(len(a) for a in g.active(0))
     cu = 0.5 * (n0 + n1) * steps
-    return {"value": cu / el, "unit": "cell-updates/s", "cores": 1, "kind": "port",
-            "sample": f"oracle, riemann_2d {nx} x {sample_tracks} tracks, {steps} RK3 steps "
-                      f"after 2 warm-up steps, {el:.1f} s"}
+    return {"value": cu / el, "unit": "cell-updates/s", "cores": int(L.og_threads()), "kind": "port",
+            "sample": f"oracle, riemann_2d {nx} cells x {sample_tracks} tracks, {steps} RK3 steps "
+                      f"after 2 warm-up steps, {el:.1f} s on {int(L.og_threads())} threads"}
+
+
+def b_cu(rc) -> float:
+    """Algorithmic bytes per cell-update of a fused RK3 step: 8 * [9 (Q_live + 1)
+    + (H + 1)], Q_live = NUM_D momenta + D + tau + 2 tracers (+ 4 with
+    synchrotron), H = NUM_D + 2 hydro primitives."""
+    q_live = rc.nd + 2 + 2 + (4 if rc.sync else 0)
+    h = rc.nd + 2
+    return 8.0 * (9 * (q_live + 1) + (h + 1))
+
+
+COPY_PROBE = r"""
+import json, torch
+n = 1 << 31
+a = torch.empty(n // 8, dtype=torch.float64, device="cuda"); b = torch.empty_like(a); a.fill_(1.0)
+for _ in range(3):
+    b.copy_(a)
+e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+torch.cuda.synchronize(); e0.record()
+for _ in range(10):
+    b.copy_(a)
+e1.record(); torch.cuda.synchronize()
+print(json.dumps({"gbs": 2 * n / (e0.elapsed_time(e1) / 10 * 1e-3) / 1e9}))
+"""
+
+
+def copy_peak_gbs():
+    """Measured streaming-copy bandwidth (bytes read + written per second) of a
+    2 GiB device-to-device copy (torch's copy kernel, timed with events).  It
+    runs in a process of its own: torch carries its own HIP runtime, which
+    cannot open the device in a process where the library's runtime already
+    has."""
+    import subprocess
+    r = subprocess.run([sys.executable, "-c", COPY_PROBE], capture_output=True, text=True, timeout=300)
+    if r.returncode != 0:
+        raise RuntimeError(r.stderr[-300:])
+    return json.loads(r.stdout.strip().splitlines()[-1])["gbs"]
 
 
 def run_form(setup, solver, options, steps, warmup, timing_steps, surrogate=None):
@@ -185,15 +226,28 @@ def measured_traffic(kernel: str):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC
     summary of this workload (FETCH_SIZE and WRITE_SIZE, each scaled by what
     the counter reads for a copy kernel of known size), or None."""
-    try:
-        import json as _j
-        t = _j.load(open(TRAFFIC_FILE))
-    except (OSError, ValueError):
-        return None
+    t = _traffic()
     for name, v in t.get("kernels", {}).items():
         if name.split("<")[0] == kernel:
             return v["bytes_per_launch"]
     return None
+
+
+def _traffic() -> dict:
+    try:
+        return json.load(open(TRAFFIC_FILE))
+    except (OSError, ValueError):
+        return {}
+
+
+def measured_step_traffic():
+    """Counter bytes of one whole step (every kernel's bytes per launch times
+    its launches per step) from the same summary, or None."""
+    t = _traffic()
+    steps = t.get("steps")
+    if not steps:
+        return None
+    return sum(v["bytes_per_launch"] * v["dispatches"] for v in t.get("kernels", {}).values()) / steps
 
 
 def kernel_table(res, nq):
@@ -243,11 +297,26 @@ def main():
     print(f"[bench] plain {plain['ms_per_step']:.3f} ms/step", file=sys.stderr, flush=True)
     kern = kernel_table(plain, nq)
     dom = max(kern, key=lambda k: kern[k]["ms"] * kern[k]["launches"])
-    ach = kern[dom]["GBs"]
+    bcu = b_cu(s.cfg)
+    # the survey's byte model: a fused step moves B_cu per cell-update, a third
+    # of it per stage; the dominant (per-stage) kernel is charged one stage's
+    # share of it for the cells it advances
+    alg = bcu / 3.0 * plain["cells"]
+    ach = alg / (kern[dom]["ms"] * 1e-3) / 1e9
+    step_bytes = measured_step_traffic()
     roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": measured_traffic(dom), "kernel": dom,
-            "kernel_ms": round(kern[dom]["ms"], 4),
-            "algorithmic_bytes": kern[dom]["bytes"]}
+            "kernel_ms": round(kern[dom]["ms"], 4), "algorithmic_bytes": alg,
+            "bytes_per_cell_update": bcu,
+            "step_frac": round(plain["value"] * bcu / (HBM_PEAK_GBS * 1e9), 4),
+            "step_counter_bytes": step_bytes,
+            "kernel_own_model": {"bytes": kern[dom]["bytes"], "achieved": round(kern[dom]["GBs"], 1),
+                                 "frac": round(kern[dom]["GBs"] / HBM_PEAK_GBS, 4)}}
+    try:
+        roof["copy_peak"] = round(copy_peak_gbs(), 1)
+    except Exception as e:                # torch missing or no device memory left: say so
+        roof["copy_peak"] = None
+        roof["copy_peak_error"] = str(e)[:200]
 
     batched = None
     if not args.no_batched:
@@ -292,7 +361,8 @@ def main():
         "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded SplitMix64 perturbation)",
         "config": {"workload": f"riemann_2d {args.ny} tracks x {args.nx} cells, 2D cartesian, "
                                f"plain per-stage kernels, Brent cons2prim",
-                   "tracks": args.ny, "cells_per_track": args.nx, "parallelism": "single GPU"},
+                   "tracks": args.ny, "cells_per_track": args.nx, "parallelism": "single GPU",
+                   "timed_steps": f"{args.warmup + 1}..{args.warmup + args.steps} of the run"},
         "roofline": roof, "cpu_baseline": cpu,
         "kernels": kern,
         "c2p_iters_per_call": st["c2p_iters"] / max(st["c2p_calls"], 1),

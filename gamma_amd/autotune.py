@@ -2,7 +2,8 @@
 
 Options: threads per workgroup of the per-cell kernels (64, 128, 256),
 XCD-aware tile order (off/on), two-step hipGraph replay (off/on), persistent
-packed launch (off, 4 or 8 workgroups per CU), the register-capped builds of
+packed launch (off, or at most 4, 8, 16 or 32 workgroups per CU, each
+kernel's grid capped at what its registers and LDS let a CU hold), the register-capped builds of
 the F1 face kernel and of the update kernel (bit set, 0-3), the F1 flux sums
 inside the update kernel (off/on), the fused F1 gradient-and-face pass
 (off/on), the cons2prim solver (Brent, the
@@ -24,9 +25,9 @@ from __future__ import annotations
 import time
 
 SPACE = (("solver", (0, 1)), ("block", (64, 128, 256)), ("remap", (0, 1, 2)), ("graph", (0, 1)),
-         ("persist", (0, 4, 8)), ("occ", (0, 1, 2, 3)), ("f1fuse", (0, 1)), ("f1cell", (0, 1)))
+         ("persist", (0, 4, 8, 16, 32)), ("occ", (0, 1, 2, 3)), ("f1fuse", (0, 1)), ("f1cell", (0, 1)), ("fuse", (0, 1)))
 DEFAULTS = {"solver": 1, "block": 128, "remap": 2, "graph": 0, "persist": 0, "occ": 0, "f1fuse": 0,
-            "f1cell": 0, "mlp_thr": None}
+            "f1cell": 0, "fuse": 0, "mlp_thr": None}
 OPTION_KEYS = tuple(DEFAULTS)
 # the plain form: the library's default launch with the reference's Brent solver
 PLAIN = dict(DEFAULTS, solver=0)
@@ -36,7 +37,8 @@ def apply(grid, cand: dict, surrogate=None):
     if cand.get("mlp_thr") is None and surrogate is not None:
         grid.load_surrogate(None)
     grid.set_options(solver=cand["solver"], block=cand["block"], remap=cand["remap"], graph=cand["graph"],
-                     persist=cand["persist"], occ=cand["occ"], f1fuse=cand["f1fuse"], f1cell=cand["f1cell"])
+                     persist=cand["persist"], occ=cand["occ"], f1fuse=cand["f1fuse"], f1cell=cand["f1cell"],
+                     fuse=cand["fuse"])
     if cand.get("mlp_thr") is not None:
         grid.load_surrogate(surrogate)
         grid.set_options(mlp_thr=cand["mlp_thr"])

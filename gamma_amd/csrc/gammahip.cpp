@@ -22,6 +22,8 @@
 #include <string>
 #include <vector>
 #include <algorithm>
+#include <map>
+#include <utility>
 
 #ifdef GH_EMU
 #define __global__
@@ -724,6 +726,21 @@ struct NoGuess {
   GH_HD double operator()(bool, double, double, double, double, int) const { return -1.; }
 };
 
+#ifdef GH_EMU
+// CPU build only (a test tool): records the update's cons2prim solves as rows
+// (D, |S|, tau, previous pressure, root, shocked, stage) for every stride-th
+// solve, until cap rows are taken (gammahip_emu_capture).
+struct C2PCapture { double* buf; long long cap, n, stride, seen; };
+C2PCapture g_c2p_capture = {nullptr, 0, 0, 1, 0};
+inline void capture_c2p(double D, double S, double tau, double pprev, double p, int shocked, int stage) {
+  C2PCapture& c = g_c2p_capture;
+  if (!c.buf || c.n >= c.cap) return;
+  if (c.seen++ % c.stride) return;
+  double* r = c.buf + 7 * c.n++;
+  r[0] = D; r[1] = S; r[2] = tau; r[3] = pprev; r[4] = p; r[5] = shocked; r[6] = stage;
+}
+#endif
+
 // GEO: the grid's geometry when fixed at compile time (2D grids have one;
 // -1: the track's, as 1D ensembles mix them)
 template <int ND, int NQ, int SV, class GuessT, int GEO = -1>
@@ -802,7 +819,13 @@ GH_HD void body_update_g(const Dev& d, int j, int i, int stage, bool valid, cons
     double D = 1., S = 0., tau = 1.;
     if (cell) c2p_inputs<ND, NQ>(U, r1, geom, &D, &S, &tau);
     double gs = guess(cell, D, S, tau, p[PPP], geom);
+#ifdef GH_EMU
+    double pprev = p[PPP];
+#endif
     if (cell) cons2prim<ND, NQ>(p, U, r1, geom, G, SV, gs, d.maxit, &st, d.P.synge);
+#ifdef GH_EMU
+    if (cell) capture_c2p(D, S, tau, pprev, p[PPP], d.shock ? d.shk[k] : 0, stage);
+#endif
   }
   int ncalls = 1;
   if (stage > 1) {
@@ -821,7 +844,13 @@ GH_HD void body_update_g(const Dev& d, int j, int i, int stage, bool valid, cons
     double D = 1., S = 0., tau = 1.;
     if (cell) c2p_inputs<ND, NQ>(U, cxn, geom, &D, &S, &tau);
     double gs = guess(cell, D, S, tau, p[PPP], geom);
+#ifdef GH_EMU
+    double pprev = p[PPP];
+#endif
     if (cell) cons2prim<ND, NQ>(p, U, cxn, geom, G, SV, gs, d.maxit, &st, d.P.synge);
+#ifdef GH_EMU
+    if (cell) capture_c2p(D, S, tau, pprev, p[PPP], d.shock ? d.shk[k] : 0, stage);
+#endif
     ncalls = 2;
   }
   if (!cell) return;
@@ -1312,6 +1341,23 @@ k_f1grad(Dev d, int ni, int remap) {
   extern __shared__ double f1cache[];
   double* cache = f1cache + threadIdx.x;
   GH_TILE_LOOP(PK, if (i < ni && j < d.nj) body_f1grad_c<NQ, GEO>(d, j, i, cache, (int)blockDim.x))
+}
+// Neighbour search and transverse gradient in one pass (launch option FUSE
+// bit 0): a cell's gradient (2d.cpp:366-430) reads only its own neighbour
+// lists (2d.cpp:534-606), so the two sweeps run back to back in one thread;
+// the search's dependent loads overlap the gradient's arithmetic in other
+// wavefronts, and the lists are read back by the thread that wrote them.
+// Every lane runs the search (its shuffles need the whole wavefront).
+template <bool PK, int NQ, int GEO>
+__global__ void __launch_bounds__(KB) __attribute__((amdgpu_waves_per_eu(1, 3)))
+k_nbgrad(Dev d, int ni, int remap) {
+  extern __shared__ double f1cache[];
+  double* cache = f1cache + threadIdx.x;
+  GH_TILE_LOOP(PK, {
+    bool valid = i < ni && j < d.nj;
+    body_nb_lanes(d, j, i, valid);
+    if (valid) body_f1grad_c<NQ, GEO>(d, j, i, cache, (int)blockDim.x);
+  })
 }
 // fused gradient + own-face pass (same LDS cache as k_f1grad)
 template <bool PK, int NQ>
@@ -1932,9 +1978,11 @@ struct gammahip_ctx {
   int block = 128;         // threads per workgroup of the per-cell kernels (64..256)
   int remap = 2;           // tile order: 0 hardware, 1 XCD bands, 2 track groups
   int use_graph = 0;       // replay steps from a captured hipGraph
-  int persist = 0;         // packed launch: persistent workgroups per CU (0: off)
+  int persist = 0;         // packed launch: at most this many persistent workgroups per CU (0: off),
+                           // fewer where the kernel's occupancy allows fewer (occ_blocks)
   int occ = 0;             // bit 0: F1 face kernel built for 4 wavefronts per SIMD; bit 1: update kernel for 5
   int f1cell = 0;          // fused transverse-gradient + own-face pass (k_f1cell) instead of k_f1grad + k_f1faces
+  int fuse = 0;            // bit 0: neighbour search inside the transverse-gradient pass (k_nbgrad)
   int ncu = 256;           // compute units of the device
 #ifndef GH_EMU
   hipStream_t stream = nullptr;
@@ -1948,6 +1996,8 @@ struct gammahip_ctx {
   hipGraphExec_t graph = nullptr;   // two steps (an even number of buffer swaps)
   void* mlp_w = nullptr;            // learned-guess parameters (gammahip_mlp_load)
   void* mlp_w2b = nullptr;
+  // workgroups per CU that can be resident, per (kernel, block size, LDS bytes)
+  std::map<std::pair<const void*, std::pair<int, size_t>>, int> occ_cache;
 #endif
 };
 
@@ -2023,6 +2073,25 @@ PlainLaunch plain_launch(const gammahip_ctx* c, int ntr, int ni) {
     return {dim3((ni + 63) / 64, (ntr + W - 1) / W), 2};
   return {dim3((ni + bs - 1) / bs, ntr), c->remap == 2 ? 0 : c->remap};
 }
+// Grid of a packed (persistent) launch of kernel f: c->persist workgroups per
+// CU, capped by how many the kernel's registers and LDS let one CU hold, so
+// that a large setting fills every CU without queueing workgroups behind
+// resident ones; a multiple of 8 (one band of tiles per XCD).
+dim3 persist_grid(gammahip_ctx* c, const void* f, int bs, size_t lds) {
+  auto key = std::make_pair(f, std::make_pair(bs, lds));
+  auto it = c->occ_cache.find(key);
+  int occ;
+  if (it != c->occ_cache.end()) {
+    occ = it->second;
+  } else {
+    occ = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, f, bs, lds) != hipSuccess || occ < 1) occ = 1;
+    c->occ_cache[key] = occ;
+  }
+  int n = c->persist < occ ? c->persist : occ;
+  if (n < 1) n = 1;
+  return dim3((unsigned)(n * c->ncu));
+}
 #define L2(c, kern, ntr, ni)                                                         \
   do {                                                                               \
     PlainLaunch pl_ = plain_launch((c), (ntr), (ni));                                \
@@ -2040,7 +2109,8 @@ PlainLaunch plain_launch(const gammahip_ctx* c, int ntr, int ni) {
   do {                                                                               \
     int bs_ = (c)->block;                                                            \
     if ((c)->d.packed) {                                                             \
-      hipLaunchKernelGGL((kern<true GH_UNPAREN targs>), dim3((c)->persist * (c)->ncu), dim3(bs_), 0, \
+      hipLaunchKernelGGL((kern<true GH_UNPAREN targs>),                               \
+                         persist_grid((c), (const void*)&kern<true GH_UNPAREN targs>, bs_, 0), dim3(bs_), 0, \
                          (c)->stream, (c)->d, (int)(ni), (c)->remap);                \
     } else {                                                                         \
       PlainLaunch pl_ = plain_launch((c), (ntr), (ni));                              \
@@ -2052,7 +2122,8 @@ PlainLaunch plain_launch(const gammahip_ctx* c, int ntr, int ni) {
   do {                                                                               \
     int bs_ = (c)->block;                                                            \
     if ((c)->d.packed) {                                                             \
-      hipLaunchKernelGGL((kern<true GH_UNPAREN targs>), dim3((c)->persist * (c)->ncu), dim3(bs_), 0, \
+      hipLaunchKernelGGL((kern<true GH_UNPAREN targs>),                               \
+                         persist_grid((c), (const void*)&kern<true GH_UNPAREN targs>, bs_, 0), dim3(bs_), 0, \
                          (c)->stream, (c)->d, (int)(ni), (c)->remap, (int)(st));     \
     } else {                                                                         \
       PlainLaunch pl_ = plain_launch((c), (ntr), (ni));                              \
@@ -2133,7 +2204,33 @@ struct Pipe {
     tend(c, 1);
     if (dt) L2P(c, body_dt, k_dt, (), d.nj, d.nax);
     if (ND == 2) {
-      L2P(c, body_nb, k_nb, (), d.nj, d.nax);
+      // FUSE bit 0: the neighbour search runs inside the gradient pass below
+      // (the slot scan after it needs only the lists)
+      bool nbgrad = (c->fuse & 1) && !d.pc && !c->f1cell;
+#ifdef GH_EMU
+      nbgrad = false;              // the CPU build runs the two bodies as separate sweeps
+#endif
+      if (nbgrad) {
+#ifndef GH_EMU
+        tbeg(c, 3);
+        int bs = c->block;
+        PlainLaunch pl = plain_launch(c, d.nj, d.nax);
+        size_t lds = sizeof(double) * bs * f1_kc<NQ>() * NQ;
+        bool sph = c->cfg.geom == SPH;
+        if (d.packed) {
+          if (sph) hipLaunchKernelGGL((k_nbgrad<true, NQ, SPH>), persist_grid(c, (const void*)&k_nbgrad<true, NQ, SPH>, bs, lds),
+                                      dim3(bs), lds, c->stream, d, d.nax, c->remap);
+          else hipLaunchKernelGGL((k_nbgrad<true, NQ, CART>), persist_grid(c, (const void*)&k_nbgrad<true, NQ, CART>, bs, lds),
+                                  dim3(bs), lds, c->stream, d, d.nax, c->remap);
+        } else {
+          if (sph) hipLaunchKernelGGL((k_nbgrad<false, NQ, SPH>), pl.g, dim3(bs), lds, c->stream, d, d.nax, pl.remap);
+          else hipLaunchKernelGGL((k_nbgrad<false, NQ, CART>), pl.g, dim3(bs), lds, c->stream, d, d.nax, pl.remap);
+        }
+        tend(c, 3);
+#endif
+      } else {
+        L2P(c, body_nb, k_nb, (), d.nj, d.nax);
+      }
 #ifdef GH_EMU
       for (int j = 0; j < d.nj; ++j) {
         int base = 0, nt = d.ntrack[j];
@@ -2164,33 +2261,35 @@ struct Pipe {
         int bs = c->block;
         PlainLaunch pl = plain_launch(c, d.nj, d.nax);
         size_t lds = sizeof(double) * bs * f1_kc<NQ>() * NQ;
-        if (d.packed) hipLaunchKernelGGL((k_f1cell<true, NQ>), dim3(c->persist * c->ncu), dim3(bs), lds, c->stream, d, d.nax, c->remap);
+        if (d.packed) hipLaunchKernelGGL((k_f1cell<true, NQ>), persist_grid(c, (const void*)&k_f1cell<true, NQ>, bs, lds),
+                                         dim3(bs), lds, c->stream, d, d.nax, c->remap);
         else hipLaunchKernelGGL((k_f1cell<false, NQ>), pl.g, dim3(bs), lds, c->stream, d, d.nax, pl.remap);
 #endif
         tend(c, 5);
       } else {
         // piecewise constant: the F1 faces take the cells' own states, the
         // transverse gradient is never read
-        tbeg(c, 3);
+        if (!nbgrad) tbeg(c, 3);
 #ifdef GH_EMU
         if (!d.pc) L2(c, body_f1grad<NQ>, d.nj, d.nax);
 #else
-        if (!d.pc) {
+        if (!d.pc && !nbgrad) {
           int bs = c->block;
-          dim3 g = dim3(c->persist * c->ncu);
           PlainLaunch pl = plain_launch(c, d.nj, d.nax);
           size_t lds = sizeof(double) * bs * f1_kc<NQ>() * NQ;
           bool sph = c->cfg.geom == SPH;
           if (d.packed) {
-            if (sph) hipLaunchKernelGGL((k_f1grad<true, NQ, SPH>), g, dim3(bs), lds, c->stream, d, d.nax, c->remap);
-            else hipLaunchKernelGGL((k_f1grad<true, NQ, CART>), g, dim3(bs), lds, c->stream, d, d.nax, c->remap);
+            if (sph) hipLaunchKernelGGL((k_f1grad<true, NQ, SPH>), persist_grid(c, (const void*)&k_f1grad<true, NQ, SPH>, bs, lds),
+                                        dim3(bs), lds, c->stream, d, d.nax, c->remap);
+            else hipLaunchKernelGGL((k_f1grad<true, NQ, CART>), persist_grid(c, (const void*)&k_f1grad<true, NQ, CART>, bs, lds),
+                                    dim3(bs), lds, c->stream, d, d.nax, c->remap);
           } else {
             if (sph) hipLaunchKernelGGL((k_f1grad<false, NQ, SPH>), pl.g, dim3(bs), lds, c->stream, d, d.nax, pl.remap);
             else hipLaunchKernelGGL((k_f1grad<false, NQ, CART>), pl.g, dim3(bs), lds, c->stream, d, d.nax, pl.remap);
           }
         }
 #endif
-        tend(c, 3);
+        if (!nbgrad) tend(c, 3);
         tbeg(c, 2);
 #ifdef GH_EMU
         for (int j = 0; j < d.nj; ++j)
@@ -2366,6 +2465,7 @@ struct DevBuf {
 int plain_fail(const char* m) { fprintf(stderr, "gammahip: %s\n", m); return -1; }
 #define CK(x) do { if ((x) != hipSuccess) return plain_fail(#x); } while (0)
 }  // namespace
+
 #endif
 
 extern "C" {
@@ -2636,6 +2736,9 @@ int gammahip_set_option(gammahip_ctx* c, int key, long long value) {
       return post(c);
     case GAMMAHIP_OPT_F1FUSE: c->d.f1fused = value ? 1 : 0; return 0;
     case GAMMAHIP_OPT_F1CELL: c->f1cell = value ? 1 : 0; return 0;
+    case GAMMAHIP_OPT_FUSE:
+      if (value < 0 || value > 1) return fail(c, "fuse option must be 0 or 1");
+      c->fuse = (int)value; return 0;
     case GAMMAHIP_OPT_OCC:
       if (value < 0 || value > 3) return fail(c, "occupancy option must be in 0..3");
       c->occ = (int)value; return 0;
@@ -2703,6 +2806,36 @@ int gammahip_collect_dt(gammahip_ctx* c, double* dt_out) {
   if (dispatch(c, [&](auto P) { P.collect_dt(c); }) || post(c)) return -1;
   if (dt_out) return d2h(c, dt_out, c->d.tdt, sizeof(double));
   return 0;
+}
+
+int gammahip_collect_dt_min(gammahip_ctx* c, double* min_out) {
+  if (!c || !min_out) return -1;
+  if (c->cfg.nd == 1 && c->d.nj != 1)
+    return fail(c, "collect_dt_min: a 1D context holding several runs has one minimum per run "
+                   "(use gammahip_collect_dt and gammahip_time)");
+  drop_graph(c);
+  if (dispatch(c, [&](auto P) { P.collect_dt(c); }) || post(c)) return -1;
+  std::vector<double> tm(c->d.nj);
+  if (d2h(c, tm.data(), c->d.tmin, sizeof(double) * c->d.nj)) return -1;
+  double m = 1.e15;
+  for (double v : tm) m = fmin(m, v);   // the order of a minimum does not change it
+  *min_out = m;
+  return 0;
+}
+
+int gammahip_evolve_dt(gammahip_ctx* c, double dt) {
+  if (!c) return -1;
+  if (c->cfg.nd == 1 && c->d.nj != 1)
+    return fail(c, "evolve_dt: a 1D context holding several runs advances each by its own dt "
+                   "(use gammahip_evolve)");
+  if (!(dt > 0.) || !(dt < 1.e300)) return fail(c, "evolve_dt: dt must be positive and finite");
+  drop_graph(c);
+#ifndef GH_EMU
+  HIPCHK(hipStreamSynchronize(c->stream));
+#endif
+  std::vector<double> v(c->d.nj, dt);
+  if (h2d(c, c->d.tdt, v.data(), sizeof(double) * c->d.nj)) return -1;
+  return gammahip_evolve(c);
 }
 
 int gammahip_sync(gammahip_ctx* c) {
@@ -3128,6 +3261,18 @@ int gammahip_mfma_probe(const float* A, const float* Bm, float* Dm) {
   return 0;
 #endif
 }
+
+#ifdef GH_EMU
+// CPU build only: capture the update's cons2prim solves into buf ([cap][7]
+// doubles, see C2PCapture), one in every `stride`; buf = null stops it.
+// Returns the rows written so far.
+long long gammahip_emu_capture(double* buf, long long cap, long long stride) {
+  long long n = g_c2p_capture.n;
+  if (buf) g_c2p_capture = {buf, cap, 0, stride > 0 ? stride : 1, 0};
+  else g_c2p_capture.buf = nullptr;
+  return n;
+}
+#endif
 
 int gammahip_kernel_times(gammahip_ctx* c, double* ms, long long* n) {
   c->timing = true;     // the first call switches event timing on

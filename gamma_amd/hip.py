@@ -24,7 +24,7 @@ F_GRADMV0, F_GRADF10 = 300, 400
 FACE_X, FACE_V, FACE_LL, FACE_LR, FACE_FLUX0 = 1000, 1001, 1002, 1003, 1010
 RESET_SHOCK, DETECT_SHOCK, INJECT, INIT_GAMMAS, ADVANCE_TIME = range(5)
 C2P_BRENT, C2P_NEWTON = 0, 1
-OPT_BLOCK, OPT_REMAP, OPT_GRAPH, OPT_MLP_THR, OPT_PERSIST, OPT_OCC, OPT_F1FUSE, OPT_SOLVER, OPT_F1CELL = range(9)
+OPT_BLOCK, OPT_REMAP, OPT_GRAPH, OPT_MLP_THR, OPT_PERSIST, OPT_OCC, OPT_F1FUSE, OPT_SOLVER, OPT_F1CELL, OPT_FUSE = range(10)
 
 
 class Config(C.Structure):
@@ -76,6 +76,7 @@ def load_library(path: str = LIB_PATH):
         "gammahip_step": (I, [P, I]), "gammahip_regrid": (I, [P]),
         "gammahip_update_ghosts": (I, [P]), "gammahip_prep_for_update": (I, [P]),
         "gammahip_collect_dt": (I, [P, P]), "gammahip_evolve": (I, [P]),
+        "gammahip_collect_dt_min": (I, [P, P]), "gammahip_evolve_dt": (I, [P, D]),
         "gammahip_apply": (I, [P, I]), "gammahip_download": (I, [P, I, P]),
         "gammahip_track_info": (I, [P, P, P, P, P]), "gammahip_dims": (I, [P, P, P]),
         "gammahip_time": (I, [P, P, P]), "gammahip_stats_get": (I, [P, C.POINTER(Stats)]),
@@ -210,8 +211,19 @@ class Grid:
         self._check(self.L.gammahip_collect_dt(self.h, _p(dt)))
         return float(dt[0])
 
-    def evolve(self):
-        self._check(self.L.gammahip_evolve(self.h))
+    def evolve(self, dt: float | None = None):
+        """Grid::evolve(it, t, dt): with dt given, the device advances by it
+        (gammahip_evolve_dt); without, by its own collected dt."""
+        if dt is None:
+            self._check(self.L.gammahip_evolve(self.h))
+        else:
+            self._check(self.L.gammahip_evolve_dt(self.h, float(dt)))
+
+    def collect_dt_min(self) -> float:
+        """Grid::collect_dt as the reference returns it: the minimum before CFL_."""
+        m = np.zeros(1)
+        self._check(self.L.gammahip_collect_dt_min(self.h, _p(m)))
+        return float(m[0])
 
     def apply(self, what: int):
         self._check(self.L.gammahip_apply(self.h, what))
@@ -224,17 +236,18 @@ class Grid:
         self._check(self.L.gammahip_sync(self.h))
 
     def set_options(self, block=None, remap=None, graph=None, mlp_thr=None, persist=None, occ=None,
-                    f1fuse=None, solver=None, f1cell=None):
+                    f1fuse=None, solver=None, f1cell=None, fuse=None):
         """Launch options: threads per workgroup; tile order (0 hardware, 1 XCD
         bands, 2 track groups); hipGraph replay; trigger exponent n of the
         learned guess (threshold 10^-n); persistent packed launch with n
         workgroups per CU or 0; register-capped builds (bit 0 the F1 face
         kernel, bit 1 the update kernel); F1 flux sums inside the update
         kernel; cons2prim solver (C2P_BRENT or C2P_NEWTON); fused
-        transverse-gradient and own-face F1 pass."""
+        transverse-gradient and own-face F1 pass; fused passes (bit 0: the F1
+        neighbour search inside the transverse-gradient pass)."""
         for key, val in ((OPT_SOLVER, solver), (OPT_BLOCK, block), (OPT_REMAP, remap), (OPT_GRAPH, graph),
                          (OPT_MLP_THR, mlp_thr), (OPT_PERSIST, persist), (OPT_OCC, occ),
-                         (OPT_F1FUSE, f1fuse), (OPT_F1CELL, f1cell)):
+                         (OPT_F1FUSE, f1fuse), (OPT_F1CELL, f1cell), (OPT_FUSE, fuse)):
             if val is not None:
                 self._check(self.L.gammahip_set_option(self.h, key, int(val)))
         if solver is not None:

@@ -115,6 +115,18 @@ int  gammahip_update_ghosts(gammahip_ctx* ctx);                /* Grid::updateGh
 int  gammahip_prep_for_update(gammahip_ctx* ctx);              /* Grid::prepForUpdate */
 int  gammahip_collect_dt(gammahip_ctx* ctx, double* dt_out);   /* Grid::collect_dt * CFL_, synchronous */
 int  gammahip_evolve(gammahip_ctx* ctx);                       /* Grid::evolve with the collected dt */
+/* Grid::collect_dt as the reference returns it (src/Dimensions/2d.cpp:954-971,
+ * 1d.cpp:779-795): the minimum of the cells' local time steps, before CFL_.
+ * The device still sets its own dt = CFL * minimum, so Simu::run's
+ * dt = CFL_ * collect_dt() (simu.cpp:61) gives the same bits.  One run per
+ * context (2D, or 1D with one track).  Synchronous. */
+int  gammahip_collect_dt_min(gammahip_ctx* ctx, double* min_out);
+/* Grid::evolve(it, t, dt) with the caller's dt (src/Timestep/rk3.cpp:13,
+ * 99, euler.cpp:12): dt replaces the device's dt of every track (a caller
+ * that clamps dt is followed), then the RK3 (or Euler) update runs.  One run
+ * per context; dt > 0 and finite.  Waits for the context's stream before
+ * copying dt. */
+int  gammahip_evolve_dt(gammahip_ctx* ctx, double dt);
 enum { GAMMAHIP_RESET_SHOCK = 0, GAMMAHIP_DETECT_SHOCK = 1, GAMMAHIP_INJECT = 2,
        GAMMAHIP_INIT_GAMMAS = 3, GAMMAHIP_ADVANCE_TIME = 4 };
 int  gammahip_apply(gammahip_ctx* ctx, int what);              /* Grid::apply(&Cell::...) */
@@ -167,7 +179,8 @@ int  gammahip_sync(gammahip_ctx* ctx);
  * a wavefront runs the network when one of its cells' previous pressures
  * leaves a residual |f| above 10^-n times the magnitude of f's terms
  * (default 3), and the packed launch: n > 0 runs the per-cell kernels as
- * persistent grids of n workgroups per CU walking a list of (track, chunk)
+ * persistent grids of at most n workgroups per CU (fewer where a kernel's
+ * registers or LDS let a CU hold fewer) walking a list of (track, chunk)
  * tiles that covers only each track's live cells (uneven tracks, ensembles),
  * 0 launches one workgroup per chunk of the nmax-wide track slots; OCC bit 0
  * runs the F1 face kernel in a build capped at 128 registers (4 wavefronts
@@ -182,7 +195,10 @@ int  gammahip_sync(gammahip_ctx* ctx);
  * in registers, instead of a gradient pass and a one-thread-per-face pass. */
 enum { GAMMAHIP_OPT_BLOCK = 0, GAMMAHIP_OPT_REMAP = 1, GAMMAHIP_OPT_GRAPH = 2,
        GAMMAHIP_OPT_MLP_THR = 3, GAMMAHIP_OPT_PERSIST = 4, GAMMAHIP_OPT_OCC = 5,
-       GAMMAHIP_OPT_F1FUSE = 6, GAMMAHIP_OPT_SOLVER = 7, GAMMAHIP_OPT_F1CELL = 8 };
+       GAMMAHIP_OPT_F1FUSE = 6, GAMMAHIP_OPT_SOLVER = 7, GAMMAHIP_OPT_F1CELL = 8,
+       GAMMAHIP_OPT_FUSE = 9 };
+/* GAMMAHIP_OPT_FUSE (bit set): bit 0 runs the F1 neighbour search inside the
+ * transverse-gradient pass (one kernel instead of two; 2D grids). */
 int  gammahip_set_option(gammahip_ctx* ctx, int key, long long value);
 
 /* ---- learned cons2prim guess (no counterpart in the reference) ----

@@ -39,6 +39,9 @@
 #include "gamma_oracle.h"
 #include "og_pow.h"
 #include <math.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 #include <stdlib.h>
 #include <string.h>
 #include <stdio.h>
@@ -94,6 +97,15 @@ struct og_grid {
 };
 
 #define CELL(g, j, i) (&(g)->C[(size_t)(j) * (g)->nax + (i)])
+/* counter increments from loops that run tracks in parallel (OpenMP) */
+/* tracks run in parallel only on grids large enough to repay a parallel region */
+static long og_par_min = 65536;
+#define og_par(g) ((long)(g)->nj * (g)->nax >= og_par_min)
+#ifdef _OPENMP
+#define OG_ADD(x, v) do { _Pragma("omp atomic") (x) += (v); } while (0)
+#else
+#define OG_ADD(x, v) ((x) += (v))
+#endif
 #define FACE(g, j, i) (&(g)->I[(size_t)(j) * ((g)->nax - 1) + (i)])
 
 /* ------------------------------------------------------------------------ */
@@ -278,7 +290,7 @@ static int fs_cons2prim(const og_config *c, FS *s, double r, long *cnt) {
   double pCand = sph ? fmax(1.e-13, s->prim[OG_PPP]) : s->prim[OG_PPP];
   int iters = 0;
   double f_init = c2p_f(c, pCand, D, S, E, tau);
-  if (cnt) cnt[0]++;
+  if (cnt) OG_ADD(cnt[0], 1);
   if (fabs(f_init) > 0.) {
     double p_lo = fmax(0., (1. + 1e-13) * S - E), p_hi;
     int find_p = 1;
@@ -290,17 +302,17 @@ static int fs_cons2prim(const og_config *c, FS *s, double r, long *cnt) {
       int guard = 0;
       while (c2p_f(c, p_hi, D, S, E, tau) > 0) {
         p_hi *= 10;
-        if (++guard > 400) { find_p = 0; if (cnt) cnt[6]++; break; }
+        if (++guard > 400) { find_p = 0; if (cnt) OG_ADD(cnt[6], 1); break; }
       }
     }
     if (p_hi < p_lo) find_p = 0;
     if (find_p) {
       int hitmax;
       pCand = brent_root(c, D, S, E, tau, p_lo, p_hi, c->c2p_maxiter, &iters, &hitmax);
-      if (cnt) { cnt[1] += iters; if (hitmax) cnt[3]++; }
+      if (cnt) { OG_ADD(cnt[1], iters); if (hitmax) OG_ADD(cnt[3], 1); }
     } else {
       iters = -1;
-      if (cnt) cnt[2]++;
+      if (cnt) OG_ADD(cnt[2], 1);
     }
   }
   double p = pCand;
@@ -697,10 +709,12 @@ static double sweep_r(const og_grid *g, const Cell *cl) {
 
 /* Grid::prim2cons / state2flux  1d.cpp:910-931, 2d.cpp:1117-1150 */
 static void grid_prim2cons(og_grid *g) {
+#pragma omp parallel for schedule(static) if(og_par(g))
   for (int j = 0; j < g->nj; ++j)
     for (int i = 0; i < g->ntrack[j]; ++i) { Cell *cl = CELL(g, j, i); fs_prim2cons(&g->c, &cl->S, sweep_r(g, cl)); }
 }
 static void grid_state2flux(og_grid *g) {
+#pragma omp parallel for schedule(static) if(og_par(g))
   for (int j = 0; j < g->nj; ++j)
     for (int i = 0; i < g->ntrack[j]; ++i) { Cell *cl = CELL(g, j, i); fs_state2flux(&g->c, &cl->S, sweep_r(g, cl)); }
 }
@@ -860,6 +874,7 @@ static void recon_mv(og_grid *g, int j, int i) {
 /* Grid::computeNeighbors  2d.cpp:534-606 (F1 lists; MV lists are unused) */
 static void grid_neighbors(og_grid *g) {
   if (g->c.nd != 2) return;
+#pragma omp parallel for schedule(dynamic, 1) if(og_par(g))
   for (int j = 0; j < g->nj; ++j) {
     int im = 0, ip = 0;
     for (int i = 1; i < g->ntrack[j] - 1; ++i) {
@@ -894,12 +909,14 @@ static void grid_neighbors(og_grid *g) {
 
 /* Grid::movDir_ComputeLambda  1d.cpp:731-739, 2d.cpp:835-846 */
 static void grid_mv_lambda(og_grid *g) {
+#pragma omp parallel for schedule(dynamic, 1) if(og_par(g))
   for (int j = 0; j < g->nj; ++j)
     for (int i = 0; i < g->ntrack[j] - 1; ++i) { recon_mv(g, j, i); face_lambda(&g->c, FACE(g, j, i)); }
 }
 
 /* Grid::updateKinematics  1d.cpp:741-751, 2d.cpp:848-861 */
 static void grid_kinematics(og_grid *g) {
+#pragma omp parallel for schedule(static) if(og_par(g))
   for (int j = 0; j < g->nj; ++j)
     for (int i = 0; i < g->ntrack[j] - 1; ++i) {
       Face *f = FACE(g, j, i);
@@ -911,7 +928,7 @@ static void grid_kinematics(og_grid *g) {
 }
 
 /* Grid::computeTransGradients  2d.cpp:366-430 */
-static void trans_gradients(og_grid *g, int j, int i) {
+static void trans_gradients(og_grid *g, int j, int i, double *gf) {
   const og_config *c = &g->c;
   Cell *c0 = CELL(g, j, i);
   double x0 = c0->G.x[MV];
@@ -948,14 +965,15 @@ static void trans_gradients(og_grid *g, int j, int i) {
       ind++;
     }
   }
-  for (int q = 0; q < c->nq; ++q) { gavg[q] /= dAtot; c0->grad_f1[q] = gavg[q]; }
+  for (int q = 0; q < c->nq; ++q) { gavg[q] /= dAtot; gf[q] = gavg[q]; }
   for (int n = 0; n < nn; ++n)
-    for (int q = 0; q < c->nq; ++q) c0->grad_f1[q] = minmod(c0->grad_f1[q], grads[n * QM + q]);
+    for (int q = 0; q < c->nq; ++q) gf[q] = minmod(gf[q], grads[n * QM + q]);
   free(grads);
 }
 
 /* Grid::reconstructStates, F1 branch  2d.cpp:491-527 */
-static void recon_f1(og_grid *g, int j, Cell *cL, Cell *cR, Face *f) {
+static void recon_f1(og_grid *g, int j, const Cell *cL, const Cell *cR, const double *gL, const double *gR,
+                     Face *f) {
   const og_config *c = &g->c;
   /* piecewise constant (2d.cpp:439-447): the cells' own states */
   if (c->recon == 1 || j == 0 || j == g->nj - 2 || cL->neigh[0].cnt == 0 || cR->neigh[1].cnt == 0) {
@@ -967,17 +985,43 @@ static void recon_f1(og_grid *g, int j, Cell *cL, Cell *cR, Face *f) {
   for (int q = 0; q < c->nq; ++q) {
     double qL = cL->S.prim[q] + cL->grad_mv[q] * dxmL;
     double qR = cR->S.prim[q] + cR->grad_mv[q] * dxmR;
-    f->SL.prim[q] = qL + cL->grad_f1[q] * dxfL;
-    f->SR.prim[q] = qR + cR->grad_f1[q] * dxfR;
+    f->SL.prim[q] = qL + gL[q] * dxfL;
+    f->SR.prim[q] = qR + gR[q] * dxfR;
   }
   fs_prim2cons(c, &f->SL, f->x[0]); fs_prim2cons(c, &f->SR, f->x[0]);
   fs_state2flux(c, &f->SL, f->x[0]); fs_state2flux(c, &f->SR, f->x[0]);
 }
 
-/* Grid::computeFluxes  1d.cpp:753-776, 2d.cpp:863-951 (single thread) */
+/* One F1 face of the loop below: its flux and, with shock detection, the
+ * forward and reverse shock measures it hands to its two cells. */
+typedef struct { double flux[QM]; double sdf, psf, sdr, psr; } F1Face;
+
+/* Interface::measureShock of an F1 face (radiation_sph.cpp:160-175) as
+ * values: compute_Sd forward (onto the lower cell) and reverse (onto the
+ * upper); a pspec of -inf leaves the cell's value. */
+static void f1_shock_values(const og_config *c, const Face *f, F1Face *r) {
+  r->psf = -INFINITY;
+  r->sdf = compute_sd(c, &f->SL, &f->SR, f->dim, &r->psf, 0);
+  r->psr = -INFINITY;
+  r->sdr = compute_sd(c, &f->SR, &f->SL, f->dim, &r->psr, 1);
+}
+
+/* Grid::computeFluxes  1d.cpp:753-776, 2d.cpp:863-951.
+ * MV faces: tracks in parallel (the reference's omp loop over j; a face
+ * writes only its own track).  F1 faces: the reference walks the tracks
+ * upward and, per cell, recomputes the transverse gradient and forms the
+ * faces of its upper list, summing their fluxes into the two cells.  The
+ * same bits come out of three passes: (1) every cell's new gradient into a
+ * side buffer (tracks in parallel); (2) every face (tracks in parallel),
+ * reading each cell's gradient as the upward walk finds it -- new for a cell
+ * the walk has reached (lower track, or the same track up to the face's own
+ * cell), old otherwise; (3) the sums into the cells and the shock measures in
+ * the walk's order, then the new gradients committed. */
 static int grid_fluxes(og_grid *g) {
   const og_config *c = &g->c;
   int nq = c->nq;
+  int err = 0;
+#pragma omp parallel for schedule(dynamic, 1) reduction(|:err) if(og_par(g))
   for (int j = 0; j < g->nj; ++j) {
     int nt = g->ntrack[j];
     for (int i = 0; i < nt - 1; ++i) {
@@ -987,28 +1031,51 @@ static int grid_fluxes(og_grid *g) {
     }
     for (int i = 1; i < nt - 1; ++i) {
       Cell *cl = CELL(g, j, i);
-      if (cell_update_dt_mv(cl, FACE(g, j, i - 1), FACE(g, j, i))) return -30;
+      if (cell_update_dt_mv(cl, FACE(g, j, i - 1), FACE(g, j, i))) { err = 1; break; }
       for (int q = 0; q < nq; ++q) {
         cl->flux[0][MV][q] = FACE(g, j, i - 1)->flux[q];
         cl->flux[1][MV][q] = FACE(g, j, i)->flux[q];
       }
     }
   }
+  if (err) return -30;
   if (c->nd != 2) return 0;
+#pragma omp parallel for schedule(static) if(og_par(g))
   for (int j = 0; j < g->nj; ++j)
     for (int i = 0; i < g->ntrack[j]; ++i)
       for (int s = 0; s < 2; ++s)
         for (int q = 0; q < nq; ++q) CELL(g, j, i)->flux[s][F1][q] = 0.;
-  for (int j = 0; j < g->nj - 1; ++j) {
+  int nj1 = g->nj - 1;
+  size_t ncell = (size_t)g->nj * g->nax;
+  long *off = (long *)malloc(sizeof(long) * (size_t)(nj1 + 1));
+  off[0] = 0;
+  for (int j = 0; j < nj1; ++j) {
+    long n = 0;
+    for (int i = 0; i < g->ntrack[j]; ++i) n += CELL(g, j, i)->neigh[1].cnt;
+    off[j + 1] = off[j] + n;
+  }
+  F1Face *fb = (F1Face *)malloc(sizeof(F1Face) * (size_t)(off[nj1] > 0 ? off[nj1] : 1));
+  double *gnew = (double *)malloc(sizeof(double) * QM * ncell);
+  /* (1) */
+#pragma omp parallel for schedule(dynamic, 1) if(og_par(g))
+  for (int j = 0; j < nj1; ++j)
+    for (int i = 0; i < g->ntrack[j]; ++i) trans_gradients(g, j, i, gnew + QM * ((size_t)j * g->nax + i));
+  /* (2) */
+#pragma omp parallel for schedule(dynamic, 1) if(og_par(g))
+  for (int j = 0; j < nj1; ++j) {
     Cell *ce = CELL(g, j, g->iLbnd[j] + 1);
     double jpos = ce->G.x[F1] + ce->G.dx[F1] / 2.;
+    long k = off[j];
     for (int i = 0; i < g->ntrack[j]; ++i) {
       Cell *c0 = CELL(g, j, i);
+      const double *g0 = gnew + QM * ((size_t)j * g->nax + i);
       double x0 = c0->G.x[MV];
       double xL0 = x0 - c0->G.dx[MV] / 2., xR0 = x0 + c0->G.dx[MV] / 2.;
-      trans_gradients(g, j, i);
-      for (int n = 0; n < c0->neigh[1].cnt; ++n) {
-        Cell *cn = CELL(g, c0->neigh[1].j, c0->neigh[1].lo + n);
+      for (int n = 0; n < c0->neigh[1].cnt; ++n, ++k) {
+        int jn = c0->neigh[1].j, in = c0->neigh[1].lo + n;
+        Cell *cn = CELL(g, jn, in);
+        int reached = jn < j || (jn == j && in <= i);
+        const double *gn = reached ? gnew + QM * ((size_t)jn * g->nax + in) : cn->grad_f1;
         double xn = cn->G.x[MV];
         double xLn = xn - cn->G.dx[MV] / 2., xRn = xn + cn->G.dx[MV] / 2.;
         Face Int; face_ctor(&Int, F1);
@@ -1017,17 +1084,43 @@ static int grid_fluxes(og_grid *g) {
         Int.x[MV] = (fmax(xL0, xLn) + fmin(xR0, xRn)) / 2.;
         Int.dx = fmax(0, fmin(xR0, xRn) - fmax(xL0, xLn));
         face_dA(c, &Int);
-        recon_f1(g, j, c0, cn, &Int);
+        recon_f1(g, j, c0, cn, g0, gn, &Int);
         face_lambda(c, &Int);
         face_flux(c, &Int);
-        if (c->shock) face_measure_shock(c, &Int, c0, cn);
+        for (int q = 0; q < nq; ++q) fb[k].flux[q] = Int.flux[q];
+        if (c->shock) f1_shock_values(c, &Int, &fb[k]);
+      }
+    }
+  }
+  /* (3) */
+  for (int j = 0; j < nj1; ++j) {
+    long k = off[j];
+    for (int i = 0; i < g->ntrack[j]; ++i) {
+      Cell *c0 = CELL(g, j, i);
+      for (int n = 0; n < c0->neigh[1].cnt; ++n, ++k) {
+        Cell *cn = CELL(g, c0->neigh[1].j, c0->neigh[1].lo + n);
+        if (c->shock) {
+          c0->Sd = fmax(c0->Sd, fb[k].sdf);
+          if (fb[k].psf > c0->pspec) c0->pspec = fb[k].psf;
+          cn->Sd = fmax(cn->Sd, fb[k].sdr);
+          if (fb[k].psr > cn->pspec) cn->pspec = fb[k].psr;
+        }
         for (int q = 0; q < nq; ++q) {
-          c0->flux[1][F1][q] += Int.flux[q];
-          cn->flux[0][F1][q] += Int.flux[q];
+          c0->flux[1][F1][q] += fb[k].flux[q];
+          cn->flux[0][F1][q] += fb[k].flux[q];
         }
       }
     }
   }
+#pragma omp parallel for schedule(static) if(og_par(g))
+  for (int j = 0; j < nj1; ++j)
+    for (int i = 0; i < g->ntrack[j]; ++i) {
+      Cell *c0 = CELL(g, j, i);
+      for (int q = 0; q < nq; ++q) c0->grad_f1[q] = gnew[QM * ((size_t)j * g->nax + i) + q];
+    }
+  free(gnew);
+  free(fb);
+  free(off);
   return 0;
 }
 
@@ -1046,6 +1139,7 @@ static int grid_prep(og_grid *g) {
 /* Grid::collect_dt  1d.cpp:779-795, 2d.cpp:954-971 */
 static double grid_collect_dt(og_grid *g) {
   double local_dt = 1.e15;
+#pragma omp parallel for schedule(static) reduction(min:local_dt) if(og_par(g))
   for (int j = 0; j < g->nj; ++j)
     for (int i = 0; i < g->ntrack[j]; ++i) {
       Cell *cl = CELL(g, j, i);
@@ -1061,6 +1155,7 @@ static int active_track(const og_grid *g, int j) {
 
 /* Grid::update  1d.cpp:798-810, 2d.cpp:974-996 ; Interface::move interface.cpp:24-30 */
 static void grid_update(og_grid *g, double dt) {
+#pragma omp parallel for schedule(static) if(og_par(g))
   for (int j = 0; j < g->nj; ++j) {
     if (!active_track(g, j)) continue;
     for (int i = 0; i < g->ntrack[j] - 1; ++i) {
@@ -1069,11 +1164,12 @@ static void grid_update(og_grid *g, double dt) {
       face_dA(&g->c, f);
     }
   }
+#pragma omp parallel for schedule(dynamic, 1) if(og_par(g))
   for (int j = 0; j < g->nj; ++j) {
     if (!active_track(g, j)) continue;
     for (int i = g->iLbnd[j] + 1; i <= g->iRbnd[j] - 1; ++i) {
       double xL = FACE(g, j, i - 1)->x[MV], xR = FACE(g, j, i)->x[MV];
-      if (xR < xL) g->cnt[7]++;
+      if (xR < xL) OG_ADD(g->cnt[7], 1);
       cell_update(g, CELL(g, j, i), dt, xL, xR);
     }
   }
@@ -1081,6 +1177,7 @@ static void grid_update(og_grid *g, double dt) {
 
 /* Grid::copyState0  1d.cpp:813-828, 2d.cpp:999-1020 */
 static void grid_copy_state0(og_grid *g) {
+#pragma omp parallel for schedule(static) if(og_par(g))
   for (int j = 0; j < g->nj; ++j) {
     for (int i = 0; i < g->ntrack[j]; ++i) { Cell *cl = CELL(g, j, i); cl->S0 = cl->S; cl->G0 = cl->G; }
     for (int i = 0; i < g->ntrack[j] - 1; ++i) {
@@ -1093,6 +1190,7 @@ static void grid_copy_state0(og_grid *g) {
 
 /* Grid::CellGeomFromInterfacePos  1d.cpp:831-841, 2d.cpp:1023-1036 */
 static void grid_cellgeom_from_faces(og_grid *g) {
+#pragma omp parallel for schedule(static) if(og_par(g))
   for (int j = 0; j < g->nj; ++j)
     for (int i = 1; i < g->ntrack[j] - 1; ++i)
       cell_move(&g->c, CELL(g, j, i), FACE(g, j, i - 1)->x[MV], FACE(g, j, i)->x[MV]);
@@ -1101,6 +1199,7 @@ static void grid_cellgeom_from_faces(og_grid *g) {
 /* RK3 stage average  rk3.cpp:27-55 / 63-92 (1D), 113-149 / 156-192 (2D) */
 static void grid_rk_average(og_grid *g, double w0, double w1) {
   const og_config *c = &g->c;
+#pragma omp parallel for schedule(static) if(og_par(g))
   for (int j = 0; j < g->nj; ++j) {
     if (!active_track(g, j)) continue;
     for (int i = g->iLbnd[j] + 1; i <= g->iRbnd[j] - 1; ++i) {
@@ -1110,6 +1209,7 @@ static void grid_rk_average(og_grid *g, double w0, double w1) {
         cl->S.cons[q] = w0 * cl->S0.cons[q] * dV0 + w1 * cl->S.cons[q] * dV;
     }
   }
+#pragma omp parallel for schedule(static) if(og_par(g))
   for (int j = 0; j < g->nj; ++j) {
     if (!active_track(g, j)) continue;
     for (int i = g->iLbnd[j]; i <= g->iRbnd[j] - 1; ++i) {
@@ -1119,6 +1219,7 @@ static void grid_rk_average(og_grid *g, double w0, double w1) {
     }
   }
   grid_cellgeom_from_faces(g);
+#pragma omp parallel for schedule(dynamic, 1) if(og_par(g))
   for (int j = 0; j < g->nj; ++j) {
     if (!active_track(g, j)) continue;
     for (int i = g->iLbnd[j] + 1; i <= g->iRbnd[j] - 1; ++i) {
@@ -1370,6 +1471,7 @@ static void grid_detect_shock(og_grid *g) {        /* radiation_sph.cpp:177-181 
 /* radiation_injectParticles over every allocated cell (1d.cpp:878-886);
  * cells beyond ntrack are never read again before being overwritten. */
 static void grid_inject(og_grid *g) {
+#pragma omp parallel for schedule(static) if(og_par(g))
   for (int j = 0; j < g->nj; ++j)
     for (int i = 0; i < g->ntrack[j]; ++i) cell_inject(&g->c, CELL(g, j, i));
 }
@@ -1520,6 +1622,23 @@ static int og_step(og_grid *g) {
   g->it++;
   if (g->c.sync) grid_inject(g);
   return 0;
+}
+
+/* threads of the track-parallel loops (OpenMP; 1 without it) */
+int og_threads(void) {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
+void og_set_par_min(long cells) { og_par_min = cells; }
+void og_set_threads(int n) {
+#ifdef _OPENMP
+  if (n > 0) omp_set_num_threads(n);
+#else
+  (void)n;
+#endif
 }
 
 int og_run(og_grid *g, int nsteps) {

@@ -91,6 +91,12 @@ int      og_load(og_grid *g, const double *x, const double *dx,
 int      og_init(og_grid *g, double t0);
 /* nsteps iterations of the Simu::run loop body; returns 0 or an error code */
 int      og_run(og_grid *g, int nsteps);
+/* threads the track loops run on (OpenMP), and setting them (results do not
+ * depend on the count) */
+int      og_threads(void);
+void     og_set_threads(int n);
+/* grids of fewer cells (tracks x slots) than this run their loops on one thread */
+void     og_set_par_min(long cells);
 
 /* state access */
 int      og_ntracks(const og_grid *g);     /* nde_nax[F1] (1 in 1D) */

@@ -88,8 +88,11 @@ def lib(path: str | None = None):
             "og_prep": (I, [P]), "og_update_ghosts": (I, [P]), "og_get_neighbors": (V, [P, I, I, P]),
             "og_split": (V, [P, I, I]), "og_merge": (V, [P, I, I]), "og_regrid_decision": (I, [P, I, I]),
             "og_record_stages": (I, [P, I]), "og_get_stage": (I, [P, I, I, P]),
+            "og_threads": (I, []), "og_set_threads": (V, [I]), "og_set_par_min": (V, [C.c_long]),
         }
         for name, (res, args) in sig.items():
+            if not hasattr(L, name):
+                continue
             fn = getattr(L, name)
             fn.restype = res
             fn.argtypes = args

@@ -34,7 +34,7 @@ from test_parity import TOL, reference_spread
 def _same(a, b, fields):
     assert np.array_equal(a.tracks()["ntrack"], b.tracks()["ntrack"]), "track lengths differ"
     for f in fields:
-        assert np.array_equal(a.cells(f), b.cells(f)), f"field {f} differs"
+        assert np.array_equal(a.cells(f), b.cells(f), equal_nan=True), f"field {f} differs"
     np.testing.assert_array_equal(a.time()[0], b.time()[0])
 
 

@@ -516,3 +516,31 @@ def test_planted_mistakes_are_caught(n, mutant_dir):
     path = _mutant(mutant_dir, n)
     with pytest.raises(AssertionError):
         check(path)
+
+
+# ---------------------------------------------------------------- threads
+def test_oracle_threads_do_not_change_bits():
+    """The oracle's track loops run in parallel (OpenMP) where the reference's
+    do (2d.cpp omp loops over tracks); its F1 loop keeps the reference's
+    upward walk by computing gradients and faces in passes and summing in the
+    walk's order.  One thread and all threads must give the same bits: 2D
+    Riemann problem with regrid, 2D spherical jet with shock detection and
+    synchrotron, and the 1D blast wave."""
+    L = oc.lib()
+    nmax = L.og_threads()
+    cases = [(S.riemann_2d(96, 12), 25), (S.bm_jet_2d(10, 0.3, 40, 120), 25),
+             (S.blandford_mckee_1d(300), 40)]
+    L.og_set_par_min(0)                  # parallel loops on these small grids too
+    try:
+        for s, n in cases:
+            runs = []
+            for th in (1, max(nmax, 4)):
+                L.og_set_threads(th)
+                o = oc.OracleGrid(s)
+                o.run(n)
+                runs.append([o.cells(f) for f in range(s.cfg.nq)] + [o.faces(0)] + [o.tracks()["ntrack"]])
+            for a, b in zip(*runs):
+                assert np.array_equal(a, b, equal_nan=True)
+    finally:
+        L.og_set_threads(nmax)
+        L.og_set_par_min(65536)

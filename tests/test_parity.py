@@ -372,8 +372,11 @@ def test_gpu_pow_equals_oracle_bitwise():
     dict(block=256, remap=2, graph=1, occ=3),
     dict(block=128, remap=2, f1fuse=1, f1cell=1),
     dict(block=64, remap=0),
+    dict(block=128, remap=2, fuse=1, f1fuse=1),
+    dict(block=256, remap=1, persist=16, graph=1, fuse=1),
 ], ids=["remap-graph", "packed", "packed-remap-graph-capped", "f1cell", "packed-f1cell-f1fuse",
-        "track-groups-graph-capped", "track-groups-f1fuse-f1cell", "hardware-order"])
+        "track-groups-graph-capped", "track-groups-f1fuse-f1cell", "hardware-order", "fused-nbgrad",
+        "packed-fused-nbgrad-graph"])
 def test_gpu_parity_launch_options(case, opts):
     """Launch options change where work runs, never the arithmetic: the
     trajectory under every option matches the oracle as the default launch does."""
