@@ -23,6 +23,7 @@
 #include <vector>
 #include <algorithm>
 #include <map>
+#include <mutex>
 #include <utility>
 
 #ifdef GH_EMU
@@ -1875,6 +1876,19 @@ __global__ void __launch_bounds__(64) k_mlp_infer(const float* __restrict__ w, c
   if (valid) out[s] = pressure_of(w, D, y);
 }
 
+// B operands of the hidden layer packed per lane on the device, the layout of
+// pack_w2 (one thread per (k-block, n-block, lane), 8 values each), from
+// parameters already in device memory
+__global__ void __launch_bounds__(512) k_mlp_pack_w2(const float* __restrict__ w, __bf16* __restrict__ out) {
+  int t = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (t >= 8 * 64) return;
+  int l = t % 64, blk = t / 64, kb = blk / 2, nb = blk % 2;
+  for (int e = 0; e < 8; ++e) {
+    int n = nb * 32 + l % 32, k = kb * 16 + 8 * (l / 32) + e;
+    out[t * 8 + e] = (__bf16)w[OW2 + n * H + k];
+  }
+}
+
 __global__ void k_c2p_batch(const double* __restrict__ st, const double* __restrict__ guess,
                             const double* __restrict__ guess2, long long n, double G, int solver, int maxit,
                             double* __restrict__ p_out, int* __restrict__ iters, int* __restrict__ fail,
@@ -2466,6 +2480,36 @@ int plain_fail(const char* m) { fprintf(stderr, "gammahip: %s\n", m); return -1;
 #define CK(x) do { if ((x) != hipSuccess) return plain_fail(#x); } while (0)
 }  // namespace
 
+// Device memory of gammahip_mlp_infer_noalloc, in place of the buffers
+// gammahip_mlp_infer allocates per call: the hidden layer's packed BF16 B
+// operands (8 x 64 lanes x 8 values = 8 KiB).  Each device holds its own copy
+// until the process ends.  In host code `scratch` names a host-side stand-in:
+// kernels get the device address from noalloc_scratch() instead.
+constexpr size_t NOALLOC_SCRATCH_BYTES = 16384;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+
+namespace {
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex noalloc_mu;
+void* noalloc_ptr[NOALLOC_MAX_DEVICES];
+
+// device address of the current device's copy of `scratch`, looked up once per
+// device (with that device current) and kept
+int noalloc_scratch(void** out) {
+  int dev = -1;
+  CK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) return plain_fail("noalloc: device index out of range");
+  std::lock_guard<std::mutex> lock(noalloc_mu);
+  if (!noalloc_ptr[dev]) {
+    void* p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) return plain_fail("noalloc: hipGetSymbolAddress(scratch) failed");
+    noalloc_ptr[dev] = p;
+  }
+  *out = noalloc_ptr[dev];
+  return 0;
+}
+}  // namespace
 #endif
 
 extern "C" {
@@ -3175,6 +3219,31 @@ int gammahip_mlp_infer(const float* params, const double* states, long long n, d
   hipLaunchKernelGGL(k_mlp_infer, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, 0, w, w2b, st, n, out, mfma);
   CK(hipGetLastError());
   CK(hipMemcpy(p_out, out, sizeof(double) * n, hipMemcpyDeviceToHost));
+  return 0;
+#endif
+}
+
+int gammahip_mlp_infer_noalloc(const float* params, const double* states, long long n, double* p_out, int mfma,
+                               void* stream) {
+#ifdef GH_EMU
+  (void)params; (void)states; (void)n; (void)p_out; (void)mfma; (void)stream;
+  return -1;
+#else
+  using namespace gh::mlp;
+  constexpr size_t need = sizeof(bf16x8) * 8 * 64;
+  static_assert(need <= NOALLOC_SCRATCH_BYTES, "packed operands exceed the scratch array");
+  if (!params || !states || !p_out) return plain_fail("mlp_infer_noalloc: null device pointer");
+  if (mfma != 0 && mfma != 1) return plain_fail("mlp_infer_noalloc: mfma must be 0 or 1");
+  if (n < 1 || (n + 63) / 64 > (long long)INT_MAX)
+    return plain_fail("mlp_infer_noalloc: n must be in [1, 64 * INT_MAX]");
+  void* sp = nullptr;
+  if (noalloc_scratch(&sp)) return -1;
+  hipStream_t s = (hipStream_t)stream;
+  bf16x8* w2b = (bf16x8*)sp;
+  hipLaunchKernelGGL(k_mlp_pack_w2, dim3(1), dim3(512), 0, s, params, (__bf16*)w2b);
+  hipLaunchKernelGGL(k_mlp_infer, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, s, params, w2b, states, n, p_out,
+                     mfma);
+  CK(hipGetLastError());
   return 0;
 #endif
 }

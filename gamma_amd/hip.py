@@ -86,6 +86,7 @@ def load_library(path: str = LIB_PATH):
         "gammahip_mlp_train": (I, [P, C.c_longlong, P, I, I, C.c_float, C.c_ulonglong, P]),
         "gammahip_mlp_grad": (I, [P, P, I, P, P]),
         "gammahip_mlp_infer": (I, [P, P, C.c_longlong, P, I]),
+        "gammahip_mlp_infer_noalloc": (I, [P, P, C.c_longlong, P, I, P]),
         "gammahip_mfma_probe": (I, [P, P, P]),
         "gammahip_c2p_batch": (I, [P, P, P, C.c_longlong, D, I, I, P, P, P, P]),
         "gammahip_mlp_load": (I, [P, P]),

@@ -215,6 +215,18 @@ int  gammahip_mlp_train(const double* states, long long n, float* params, int st
 int  gammahip_mlp_grad(const float* params, const double* states, int n, float* grad, float* loss);
 int  gammahip_mlp_infer(const float* params, const double* states, long long n, double* p_out,
                         int mfma);
+/* gammahip_mlp_infer without any allocation (no counterpart in the reference;
+ * for callers that cannot allocate device memory: code under HIP graph
+ * capture, tight serving loops).  params [gammahip_mlp_nparams()], states
+ * [n][3] and p_out [n] are DEVICE pointers; the call is asynchronous on
+ * stream (a hipStream_t; null = the default stream) and returns the bits of
+ * gammahip_mlp_infer.  The packed BF16 hidden-layer operands go into a fixed
+ * 16 KiB device array of the library (one per device, held until the process
+ * ends; 8 KiB used, whatever n), fully rewritten by every call: calls that
+ * overlap on one device race on it.  Rejects null pointers, mfma not 0/1 and
+ * n outside [1, 64 * INT_MAX] with -1 and a message on stderr. */
+int  gammahip_mlp_infer_noalloc(const float* params, const double* states, long long n, double* p_out,
+                                int mfma, void* stream);
 int  gammahip_mfma_probe(const float* A /* 32x16 */, const float* B /* 16x32 */, float* D /* 32x32 */);
 /* Cartesian cons2prim of raw states [n][3] = (D, |S|, tau) started from
  * guess[n] (FluidState::cons2prim, src/Hydro/rel_cart.cpp:125-229, with the
