@@ -52,7 +52,13 @@ def capture(setup, nsteps: int, stride: int = 1, cap: int = 1 << 20, warmup: int
     buf = np.zeros((cap, 7))
     L.gammahip_emu_capture(buf.ctypes.data, cap, stride)
     try:
-        g.step(nsteps)
+        done = 0
+        while done < nsteps:              # in chunks, reporting progress on long runs
+            k = min(100, nsteps - done)
+            g.step(k)
+            done += k
+            if nsteps > 100:
+                print(f"[c2p starts] captured {done}/{nsteps} steps", file=sys.stderr, flush=True)
     finally:
         n = L.gammahip_emu_capture(None, 0, 0)
     g.close()
@@ -74,15 +80,30 @@ def starts(rows: np.ndarray, gamma: float, w=None, maxit: int = 100) -> dict:
     ok = np.ones(len(rows), dtype=bool)
     for r in runs.values():
         ok &= r["fail"] == 0
+    # the update kernel's trigger: a wavefront asks the network only where the
+    # previous pressure leaves |f| above 10^-n of the magnitude of f's terms
+    from mlp_ref import c2p_residual
+    f, _, fs = c2p_residual(prev, st[:, 0], st[:, 1], st[:, 2], gamma)
+    rel = np.abs(f) / fs
     out = {"states": int(len(rows)), "shocked_states": int(shocked.sum())}
+    mean = lambda a, m: float(a[m].mean()) if m.any() else None
     for k, r in runs.items():
-        e = {"evals_per_solve": float(r["iters"][ok].mean()) if ok.any() else None,
-             "failures": int((r["fail"] != 0).sum())}
-        m = ok & shocked
-        e["evals_per_solve_shocked"] = float(r["iters"][m].mean()) if m.any() else None
+        e = {"evals_per_solve": mean(r["iters"], ok), "failures": int((r["fail"] != 0).sum()),
+             "evals_per_solve_shocked": mean(r["iters"], ok & shocked)}
+        for n in (2, 3, 4):
+            e[f"evals_per_solve_where_trigger_{n}"] = mean(r["iters"], ok & (rel > 10.0 ** -n))
         if k == "network_judged":
-            e["network_won"] = float(r["used"][ok].mean()) if ok.any() else None
+            e["network_won"] = mean(r["used"].astype(float), ok)
         out[k] = e
+    for n in (2, 3, 4):
+        t = rel > 10.0 ** -n
+        out[f"trigger_{n}_fraction"] = float(t.mean())
+        if "network_judged" in runs:
+            # the kernel's scheme: the judged start where the trigger fires,
+            # the previous pressure elsewhere
+            ev = np.where(t, runs["network_judged"]["iters"], runs["previous_p"]["iters"])
+            out[f"triggered_{n}"] = {"evals_per_solve": mean(ev, ok),
+                                     "evals_per_solve_shocked": mean(ev, ok & shocked)}
     # every start must land on the same root (Newton's answer does not depend on it)
     ref = runs["previous_p"]["p"]
     out["same_root"] = bool(all(np.array_equal(r["p"][ok], ref[ok]) for r in runs.values()))
