@@ -275,6 +275,8 @@ def main():
     ap.add_argument("--no-other", action="store_true",
                     help="skip the uneven-track configurations")
     ap.add_argument("--other-steps", type=int, default=100)
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run the 2 GiB device copy whose counters calibrate the traffic figures")
     ap.add_argument("--profile-config", choices=("bm_jet_2d", "ensemble_1d"), default=None,
                     help="run only that configuration's plain form for --steps steps (for profilers)")
     args = ap.parse_args()
@@ -284,6 +286,9 @@ def main():
                                    "designed, not built", "n_gpus": args.gpus}))
         sys.exit(2)
 
+    if args.copy_probe:
+        exec(COPY_PROBE, {})
+        return
     from gamma_amd import autotune, hip, setups
     if args.profile_config:
         s = setups.bm_jet_2d() if args.profile_config == "bm_jet_2d" else setups.ensemble_1d()[0]
