@@ -86,6 +86,7 @@ struct Dev {
   double *ff1dn, *ff1sddn;
   double *f1net;           // [NQ][nj][nax] net F1 flux (dn - up) of each cell, this stage
   int f1fused;             // 1: the update kernel sums the F1 fluxes itself (no k_f1sum pass)
+  int gmvre;               // 1: the F1 passes recompute MV gradients from the primitives (gmv_at)
   unsigned long long* cnt;
   long long *nsplit, *nmerge;
   // learned cons2prim guess: FP32 parameters and the packed BF16 hidden-layer
@@ -418,6 +419,35 @@ GH_HD void body_nb(const Dev& d, int j, int i) {
   }
 }
 
+// MV gradient of cell (j, m) as this stage's MV face pass left it (2D): face m
+// stores gLs for cell m (1 <= m <= nt-3), face nt-3 stores gRs for cell nt-2
+// -- the same minmod of the two one-sided slopes, whose arguments commute.
+// With d.gmvre the F1 passes recompute it from the primitives they read
+// anyway instead of loading the stored plane; cells 0 and nt-1 (and every
+// cell under piecewise-constant reconstruction or on tracks shorter than 4)
+// keep the value their ghost copy carried, which is read.
+template <int NQ, int GEO = -1>
+GH_HD void gmv_at(const Dev& d, int j, int m, double* g) {
+  long long k = CI(d, j, m);
+  int nt = d.ntrack[j];
+  if (!d.gmvre || d.pc || nt < 4 || m < 1 || m > nt - 2) {
+    load_v<2, NQ>(d, d.gmv, k, g);
+    return;
+  }
+  int geom = GEO >= 0 ? GEO : d.tgeom[j];
+  double pa[NQ], pb[NQ], pc[NQ];
+  load_v<2, NQ>(d, d.prim, k - 1, pa);
+  load_v<2, NQ>(d, d.prim, k, pb);
+  load_v<2, NQ>(d, d.prim, k + 1, pc);
+  double xa = geo_of_g<2>(d, j, k - 1, geom).cen0, xb = geo_of_g<2>(d, j, k, geom).cen0;
+  double xc = geo_of_g<2>(d, j, k + 1, geom).cen0;
+  for (int q = 0; q < NQ; ++q) {
+    double gl = (pb[q] - pa[q]) / (xb - xa);
+    double gn = (pc[q] - pb[q]) / (xc - xb);
+    g[q] = minmod(gl, gn);
+  }
+}
+
 // Grid::computeTransGradients (2d.cpp:366-430) of cell (j,i) into gf.  The
 // second pass (minmod against every neighbour's gradient) reuses the
 // gradients of the first KC neighbours from cache (KC x NQ doubles with
@@ -457,7 +487,7 @@ GH_HD void trans_grad(const Dev& d, int j, long long k, const double* p0, const 
         double dxmL = xm - g0.cen0, dxmR = xm - gn.cen0;
         double pn[NQ], mn[NQ];
         load_v<2, NQ>(d, d.prim, kn, pn);
-        load_v<2, NQ>(d, d.gmv, kn, mn);
+        gmv_at<NQ, GEO>(d, tj, lo + n, mn);
         if (pass == 0) dAtot += dA;
         for (int q = 0; q < NQ; ++q) {
           double qL = p0[q] + m0[q] * dxmL;
@@ -493,7 +523,7 @@ GH_HD void body_f1grad_c(const Dev& d, int j, int i, double* cache, int cs) {
   Geo g0 = geo_of_g<2>(d, j, k, GEO >= 0 ? GEO : d.tgeom[j]);
   double pc[NQ], mc[NQ], fc[NQ];
   load_v<2, NQ>(d, d.prim, k, pc);
-  load_v<2, NQ>(d, d.gmv, k, mc);
+  gmv_at<NQ, GEO>(d, j, i, mc);
   trans_grad<NQ, f1_kc<NQ>(), GEO>(d, j, k, pc, mc, g0, fc, cache, cs);
   store_v<2, NQ>(d, d.gf1n, k, fc);
 }
@@ -540,7 +570,7 @@ GH_HD void f1face_at(const Dev& d, int j, int i, int s, const double* pc, const 
     double dxfL = jpos - g0.cen1, dxfR = jpos - gn.cen1;
     double pn[NQ], mn[NQ], fn[NQ];
     load_v<2, NQ>(d, d.prim, kn, pn);
-    load_v<2, NQ>(d, d.gmv, kn, mn);
+    gmv_at<NQ, GEO>(d, tj, lo + n, mn);
     load_v<2, NQ>(d, d.gf1, kn, fn);
     for (int q = 0; q < NQ; ++q) {
       double qL = pc[q] + mc[q] * dxmL;
@@ -588,7 +618,7 @@ GH_HD void body_f1face_s(const Dev& d, int j, int s) {
   long long k = CI(d, j, i);
   double pc[NQ], mc[NQ], fc[NQ];
   load_v<2, NQ>(d, d.prim, k, pc);
-  load_v<2, NQ>(d, d.gmv, k, mc);
+  gmv_at<NQ, GEO>(d, j, i, mc);
   load_v<2, NQ>(d, d.gf1n, k, fc);
   f1face_at<NQ, SHK, GEO>(d, j, i, s, pc, mc, fc, geo_of_g<2>(d, j, k, GEO >= 0 ? GEO : d.tgeom[j]));
 }
@@ -611,7 +641,7 @@ GH_HD void body_f1cell_c(const Dev& d, int j, int i, double* cache, int cs) {
   Geo g0 = geo_of<2>(d, j, k);
   double pc[NQ], mc[NQ], fc[NQ];
   load_v<2, NQ>(d, d.prim, k, pc);
-  load_v<2, NQ>(d, d.gmv, k, mc);
+  gmv_at<NQ>(d, j, i, mc);
   trans_grad<NQ, f1_kc<NQ>()>(d, j, k, pc, mc, g0, fc, cache, cs);
   store_v<2, NQ>(d, d.gf1n, k, fc);
   int cnt = d.nbucnt[k], off = d.f1off[k];
@@ -2781,8 +2811,10 @@ int gammahip_set_option(gammahip_ctx* c, int key, long long value) {
     case GAMMAHIP_OPT_F1FUSE: c->d.f1fused = value ? 1 : 0; return 0;
     case GAMMAHIP_OPT_F1CELL: c->f1cell = value ? 1 : 0; return 0;
     case GAMMAHIP_OPT_FUSE:
-      if (value < 0 || value > 1) return fail(c, "fuse option must be 0 or 1");
-      c->fuse = (int)value; return 0;
+      if (value < 0 || value > 3) return fail(c, "fuse option must be in 0..3");
+      c->fuse = (int)value;
+      c->d.gmvre = (value & 2) ? 1 : 0;
+      return 0;
     case GAMMAHIP_OPT_OCC:
       if (value < 0 || value > 3) return fail(c, "occupancy option must be in 0..3");
       c->occ = (int)value; return 0;

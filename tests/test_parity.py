@@ -374,9 +374,11 @@ def test_gpu_pow_equals_oracle_bitwise():
     dict(block=64, remap=0),
     dict(block=128, remap=2, fuse=1, f1fuse=1),
     dict(block=256, remap=1, persist=16, graph=1, fuse=1),
+    dict(block=128, remap=2, fuse=2),
+    dict(block=64, remap=1, fuse=3, f1fuse=1, occ=3),
 ], ids=["remap-graph", "packed", "packed-remap-graph-capped", "f1cell", "packed-f1cell-f1fuse",
         "track-groups-graph-capped", "track-groups-f1fuse-f1cell", "hardware-order", "fused-nbgrad",
-        "packed-fused-nbgrad-graph"])
+        "packed-fused-nbgrad-graph", "gmv-recompute", "nbgrad-gmv-recompute-f1fuse-capped"])
 def test_gpu_parity_launch_options(case, opts):
     """Launch options change where work runs, never the arithmetic: the
     trajectory under every option matches the oracle as the default launch does."""
@@ -391,3 +393,23 @@ def test_gpu_parity_launch_options(case, opts):
     res = check_pair(o, g, shock=bool(s.cfg.shock), exact_expected=case in GPU_EXACT,
                      bars=gpu_bars(case, s, n))
     print(f"[gpu parity options] {case} {opts}: {res}", flush=True)
+
+
+@pytest.mark.parametrize("case", ["riemann_2d", "bm_jet_2d"])
+def test_emu_gmv_recompute_is_bitwise(emu_lib, case):
+    """The F1 passes recomputing MV gradients from the primitives
+    (GAMMAHIP_OPT_FUSE bit 1) give the bits of the passes that load the stored
+    gradients: same cells, gradients, shock flags and regrid counts (CPU build)."""
+    mk, n = CASES[case]
+    s = mk()
+    a = hip.Grid(s, lib_path=emu_lib)
+    a.step(n)
+    b = hip.Grid(s, lib_path=emu_lib)
+    b.set_options(fuse=2)
+    b.step(n)
+    fields = list(range(s.cfg.nq)) + [hip.F_X, hip.F_DX] + [hip.F_GRADF10 + q for q in range(s.cfg.nq)]
+    if s.cfg.shock:
+        fields += [hip.F_SD, hip.F_SHOCKED]
+    for f in fields:
+        assert np.array_equal(a.cells(f), b.cells(f), equal_nan=True), f
+    assert a.stats()["splits"] == b.stats()["splits"] and a.stats()["merges"] == b.stats()["merges"]
