@@ -25,7 +25,7 @@ from __future__ import annotations
 import time
 
 SPACE = (("solver", (0, 1)), ("block", (64, 128, 256)), ("remap", (0, 1, 2)), ("graph", (0, 1)),
-         ("persist", (0, 4, 8, 16, 32)), ("occ", (0, 1, 2, 3)), ("f1fuse", (0, 1)), ("f1cell", (0, 1)), ("fuse", (0, 1, 2, 3)))
+         ("persist", (0, 4, 8, 16, 32)), ("occ", (0, 1, 2, 3)), ("f1fuse", (0, 1)), ("f1cell", (0, 1)), ("fuse", (0, 1, 2, 4)))
 DEFAULTS = {"solver": 1, "block": 128, "remap": 2, "graph": 0, "persist": 0, "occ": 0, "f1fuse": 0,
             "f1cell": 0, "fuse": 0, "mlp_thr": None}
 OPTION_KEYS = tuple(DEFAULTS)

@@ -628,6 +628,28 @@ GH_HD void body_f1face(const Dev& d, int j, int s) {
   else body_f1face_s<NQ, false>(d, j, s);
 }
 
+// Face pass, one thread per owner cell (launch option FUSE bit 2): the cell's
+// state is loaded once and the faces of its own list formed in turn (where
+// the tracks are aligned a cell has one upper neighbour).  The slot pass finds
+// its owner through the slot table, one dependent load more.  Same faces,
+// same arithmetic: the slots of every cell with a list, stale lists of the
+// track-end cells included.
+template <int NQ, bool SHK, int GEO = -1>
+GH_HD void body_f1own(const Dev& d, int j, int i) {
+  if (j > d.nj - 2) return;
+  int nt = d.ntrack[j];
+  if (i >= nt) return;
+  long long k = CI(d, j, i);
+  int cnt = d.nbucnt[k], off = d.f1off[k];
+  if (cnt <= 0) return;
+  double pc[NQ], mc[NQ], fc[NQ];
+  load_v<2, NQ>(d, d.prim, k, pc);
+  gmv_at<NQ, GEO>(d, j, i, mc);
+  load_v<2, NQ>(d, d.gf1n, k, fc);
+  Geo g0 = geo_of_g<2>(d, j, k, GEO >= 0 ? GEO : d.tgeom[j]);
+  for (int n = 0; n < cnt && off + n < d.fcap; ++n) f1face_at<NQ, SHK, GEO>(d, j, i, off + n, pc, mc, fc, g0);
+}
+
 // Fused cell pass: the cell's transverse gradient (as body_f1grad_c), then
 // the faces of its own F1 list with the owner's state kept in registers.
 // The same arithmetic as the two passes; the gradient is still stored, as the
@@ -1415,6 +1437,13 @@ __global__ void __launch_bounds__(KB, W) k_f1faces(Dev d, int remap) {
   int tot = min(d.f1tot[j], d.fcap);
   if (s < tot) body_f1face_s<NQ, SHK, GEO>(d, j, s);
 }
+// the face pass with one thread per owner cell (body_f1own)
+template <int NQ, int W, bool SHK, int GEO>
+__global__ void __launch_bounds__(KB, W) k_f1own(Dev d, int ni, int remap) {
+  int i, j;
+  tile_of(remap, j, i);
+  if (i < ni && j < d.nj) body_f1own<NQ, SHK, GEO>(d, j, i);
+}
 // W: minimum wavefronts per SIMD (1 = the compiler's choice; 5 caps the
 // registers at 96 and spills the rest), picked by the launch option; GEO as
 // body_update_g (-1 in 1D).  The 2D Cartesian six-field body fits 96
@@ -2026,7 +2055,9 @@ struct gammahip_ctx {
                            // fewer where the kernel's occupancy allows fewer (occ_blocks)
   int occ = 0;             // bit 0: F1 face kernel built for 4 wavefronts per SIMD; bit 1: update kernel for 5
   int f1cell = 0;          // fused transverse-gradient + own-face pass (k_f1cell) instead of k_f1grad + k_f1faces
-  int fuse = 0;            // bit 0: neighbour search inside the transverse-gradient pass (k_nbgrad)
+  int fuse = 0;            // bit 0: neighbour search inside the transverse-gradient pass (k_nbgrad);
+                           // bit 1: MV gradients recomputed in the F1 passes (Dev::gmvre);
+                           // bit 2: F1 face pass one thread per owner cell (k_f1own)
   int ncu = 256;           // compute units of the device
 #ifndef GH_EMU
   hipStream_t stream = nullptr;
@@ -2335,11 +2366,33 @@ struct Pipe {
 #endif
         if (!nbgrad) tend(c, 3);
         tbeg(c, 2);
+        bool f1own = (c->fuse & 4) != 0;
 #ifdef GH_EMU
-        for (int j = 0; j < d.nj; ++j)
-          for (int sl = 0; sl < d.fcap; ++sl) body_f1face<NQ>(d, j, sl);
+        if (f1own) {
+          for (int j = 0; j < d.nj; ++j)
+            for (int i = 0; i < d.nax; ++i) {
+              if (d.shock) body_f1own<NQ, true>(d, j, i);
+              else body_f1own<NQ, false>(d, j, i);
+            }
+        } else {
+          for (int j = 0; j < d.nj; ++j)
+            for (int sl = 0; sl < d.fcap; ++sl) body_f1face<NQ>(d, j, sl);
+        }
 #else
-        {
+        if (f1own) {
+          int bs = c->block;
+          PlainLaunch pl = plain_launch(c, d.nj, d.nax);     // one thread per cell slot
+          constexpr int WC = NQ > 6 ? 3 : 4;
+          bool sph = c->cfg.geom == SPH, cap = (c->occ & 1) != 0;
+          auto launch = [&](auto kern) { hipLaunchKernelGGL(kern, pl.g, dim3(bs), 0, c->stream, d, d.nax, pl.remap); };
+          if (d.shock) {
+            if (sph) { if (cap) launch(k_f1own<NQ, WC, true, SPH>); else launch(k_f1own<NQ, 1, true, SPH>); }
+            else { if (cap) launch(k_f1own<NQ, WC, true, CART>); else launch(k_f1own<NQ, 1, true, CART>); }
+          } else {
+            if (sph) { if (cap) launch(k_f1own<NQ, WC, false, SPH>); else launch(k_f1own<NQ, 1, false, SPH>); }
+            else { if (cap) launch(k_f1own<NQ, WC, false, CART>); else launch(k_f1own<NQ, 1, false, CART>); }
+          }
+        } else {
           int bs = c->block;
           PlainLaunch pl = plain_launch(c, d.nj, d.fcap);  // one thread per F1 slot of the capacity
           // 2D grids have one geometry: the kernel is built for it
@@ -2811,7 +2864,7 @@ int gammahip_set_option(gammahip_ctx* c, int key, long long value) {
     case GAMMAHIP_OPT_F1FUSE: c->d.f1fused = value ? 1 : 0; return 0;
     case GAMMAHIP_OPT_F1CELL: c->f1cell = value ? 1 : 0; return 0;
     case GAMMAHIP_OPT_FUSE:
-      if (value < 0 || value > 3) return fail(c, "fuse option must be in 0..3");
+      if (value < 0 || value > 7) return fail(c, "fuse option must be in 0..7");
       c->fuse = (int)value;
       c->d.gmvre = (value & 2) ? 1 : 0;
       return 0;

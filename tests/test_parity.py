@@ -376,9 +376,12 @@ def test_gpu_pow_equals_oracle_bitwise():
     dict(block=256, remap=1, persist=16, graph=1, fuse=1),
     dict(block=128, remap=2, fuse=2),
     dict(block=64, remap=1, fuse=3, f1fuse=1, occ=3),
+    dict(block=128, remap=2, fuse=4),
+    dict(block=64, remap=1, fuse=5, f1fuse=1, occ=1, graph=1),
 ], ids=["remap-graph", "packed", "packed-remap-graph-capped", "f1cell", "packed-f1cell-f1fuse",
         "track-groups-graph-capped", "track-groups-f1fuse-f1cell", "hardware-order", "fused-nbgrad",
-        "packed-fused-nbgrad-graph", "gmv-recompute", "nbgrad-gmv-recompute-f1fuse-capped"])
+        "packed-fused-nbgrad-graph", "gmv-recompute", "nbgrad-gmv-recompute-f1fuse-capped",
+        "f1-faces-per-owner", "nbgrad-f1-faces-per-owner-capped-graph"])
 def test_gpu_parity_launch_options(case, opts):
     """Launch options change where work runs, never the arithmetic: the
     trajectory under every option matches the oracle as the default launch does."""
@@ -396,16 +399,18 @@ def test_gpu_parity_launch_options(case, opts):
 
 
 @pytest.mark.parametrize("case", ["riemann_2d", "bm_jet_2d"])
-def test_emu_gmv_recompute_is_bitwise(emu_lib, case):
+@pytest.mark.parametrize("fuse", [2, 4, 6])
+def test_emu_fused_options_are_bitwise(emu_lib, case, fuse):
     """The F1 passes recomputing MV gradients from the primitives
-    (GAMMAHIP_OPT_FUSE bit 1) give the bits of the passes that load the stored
-    gradients: same cells, gradients, shock flags and regrid counts (CPU build)."""
+    (GAMMAHIP_OPT_FUSE bit 1) and the F1 face pass with one thread per owner
+    cell (bit 2) give the bits of the default passes: same cells, gradients,
+    shock flags and regrid counts (CPU build)."""
     mk, n = CASES[case]
     s = mk()
     a = hip.Grid(s, lib_path=emu_lib)
     a.step(n)
     b = hip.Grid(s, lib_path=emu_lib)
-    b.set_options(fuse=2)
+    b.set_options(fuse=fuse)
     b.step(n)
     fields = list(range(s.cfg.nq)) + [hip.F_X, hip.F_DX] + [hip.F_GRADF10 + q for q in range(s.cfg.nq)]
     if s.cfg.shock:
