@@ -76,14 +76,10 @@ struct Dev {
   int *shk, *nbdj, *nbdlo, *nbdcnt, *nbuj, *nbulo, *nbucnt, *f1off, *f1tot;
   int *f1own;              // [nj][fcap] cell owning each F1 face slot
   int *incf;               // first fresh lower-track cell whose F1 list holds this cell
-  int *dncnt, *dnoff;      // fresh lower-track sources of each cell, their offset in the track's down slots
+  int *dncnt;              // number of fresh lower-track cells whose F1 lists hold it (consecutive from incf)
   // faces
   double *fx, *fxn, *fx0, *fv, *flL, *flR, *fflux;
   double *ff1, *ff1sd;
-  // copies of the F1 face fluxes (and shock values) ordered by upper cell:
-  // the faces a cell receives from fresh lists of the track below sit in
-  // down slots dnoff.. of its own track, in the order of their source cells
-  double *ff1dn, *ff1sddn;
   double *f1net;           // [NQ][nj][nax] net F1 flux (dn - up) of each cell, this stage
   int f1fused;             // 1: the update kernel sums the F1 fluxes itself (no k_f1sum pass)
   int gmvre;               // 1: the F1 passes recompute MV gradients from the primitives (gmv_at)
@@ -354,31 +350,35 @@ GH_HD void body_dt(const Dev& d, int j, int i) {
 // Galloping search from the hint h (adjacent tracks have nearly the same
 // structure, so the answer is usually within a few faces of the cell's own
 // index), then bisection inside the bracket found: the same lower bound as a
-// full binary search on a sorted track.
-GH_HD int lb_face(const Dev& d, int ja, double xr, int h) {
-  int n = d.ntrack[ja] - 1;                  // searchable faces [0, n)
-  const double* X = d.fx + (long long)ja * d.nax;
+// full binary search on a sorted track.  X(t) returns face t's position
+// (t in [0, n) only).
+template <class XF>
+GH_HD int lb_face_x(int n, const XF& X, double xr, int h) {
   if (h < 0) h = 0;
   if (h > n) h = n;
   int lo, hi;
-  if (h < n && X[h] < xr) {                  // answer right of h
+  if (h < n && X(h) < xr) {                  // answer right of h
     lo = h + 1;
     int step = 1;
     hi = lo;
-    while (hi < n && X[hi] < xr) { lo = hi + 1; hi += step; step <<= 1; }
+    while (hi < n && X(hi) < xr) { lo = hi + 1; hi += step; step <<= 1; }
     if (hi > n) hi = n;
   } else {                                   // answer at or left of h
     hi = h;
     int step = 1;
     lo = h - 1;
-    while (lo >= 0 && X[lo] >= xr) { hi = lo; lo -= step; step <<= 1; }
+    while (lo >= 0 && X(lo) >= xr) { hi = lo; lo -= step; step <<= 1; }
     lo = lo < 0 ? 0 : lo + 1;
   }
   while (lo < hi) {
     int mid = (lo + hi) >> 1;
-    if (X[mid] >= xr) hi = mid; else lo = mid + 1;
+    if (X(mid) >= xr) hi = mid; else lo = mid + 1;
   }
   return lo;
+}
+GH_HD int lb_face(const Dev& d, int ja, double xr, int h) {
+  const double* X = d.fx + (long long)ja * d.nax;
+  return lb_face_x(d.ntrack[ja] - 1, [X](int t) { return X[t]; }, xr, h);  // searchable faces [0, nt-1)
 }
 
 // Grid::computeNeighbors: the two-pointer sweep of 2d.cpp:551-576 evaluated
@@ -542,18 +542,17 @@ GH_HD void body_f1grad(const Dev& d, int j, int i) {
 // the top ghost track pointing into their own track read the previous values
 // too (their faces only feed ghost cells).
 template <int NQ, bool SHK, int GEO = -1>
-GH_HD void f1face_at(const Dev& d, int j, int i, int s, const double* pc, const double* mc, const double* fc,
-                     const Geo& g0) {
+GH_HD void f1face_at(const Dev& d, int j, int i, int s, int tj, int in, const double* pc, const double* mc,
+                     const double* fc, const Geo& g0) {
   int geom = GEO >= 0 ? GEO : d.tgeom[j];
   Trig Tj = trig_at_g<2>(d, j, geom);
   double G = d.P.gamma;
   long long k = CI(d, j, i);
-  int tj = d.nbuj[k], lo = d.nbulo[k], n = s - d.f1off[k];
   double x0 = d.cx[k];
   double xL0 = x0 - d.cdx[k] / 2., xR0 = x0 + d.cdx[k] / 2.;
   double jpos = d.ty[j] + d.tdy[j] / 2.;
   bool pc0 = (d.pc || j == 0 || j == d.nj - 2 || d.nbdcnt[k] == 0);
-  long long kn = CI(d, tj, lo + n);
+  long long kn = CI(d, tj, in);
   double xn = d.cx[kn];
   double xLn = xn - d.cdx[kn] / 2., xRn = xn + d.cdx[kn] / 2.;
   double xm = (fmax(xL0, xLn) + fmin(xR0, xRn)) / 2.;
@@ -563,14 +562,14 @@ GH_HD void f1face_at(const Dev& d, int j, int i, int s, const double* pc, const 
   double rL = xm, rR = xm;
   if (pc0 || d.nbucnt[kn] == 0) {
     rL = cell_pc_g<2, NQ>(d, j, i, pL, cL, geom);
-    rR = cell_pc_g<2, NQ>(d, tj, lo + n, pR, cR, geom);
+    rR = cell_pc_g<2, NQ>(d, tj, in, pR, cR, geom);
   } else {
     Geo gn = geo_of_g<2>(d, tj, kn, geom);
     double dxmL = xm - g0.cen0, dxmR = xm - gn.cen0;
     double dxfL = jpos - g0.cen1, dxfR = jpos - gn.cen1;
     double pn[NQ], mn[NQ], fn[NQ];
     load_v<2, NQ>(d, d.prim, kn, pn);
-    gmv_at<NQ, GEO>(d, tj, lo + n, mn);
+    gmv_at<NQ, GEO>(d, tj, in, mn);
     load_v<2, NQ>(d, d.gf1, kn, fn);
     for (int q = 0; q < NQ; ++q) {
       double qL = pc[q] + mc[q] * dxmL;
@@ -589,22 +588,12 @@ GH_HD void f1face_at(const Dev& d, int j, int i, int s, const double* pc, const 
   }, fl);
   long long fk = (long long)j * d.fcap + s;
   for (int q = 0; q < NQ; ++q) d.ff1[q * d.fs1 + fk] = fl[q];
-  // fresh lists (cells 1..nt-2): also the upper cell's down slot
-  long long fd = -1;
-  if (i >= 1 && i <= d.ntrack[j] - 2) {
-    long long ku = CI(d, tj, lo + n);
-    long long slot = d.dnoff[ku] + (i - d.incf[ku]);
-    if (slot < d.fcap) fd = (long long)tj * d.fcap + slot;
-  }
-  if (fd >= 0)
-    for (int q = 0; q < NQ; ++q) d.ff1dn[q * d.fs1 + fd] = fl[q];
   if (SHK) {
     double psf = -1.e300, psr = -1.e300;
     double sdf = shock_sd<2, NQ>(pL, pR, 1, false, G, &psf, d.vpspec);
     double sdr = shock_sd<2, NQ>(pR, pL, 1, true, G, &psr, d.vpspec);
     d.ff1sd[0 * d.fs1 + fk] = sdf; d.ff1sd[1 * d.fs1 + fk] = psf;
     d.ff1sd[2 * d.fs1 + fk] = sdr; d.ff1sd[3 * d.fs1 + fk] = psr;
-    if (fd >= 0) { d.ff1sddn[fd] = sdr; d.ff1sddn[d.fs1 + fd] = psr; }
   }
 }
 
@@ -620,7 +609,8 @@ GH_HD void body_f1face_s(const Dev& d, int j, int s) {
   load_v<2, NQ>(d, d.prim, k, pc);
   gmv_at<NQ, GEO>(d, j, i, mc);
   load_v<2, NQ>(d, d.gf1n, k, fc);
-  f1face_at<NQ, SHK, GEO>(d, j, i, s, pc, mc, fc, geo_of_g<2>(d, j, k, GEO >= 0 ? GEO : d.tgeom[j]));
+  f1face_at<NQ, SHK, GEO>(d, j, i, s, d.nbuj[k], d.nbulo[k] + (s - d.f1off[k]), pc, mc, fc,
+                          geo_of_g<2>(d, j, k, GEO >= 0 ? GEO : d.tgeom[j]));
 }
 template <int NQ>
 GH_HD void body_f1face(const Dev& d, int j, int s) {
@@ -647,7 +637,9 @@ GH_HD void body_f1own(const Dev& d, int j, int i) {
   gmv_at<NQ, GEO>(d, j, i, mc);
   load_v<2, NQ>(d, d.gf1n, k, fc);
   Geo g0 = geo_of_g<2>(d, j, k, GEO >= 0 ? GEO : d.tgeom[j]);
-  for (int n = 0; n < cnt && off + n < d.fcap; ++n) f1face_at<NQ, SHK, GEO>(d, j, i, off + n, pc, mc, fc, g0);
+  int tj = d.nbuj[k], lo = d.nbulo[k];
+  for (int n = 0; n < cnt && off + n < d.fcap; ++n)
+    f1face_at<NQ, SHK, GEO>(d, j, i, off + n, tj, lo + n, pc, mc, fc, g0);
 }
 
 // Fused cell pass: the cell's transverse gradient (as body_f1grad_c), then
@@ -666,10 +658,10 @@ GH_HD void body_f1cell_c(const Dev& d, int j, int i, double* cache, int cs) {
   gmv_at<NQ>(d, j, i, mc);
   trans_grad<NQ, f1_kc<NQ>()>(d, j, k, pc, mc, g0, fc, cache, cs);
   store_v<2, NQ>(d, d.gf1n, k, fc);
-  int cnt = d.nbucnt[k], off = d.f1off[k];
+  int cnt = d.nbucnt[k], off = d.f1off[k], tj = d.nbuj[k], lo = d.nbulo[k];
   for (int n = 0; n < cnt && off + n < d.fcap; ++n) {
-    if (d.shock) f1face_at<NQ, true>(d, j, i, off + n, pc, mc, fc, g0);
-    else f1face_at<NQ, false>(d, j, i, off + n, pc, mc, fc, g0);
+    if (d.shock) f1face_at<NQ, true>(d, j, i, off + n, tj, lo + n, pc, mc, fc, g0);
+    else f1face_at<NQ, false>(d, j, i, off + n, tj, lo + n, pc, mc, fc, g0);
   }
 }
 template <int NQ>
@@ -680,21 +672,20 @@ GH_HD void body_f1cell(const Dev& d, int j, int i) {
 
 // Visit, in the reference's accumulation order (track j' ascending, cell i'
 // ascending, 2d.cpp:901-906), every F1 face whose upper cell is (J, i):
-// fn(false, fk) for a face in the lower cell's slots of ff1 / ff1sd, fn(true,
-// fk) for a face in (J, i)'s own down slots of ff1dn / ff1sddn.  Fresh lists
-// of track J-1 are non-decreasing ranges, so the sources of (J, i) there are
-// consecutive cells and their faces sit in consecutive down slots (filled by
-// the face pass); the stale lists that ghost copies carry on cells 0 and nt-1
-// are checked one by one.
+// fn(fk) with fk the face's slot in the lower cell's list (ff1 / ff1sd).
+// Fresh lists of track J-1 are non-decreasing ranges, so the sources of
+// (J, i) there are the consecutive cells incf .. incf + dncnt - 1; the stale
+// lists that ghost copies carry on cells 0 and nt-1 are checked one by one.
 template <class F>
 GH_HD void for_incoming_f1(const Dev& d, int J, int i, F&& fn) {
+  auto src = [&](int jp, long long kp) {
+    long long slot = d.f1off[kp] + (i - d.nbulo[kp]);
+    if (slot < d.fcap) fn((long long)jp * d.fcap + slot);
+  };
   auto stale = [&](int jp, int ip) {
     long long kp = CI(d, jp, ip);
     int c = d.nbucnt[kp], l = d.nbulo[kp];
-    if (d.nbuj[kp] == J && c > 0 && l <= i && i < l + c) {
-      long long slot = d.f1off[kp] + (i - l);
-      if (slot < d.fcap) fn(false, (long long)jp * d.fcap + slot);
-    }
+    if (d.nbuj[kp] == J && c > 0 && l <= i && i < l + c) src(jp, kp);
   };
   // ghost tracks below the domain carry stale lists copied from track jL+1,
   // which point two tracks up: they are the only sources besides track J-1
@@ -709,8 +700,9 @@ GH_HD void for_incoming_f1(const Dev& d, int J, int i, F&& fn) {
     int ntp = d.ntrack[jp];
     stale(jp, 0);
     long long k = CI(d, J, i);
-    int c = d.dncnt[k], off = d.dnoff[k];
-    for (int m = 0; m < c && off + m < d.fcap; ++m) fn(true, (long long)J * d.fcap + off + m);
+    int c = d.dncnt[k];
+    long long kp = c > 0 ? CI(d, jp, d.incf[k]) : 0;
+    for (int m = 0; m < c; ++m) src(jp, kp + m);
     stale(jp, ntp - 1);
   }
 }
@@ -729,9 +721,9 @@ GH_HD void body_f1_shock(const Dev& d, int j, int i) {
       if (d.ff1sd[d.fs1 + fk] > ps) ps = d.ff1sd[d.fs1 + fk];
     }
   }
-  for_incoming_f1(d, j, i, [&](bool dn, long long fk) {
-    double sd = dn ? d.ff1sddn[fk] : d.ff1sd[2 * d.fs1 + fk];
-    double pv = dn ? d.ff1sddn[d.fs1 + fk] : d.ff1sd[3 * d.fs1 + fk];
+  for_incoming_f1(d, j, i, [&](long long fk) {
+    double sd = d.ff1sd[2 * d.fs1 + fk];
+    double pv = d.ff1sd[3 * d.fs1 + fk];
     Sd = fmax(Sd, sd);
     if (pv > ps) ps = pv;
   });
@@ -752,9 +744,8 @@ GH_HD void f1_net(const Dev& d, int j, int i, long long k, double* net) {
     long long fk = (long long)j * d.fcap + off + n;
     for (int q = 0; q < NQ; ++q) up[q] += d.ff1[q * d.fs1 + fk];
   }
-  for_incoming_f1(d, j, i, [&](bool down, long long fk) {
-    const double* src = down ? d.ff1dn : d.ff1;
-    for (int q = 0; q < NQ; ++q) dn[q] += src[q * d.fs1 + fk];
+  for_incoming_f1(d, j, i, [&](long long fk) {
+    for (int q = 0; q < NQ; ++q) dn[q] += d.ff1[q * d.fs1 + fk];
   });
   for (int q = 0; q < NQ; ++q) net[q] = dn[q] - up[q];
 }
@@ -1328,29 +1319,54 @@ GH_KERNEL2P((), k_dt, body_dt)
 // previous lane's own bound of its right face (same face, same track, and the
 // search returns the bound whatever its starting hint), so each lane searches
 // once per side and takes the other from its neighbour; lane 0 searches both.
-// Every lane runs the shuffles (valid = false for lanes without a tile cell).
+// Every lane runs the shuffles and the barrier (valid = false for lanes
+// without a tile cell).  The faces of each neighbour track around the
+// wavefront's cells, [i0 - NBW_LO, i0 - NBW_LO + NBW) for the first cell i0,
+// are staged in LDS with one coalesced load: the searches start at the cell's
+// own index and mostly stay inside that window, so their chains of dependent
+// probes read LDS instead of memory (probes outside it read memory, so the
+// answer is the same).
+constexpr int NBW = 96, NBW_LO = 16;
 __device__ void body_nb_lanes(const Dev& d, int j, int i, bool valid) {
+  __shared__ double nbwin[KB / 64][2][NBW];
+  int lane = (int)(threadIdx.x & 63);
+  double* win0 = nbwin[threadIdx.x >> 6][0];
+  int w0 = i - lane - NBW_LO;
+  if (w0 < 0) w0 = 0;
+  for (int side = 0; side < 2; ++side) {
+    int ja = side == 0 ? j - 1 : j + 1;
+    if (ja >= 0 && ja < d.nj) {
+      const double* X = d.fx + (long long)ja * d.nax;
+      for (int t = lane; t < NBW; t += 64)
+        if (w0 + t < d.nax) win0[side * NBW + t] = X[w0 + t];
+    }
+  }
+  __syncthreads();
   int nt = valid ? d.ntrack[j] : 0;
   bool cell = valid && i >= 1 && i <= nt - 2;
   long long k = cell ? CI(d, j, i) : 0;
   double xjL = 0., xjR = 0.;
   if (cell) { xjL = d.cx[k] - d.cdx[k] / 2.; xjR = d.cx[k] + d.cdx[k] / 2.; }
-  int lane = (int)(threadIdx.x & 63);
   int lo2[2] = {0, 0}, cnt2[2] = {0, 0};
   for (int side = 0; side < 2; ++side) {
     int ja = side == 0 ? j - 1 : j + 1;
     bool has = cell && (side == 0 ? j != 0 : j != d.nj - 1);
-    int L = has ? lb_face(d, ja, xjR, i) : 0;
+    const double* X = d.fx + (long long)ja * d.nax;
+    const double* win = win0 + side * NBW;
+    // the window covers [w0, w0 + NBW) but never reaches past the plane's row
+    auto XF = [X, win, w0](int t) { unsigned u = (unsigned)(t - w0); return u < (unsigned)NBW ? win[u] : X[t]; };
+    int n = has ? d.ntrack[ja] - 1 : 0;
+    int L = has ? lb_face_x(n, XF, xjR, i) : 0;
     int lp = __shfl(L, lane == 0 ? 0 : lane - 1, 64);
     if (has) {
-      int nta = d.ntrack[ja];
+      int nta = n + 1;
       int fin = 0;
       if (i > 1) {
-        if (lane == 0) lp = lb_face(d, ja, d.cx[k - 1] + d.cdx[k - 1] / 2., i - 1);
+        if (lane == 0) lp = lb_face_x(n, XF, d.cx[k - 1] + d.cdx[k - 1] / 2., i - 1);
         fin = lp < nta - 2 ? lp : nta - 2;
       }
       int s0 = fin;
-      if (d.fx[CI(d, ja, s0)] > xjL && s0 > 0) s0--;
+      if (XF(s0) > xjL && s0 > 0) s0--;
       if (L < s0) L = s0;
       lo2[side] = s0 + 1;
       cnt2[side] = L - s0;
@@ -1551,31 +1567,24 @@ __global__ void __launch_bounds__(KB) k_f1_scan(Dev d) {
   __shared__ int wsum[KB / 64];
   int j = blockIdx.x;
   int nt = d.ntrack[j];
-  for (int pass = 0; pass < 2; ++pass) {
-    const int* cnt = pass == 0 ? d.nbucnt : d.dncnt;
-    int base = 0;
-    int vnext = (int)threadIdx.x < nt ? cnt[CI(d, j, threadIdx.x)] : 0;
-    for (int i0 = 0; i0 < nt; i0 += KB) {
-      int i = i0 + (int)threadIdx.x;
-      int v = vnext;
-      if (pass == 0 && v < 0) v = 0;
-      vnext = i + KB < nt ? cnt[CI(d, j, i + KB)] : 0;
-      int tot;
-      int off = base + wg_scan_incl(v, wsum, &tot) - v;
-      if (i < nt) {
-        if (pass == 0) {
-          d.f1off[CI(d, j, i)] = off;
-          for (int n = 0; n < v && off + n < d.fcap; ++n) d.f1own[(long long)j * d.fcap + off + n] = i;
-        } else {
-          d.dnoff[CI(d, j, i)] = off;
-        }
-      }
-      base += tot;
+  const int* cnt = d.nbucnt;
+  int base = 0;
+  int vnext = (int)threadIdx.x < nt ? cnt[CI(d, j, threadIdx.x)] : 0;
+  for (int i0 = 0; i0 < nt; i0 += KB) {
+    int i = i0 + (int)threadIdx.x;
+    int v = vnext < 0 ? 0 : vnext;
+    vnext = i + KB < nt ? cnt[CI(d, j, i + KB)] : 0;
+    int tot;
+    int off = base + wg_scan_incl(v, wsum, &tot) - v;
+    if (i < nt) {
+      d.f1off[CI(d, j, i)] = off;
+      for (int n = 0; n < v && off + n < d.fcap; ++n) d.f1own[(long long)j * d.fcap + off + n] = i;
     }
-    if (pass == 0 && threadIdx.x == 0) {
-      d.f1tot[j] = base;
-      if (base > d.fcap) atomicAdd(cnt_at(d, CNT_F1OVF), 1ull);
-    }
+    base += tot;
+  }
+  if (threadIdx.x == 0) {
+    d.f1tot[j] = base;
+    if (base > d.fcap) atomicAdd(cnt_at(d, CNT_F1OVF), 1ull);
   }
 }
 // The applyRegrid loop of one track on one workgroup, in the sequential
@@ -2318,12 +2327,6 @@ struct Pipe {
         }
         d.f1tot[j] = base;
         if (base > d.fcap) d.cnt[CNT_F1OVF]++;
-        base = 0;
-        for (int i = 0; i < nt; ++i) {
-          long long k = CI(d, j, i);
-          d.dnoff[k] = base;
-          base += d.dncnt[k];
-        }
       }
 #else
       hipLaunchKernelGGL(k_f1_scan, dim3(d.nj), dim3(KB), 0, c->stream, d);
@@ -2679,10 +2682,8 @@ int gammahip_create(const gammahip_config* cfg, gammahip_ctx** out) {
     e |= dalloc(c, &d.f1own, (size_t)nj * d.fcap);
     e |= dalloc(c, &d.ff1, nqs * (size_t)d.fs1);
     if (d.shock) e |= dalloc(c, &d.ff1sd, 4 * (size_t)d.fs1);
-    e |= dalloc(c, &d.ff1dn, nqs * (size_t)d.fs1);
     e |= dalloc(c, &d.f1net, nqs * cs);
-    if (d.shock) e |= dalloc(c, &d.ff1sddn, 2 * (size_t)d.fs1);
-    e |= dalloc(c, &d.dncnt, cs); e |= dalloc(c, &d.dnoff, cs);
+    e |= dalloc(c, &d.dncnt, cs);
   }
   if (d.shock) { e |= dalloc(c, &d.sd, cs); e |= dalloc(c, &d.pspec, cs); e |= dalloc(c, &d.shk, cs); }
   if (e) return -1;
@@ -3005,11 +3006,11 @@ std::vector<StatePart> state_parts(const Dev& d) {
     // track's transverse gradient of the previous stage (2d.cpp:901 loops
     // tracks upward), ghost tracks keep the lists of earlier stages
     add(d.gmv, sizeof(double) * nqs * cs); add(d.gf1, sizeof(double) * nqs * cs);
-    for (int* q : {d.nbdj, d.nbdlo, d.nbdcnt, d.nbuj, d.nbulo, d.nbucnt, d.f1off, d.incf, d.dncnt, d.dnoff})
+    for (int* q : {d.nbdj, d.nbdlo, d.nbdcnt, d.nbuj, d.nbulo, d.nbucnt, d.f1off, d.incf, d.dncnt})
       add(q, sizeof(int) * cs);
     add(d.f1tot, sizeof(int) * nj); add(d.f1own, sizeof(int) * nj * d.fcap);
-    add(d.ff1, sizeof(double) * nqs * fs1); add(d.ff1dn, sizeof(double) * nqs * fs1);
-    if (d.shock) { add(d.ff1sd, sizeof(double) * 4 * fs1); add(d.ff1sddn, sizeof(double) * 2 * fs1); }
+    add(d.ff1, sizeof(double) * nqs * fs1);
+    if (d.shock) add(d.ff1sd, sizeof(double) * 4 * fs1);
   }
   return v;
 }
