@@ -75,8 +75,6 @@ struct Dev {
   double *prim, *cx, *cdx, *dtloc, *U0, *dV0, *gmv, *gf1, *gf1n, *sd, *pspec;
   int *shk, *nbdj, *nbdlo, *nbdcnt, *nbuj, *nbulo, *nbucnt, *f1off, *f1tot;
   int *f1own;              // [nj][fcap] cell owning each F1 face slot
-  int *incf;               // first fresh lower-track cell whose F1 list holds this cell
-  int *dncnt;              // number of fresh lower-track cells whose F1 lists hold it (consecutive from incf)
   // faces
   double *fx, *fxn, *fx0, *fv, *flL, *flR, *fflux;
   double *ff1, *ff1sd;
@@ -252,11 +250,6 @@ GH_HD void body_faces_s(const Dev& d, int j, int i) {
   int geom = d.tgeom[j];
   double G = d.P.gamma;
   long long k = CI(d, j, i);
-  if (ND == 2) {               // reset before the neighbour pass fills them
-    d.incf[k] = INT_MAX;
-    d.dncnt[k] = 0;
-    if (i == nt - 2) { d.incf[k + 1] = INT_MAX; d.dncnt[k + 1] = 0; }
-  }
   double xf = d.fx[k];
   double pL[NQ], cL[NQ], pR[NQ], cR[NQ];
   // side fluxes: at the track ends the cells' own states (sweep radius), inside
@@ -346,14 +339,15 @@ GH_HD void body_dt(const Dev& d, int j, int i) {
 // ---------------------------------------------------------------------------
 // 2D transverse (F1) coupling (2d.cpp:366-430, 491-527, 534-606, 890-950)
 
-// first face index k in [0, nt-2] of track ja with x >= xr, nt-1 if none.
-// Galloping search from the hint h (adjacent tracks have nearly the same
-// structure, so the answer is usually within a few faces of the cell's own
-// index), then bisection inside the bracket found: the same lower bound as a
-// full binary search on a sorted track.  X(t) returns face t's position
-// (t in [0, n) only).
-template <class XF>
-GH_HD int lb_face_x(int n, const XF& X, double xr, int h) {
+// First t in [0, n) with X(t) >= xr (n if none) for non-decreasing X(t),
+// read for t in [0, n) only.  Galloping search from the hint h (adjacent
+// tracks have nearly the same structure, so the answer is usually within a
+// few cells of the cell's own index), then bisection inside the bracket
+// found: the same lower bound as a full binary search.  On the faces of a
+// track (X = face positions, n = nt - 1): the first face index k in
+// [0, nt-2] with x >= xr, nt-1 if none.
+template <class XF, class T>
+GH_HD int lb_x(int n, const XF& X, T xr, int h) {
   if (h < 0) h = 0;
   if (h > n) h = n;
   int lo, hi;
@@ -378,7 +372,7 @@ GH_HD int lb_face_x(int n, const XF& X, double xr, int h) {
 }
 GH_HD int lb_face(const Dev& d, int ja, double xr, int h) {
   const double* X = d.fx + (long long)ja * d.nax;
-  return lb_face_x(d.ntrack[ja] - 1, [X](int t) { return X[t]; }, xr, h);  // searchable faces [0, nt-1)
+  return lb_x(d.ntrack[ja] - 1, [X](int t) { return X[t]; }, xr, h);  // searchable faces [0, nt-1)
 }
 
 // Grid::computeNeighbors: the two-pointer sweep of 2d.cpp:551-576 evaluated
@@ -413,10 +407,6 @@ GH_HD void body_nb(const Dev& d, int j, int i) {
   lo = 0; cnt = 0;
   if (j != d.nj - 1) nb_side(d, j, i, j + 1, &lo, &cnt);
   d.nbuj[k] = j + 1; d.nbulo[k] = lo; d.nbucnt[k] = cnt;
-  for (int t = lo; t < lo + cnt; ++t) {
-    GH_ATOMIC_MIN(&d.incf[CI(d, j + 1, t)], i);
-    GH_ATOMIC_ADD(&d.dncnt[CI(d, j + 1, t)], 1);
-  }
 }
 
 // MV gradient of cell (j, m) as this stage's MV face pass left it (2D): face m
@@ -673,9 +663,12 @@ GH_HD void body_f1cell(const Dev& d, int j, int i) {
 // Visit, in the reference's accumulation order (track j' ascending, cell i'
 // ascending, 2d.cpp:901-906), every F1 face whose upper cell is (J, i):
 // fn(fk) with fk the face's slot in the lower cell's list (ff1 / ff1sd).
-// Fresh lists of track J-1 are non-decreasing ranges, so the sources of
-// (J, i) there are the consecutive cells incf .. incf + dncnt - 1; the stale
-// lists that ghost copies carry on cells 0 and nt-1 are checked one by one.
+// The fresh lists of track J-1 (cells 1..nt-2, this stage's neighbour pass)
+// are ranges [lo, lo + cnt) with lo and lo + cnt non-decreasing along the
+// track, so the cells whose list holds i are consecutive: from the first
+// whose list ends past i to the last that starts at or before i, found by
+// two searches from the hint of (J, i)'s own lower list.  The stale lists
+// that ghost copies carry on cells 0 and nt-1 are checked one by one.
 template <class F>
 GH_HD void for_incoming_f1(const Dev& d, int J, int i, F&& fn) {
   auto src = [&](int jp, long long kp) {
@@ -699,10 +692,16 @@ GH_HD void for_incoming_f1(const Dev& d, int J, int i, F&& fn) {
     if (jp < 0 || jp > d.nj - 2) return;
     int ntp = d.ntrack[jp];
     stale(jp, 0);
-    long long k = CI(d, J, i);
-    int c = d.dncnt[k];
-    long long kp = c > 0 ? CI(d, jp, d.incf[k]) : 0;
-    for (int m = 0; m < c; ++m) src(jp, kp + m);
+    int m = ntp - 2;                               // fresh cells 1..m
+    if (m >= 1) {
+      const int* LO = d.nbulo + (long long)jp * d.nax + 1;
+      const int* CN = d.nbucnt + (long long)jp * d.nax + 1;
+      long long k = CI(d, J, i);
+      int h = d.nbdlo[k] - 1;
+      int a = lb_x(m, [LO, CN](int t) { return LO[t] + CN[t]; }, i + 1, h);
+      int b = a < m ? a + lb_x(m - a, [LO, a](int t) { return LO[a + t]; }, i + 1, 0) : a;
+      for (int t = a; t < b; ++t) src(jp, CI(d, jp, t + 1));
+    }
     stale(jp, ntp - 1);
   }
 }
@@ -1356,13 +1355,13 @@ __device__ void body_nb_lanes(const Dev& d, int j, int i, bool valid) {
     // the window covers [w0, w0 + NBW) but never reaches past the plane's row
     auto XF = [X, win, w0](int t) { unsigned u = (unsigned)(t - w0); return u < (unsigned)NBW ? win[u] : X[t]; };
     int n = has ? d.ntrack[ja] - 1 : 0;
-    int L = has ? lb_face_x(n, XF, xjR, i) : 0;
+    int L = has ? lb_x(n, XF, xjR, i) : 0;
     int lp = __shfl(L, lane == 0 ? 0 : lane - 1, 64);
     if (has) {
       int nta = n + 1;
       int fin = 0;
       if (i > 1) {
-        if (lane == 0) lp = lb_face_x(n, XF, d.cx[k - 1] + d.cdx[k - 1] / 2., i - 1);
+        if (lane == 0) lp = lb_x(n, XF, d.cx[k - 1] + d.cdx[k - 1] / 2., i - 1);
         fin = lp < nta - 2 ? lp : nta - 2;
       }
       int s0 = fin;
@@ -1375,10 +1374,6 @@ __device__ void body_nb_lanes(const Dev& d, int j, int i, bool valid) {
   if (!cell) return;
   d.nbdj[k] = j - 1; d.nbdlo[k] = lo2[0]; d.nbdcnt[k] = cnt2[0];
   d.nbuj[k] = j + 1; d.nbulo[k] = lo2[1]; d.nbucnt[k] = cnt2[1];
-  for (int t = lo2[1]; t < lo2[1] + cnt2[1]; ++t) {
-    GH_ATOMIC_MIN(&d.incf[CI(d, j + 1, t)], i);
-    GH_ATOMIC_ADD(&d.dncnt[CI(d, j + 1, t)], 1);
-  }
 }
 template <bool PK>
 __global__ void __launch_bounds__(KB) k_nb(Dev d, int ni, int remap) {
@@ -2678,12 +2673,11 @@ int gammahip_create(const gammahip_config* cfg, gammahip_ctx** out) {
     e |= dalloc(c, &d.gmv, nqs * cs); e |= dalloc(c, &d.gf1, nqs * cs); e |= dalloc(c, &d.gf1n, nqs * cs);
     e |= dalloc(c, &d.nbdj, cs); e |= dalloc(c, &d.nbdlo, cs); e |= dalloc(c, &d.nbdcnt, cs);
     e |= dalloc(c, &d.nbuj, cs); e |= dalloc(c, &d.nbulo, cs); e |= dalloc(c, &d.nbucnt, cs);
-    e |= dalloc(c, &d.f1off, cs); e |= dalloc(c, &d.f1tot, nj); e |= dalloc(c, &d.incf, cs);
+    e |= dalloc(c, &d.f1off, cs); e |= dalloc(c, &d.f1tot, nj);
     e |= dalloc(c, &d.f1own, (size_t)nj * d.fcap);
     e |= dalloc(c, &d.ff1, nqs * (size_t)d.fs1);
     if (d.shock) e |= dalloc(c, &d.ff1sd, 4 * (size_t)d.fs1);
     e |= dalloc(c, &d.f1net, nqs * cs);
-    e |= dalloc(c, &d.dncnt, cs);
   }
   if (d.shock) { e |= dalloc(c, &d.sd, cs); e |= dalloc(c, &d.pspec, cs); e |= dalloc(c, &d.shk, cs); }
   if (e) return -1;
@@ -2985,7 +2979,7 @@ int gammahip_dims(gammahip_ctx* c, int* ntracks, int* nax) {
 // ---- exact state (binary resume) ----
 namespace {
 constexpr int ST_MAGIC = 0x54534847;   // "GHST"
-constexpr int ST_VERSION = 3;
+constexpr int ST_VERSION = 5;
 constexpr int ST_HDR = 16;             // ints
 // bytes: the array's size; pad: its size in the buffer (a multiple of 8)
 struct StatePart { void* p; size_t bytes, pad; };
@@ -3006,7 +3000,7 @@ std::vector<StatePart> state_parts(const Dev& d) {
     // track's transverse gradient of the previous stage (2d.cpp:901 loops
     // tracks upward), ghost tracks keep the lists of earlier stages
     add(d.gmv, sizeof(double) * nqs * cs); add(d.gf1, sizeof(double) * nqs * cs);
-    for (int* q : {d.nbdj, d.nbdlo, d.nbdcnt, d.nbuj, d.nbulo, d.nbucnt, d.f1off, d.incf, d.dncnt})
+    for (int* q : {d.nbdj, d.nbdlo, d.nbdcnt, d.nbuj, d.nbulo, d.nbucnt, d.f1off})
       add(q, sizeof(int) * cs);
     add(d.f1tot, sizeof(int) * nj); add(d.f1own, sizeof(int) * nj * d.fcap);
     add(d.ff1, sizeof(double) * nqs * fs1);
