@@ -52,8 +52,8 @@ def algorithmic_bytes(kid: int, cells: float, f1faces: float, nq: int, stage_mix
     if kid == 3:   # k_f1grad: prim, grad_mv, x, dx and the neighbour lists of each cell
         # (neighbours' values shared between cells), grad_f1 out
         return cells * ((3 * nq + 2) * d + 6 * i4)
-    if kid == 4:   # k_f1sum: each F1 face flux read twice (lower cell's slot, upper
-        # cell's down slot), slot counts/offsets in, net flux out
+    if kid == 4:   # k_f1sum: each F1 face flux read twice (by the lower cell, whose
+        # slot holds it, and by the upper cell), slot counts/offsets in, net flux out
         return cells * (nq * d + 4 * i4) + 2 * f1faces * nq * d
     if kid == 5:   # k_f1cell (k_f1grad + k_f1faces fused): per cell prim, grad_mv, x, dx and
         # the neighbour lists once, grad_f1 out; per face the upper cell's old grad_f1,

@@ -245,7 +245,9 @@ class Grid:
         kernel, bit 1 the update kernel); F1 flux sums inside the update
         kernel; cons2prim solver (C2P_BRENT or C2P_NEWTON); fused
         transverse-gradient and own-face F1 pass; fused passes (bit 0: the F1
-        neighbour search inside the transverse-gradient pass)."""
+        neighbour search inside the transverse-gradient pass; bit 1: MV
+        gradients recomputed in the F1 passes; bit 2: F1 faces one thread per
+        owner cell)."""
         for key, val in ((OPT_SOLVER, solver), (OPT_BLOCK, block), (OPT_REMAP, remap), (OPT_GRAPH, graph),
                          (OPT_MLP_THR, mlp_thr), (OPT_PERSIST, persist), (OPT_OCC, occ),
                          (OPT_F1FUSE, f1fuse), (OPT_F1CELL, f1cell), (OPT_FUSE, fuse)):

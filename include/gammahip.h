@@ -197,8 +197,11 @@ enum { GAMMAHIP_OPT_BLOCK = 0, GAMMAHIP_OPT_REMAP = 1, GAMMAHIP_OPT_GRAPH = 2,
        GAMMAHIP_OPT_MLP_THR = 3, GAMMAHIP_OPT_PERSIST = 4, GAMMAHIP_OPT_OCC = 5,
        GAMMAHIP_OPT_F1FUSE = 6, GAMMAHIP_OPT_SOLVER = 7, GAMMAHIP_OPT_F1CELL = 8,
        GAMMAHIP_OPT_FUSE = 9 };
-/* GAMMAHIP_OPT_FUSE (bit set): bit 0 runs the F1 neighbour search inside the
- * transverse-gradient pass (one kernel instead of two; 2D grids). */
+/* GAMMAHIP_OPT_FUSE (bit set, 2D grids): bit 0 runs the F1 neighbour search
+ * inside the transverse-gradient pass (one kernel instead of two); bit 1 has
+ * the F1 passes recompute the MV gradients from the primitives instead of
+ * reading them; bit 2 forms the F1 faces with one thread per owner cell
+ * instead of one per face slot.  Every combination gives the same bits. */
 int  gammahip_set_option(gammahip_ctx* ctx, int key, long long value);
 
 /* ---- learned cons2prim guess (no counterpart in the reference) ----
