@@ -579,9 +579,26 @@ GH_HD void f1face_at(const Dev& d, int j, int i, int s, int tj, int in, const do
   long long fk = (long long)j * d.fcap + s;
   for (int q = 0; q < NQ; ++q) d.ff1[q * d.fs1 + fk] = fl[q];
   if (SHK) {
-    double psf = -1.e300, psr = -1.e300;
-    double sdf = shock_sd<2, NQ>(pL, pR, 1, false, G, &psf, d.vpspec);
-    double sdr = shock_sd<2, NQ>(pR, pL, 1, true, G, &psr, d.vpspec);
+    // Forward (pL over pR) and reverse (pR over pL) shocks: with finite
+    // positive pressures at most one of the two passes the detector's entry
+    // tests (higher pressure first, a relative jump of 1e-10 or more); the
+    // other returns -1 and leaves its spectral index unset.  Each lane then
+    // evaluates only the candidate direction, so a wavefront whose lanes face
+    // both ways runs the detector's powers once instead of twice.  Other
+    // pressures (zero, infinite, NaN: the entry tests can pass both ways) run
+    // both, as written.
+    double psf = -1.e300, psr = -1.e300, sdf = -1., sdr = -1.;
+    double pl = pL[PPP], pr = pR[PPP];
+    if (pl > 0. && pr > 0. && pl < HUGE_VAL && pr < HUGE_VAL) {
+      bool fw = pl >= pr;
+      double S1[NQ], S2[NQ], ps = -1.e300;
+      for (int q = 0; q < NQ; ++q) { S1[q] = fw ? pL[q] : pR[q]; S2[q] = fw ? pR[q] : pL[q]; }
+      double sd = shock_sd<2, NQ>(S1, S2, 1, !fw, G, &ps, d.vpspec);
+      if (fw) { sdf = sd; psf = ps; } else { sdr = sd; psr = ps; }
+    } else {
+      sdf = shock_sd<2, NQ>(pL, pR, 1, false, G, &psf, d.vpspec);
+      sdr = shock_sd<2, NQ>(pR, pL, 1, true, G, &psr, d.vpspec);
+    }
     d.ff1sd[0 * d.fs1 + fk] = sdf; d.ff1sd[1 * d.fs1 + fk] = psf;
     d.ff1sd[2 * d.fs1 + fk] = sdr; d.ff1sd[3 * d.fs1 + fk] = psr;
   }
