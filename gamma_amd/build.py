@@ -20,7 +20,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 SRC = os.path.join(HERE, "csrc", "gammahip.cpp")
 DEPS = [SRC, os.path.join(HERE, "csrc", "hydro.hpp"), os.path.join(HERE, "csrc", "mlp.hpp"),
-        os.path.join(HERE, "csrc", "ddpow.hpp"),
+        os.path.join(HERE, "csrc", "ddpow.hpp"), os.path.join(HERE, "csrc", "ddtables.h"),
         os.path.join(ROOT, "include", "gammahip.h")]
 
 HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"

@@ -10,17 +10,19 @@
 // double-double arithmetic (Dekker 1971; Knuth TAOCP vol. 2, 4.2.2) with only
 // +, -, *, /, fma, frexp, ldexp and rint -- operations IEEE 754 defines
 // exactly -- so the host build and the device build return the same bits.
-// The double-double result carries about 2^-76 relative error before the
+// The double-double result carries about 2^-80 relative error before the
 // final rounding, so gh_pow is the correctly rounded x^y except when x^y lies
-// within that distance of a rounding boundary.
+// within that distance of a rounding boundary.  Table-driven (Tang 1989,
+// 1990), tables of 80-digit values in ddtables.h:
 //
-//   log x: x = m 2^e with m in [sqrt(1/2), sqrt(2)); log m = 2 atanh(t),
-//          t = (m-1)/(m+1), |t| <= 0.172, series in u = t^2 to u^15
-//          (first four terms in double-double, the rest in double);
-//   exp z: z = k log 2 + r, |r| <= log(2)/2; exp(r) = exp(r/256)^256, the
-//          inner exponential by its Taylor series to r^9/9!.
+//   log x: x = m 2^e with m in [1, 2); log m = log c + log1p((m - c)/c) for
+//          the nearest c = 1 + i/128, series to the 10th power;
+//   exp z: z = (64 k + j) log(2)/64 + r, |r| <= log(2)/128; exp z =
+//          2^k 2^(j/64) exp(r), exp(r) by its Taylor series to r^9/9!.
 #pragma once
 #include <math.h>
+
+#include "ddtables.h"
 
 #ifndef GH_HD
 #define GH_HD __host__ __device__ __forceinline__
@@ -67,44 +69,54 @@ GH_HD D2 div(D2 a, D2 b) {
   return fast_two_sum(q1, q2);
 }
 
-// natural logarithm of a positive normal double, as a double-double
+// natural logarithm of a positive double, as a double-double: x = m 2^e
+// with m in [1, 2); c = 1 + i/128 the table point nearest m (|m - c| <= 2^-8,
+// m - c exact); log x = e log 2 + log c + log1p(z), z = (m - c) (1/c),
+// |z| <= 2^-8, by the series to z^10 with the terms through z^3 in
+// double-double.  For x just below 1 (e = -1, c = 2) the first two terms
+// cancel exactly.
 GH_HD D2 log_dd(double x) {
   int e;
   double m = frexp(x, &e);                       // x = m 2^e, m in [0.5, 1)
-  if (m < 0x1.6a09e667f3bcdp-1) { m *= 2.; e -= 1; }
-  D2 t = div({m - 1., 0.}, two_sum(m, 1.));      // m - 1 is exact (Sterbenz)
-  D2 u = mul(t, t);
-  double uh = u.hi;
-  double r = 1. / 31.;
-  r = 1. / 29. + uh * r; r = 1. / 27. + uh * r; r = 1. / 25. + uh * r; r = 1. / 23. + uh * r;
-  r = 1. / 21. + uh * r; r = 1. / 19. + uh * r; r = 1. / 17. + uh * r; r = 1. / 15. + uh * r;
-  r = 1. / 13. + uh * r; r = 1. / 11. + uh * r; r = 1. / 9. + uh * r;
-  D2 s = add({0x1.2492492492492p-3, 0x1.2492492492492p-57}, mul_d(u, r));     // 1/7 + u r
-  s = add({0x1.999999999999ap-3, -0x1.999999999999ap-57}, mul(u, s));          // 1/5 + u s
-  s = add({0x1.5555555555555p-2, 0x1.5555555555555p-56}, mul(u, s));           // 1/3 + u s
-  s = add({1., 0.}, mul(u, s));
-  D2 lm = mul(mul_d(t, 2.), s);
-  D2 ln2 = {0x1.62e42fefa39efp-1, 0x1.abc9e3b39803fp-56};
-  return add(mul_d(ln2, (double)e), lm);
+  m *= 2.;
+  e -= 1;
+  int i = (int)rint((m - 1.) * 128.);            // 0 .. 128
+  double c = (double)(128 + i) * 0x1p-7;
+  const double* T = gh_logc[i];
+  D2 z = mul_d({T[2], T[3]}, m - c);
+  double zh = z.hi;
+  double q = -1. / 10.;                          // -1/4 + z/5 - ... - z^6/10
+  q = fma(q, zh, 1. / 9.); q = fma(q, zh, -1. / 8.); q = fma(q, zh, 1. / 7.);
+  q = fma(q, zh, -1. / 6.); q = fma(q, zh, 1. / 5.); q = fma(q, zh, -1. / 4.);
+  D2 z2 = mul(z, z);
+  D2 z3 = mul(z2, z);
+  D2 s = add(mul(z3, {0x1.5555555555555p-2, 0x1.5555555555555p-56}), {(z2.hi * z2.hi) * q, 0.});
+  s = add({-0.5 * z2.hi, -0.5 * z2.lo}, s);
+  s = add(z, s);
+  return add(add(mul_d({gh_ln2[0], gh_ln2[1]}, (double)e), {T[0], T[1]}), s);
 }
 
-// exp of a double-double as E * 2^k (E a double-double in [0.7, 1.5)),
-// for |z| within the range of doubles
+// exp of a double-double as E * 2^k (E a double-double in [1, 2.02)), for |z|
+// within the range of doubles: z = (64 k + j) log(2)/64 + r, j in 0..63,
+// |r| <= log(2)/128; exp z = 2^k 2^(j/64) exp(r), exp(r) by its Taylor series
+// to r^9 with the terms through r^3 in double-double.
 GH_HD D2 exp_dd2(D2 z, int* k2) {
-  D2 ln2 = {0x1.62e42fefa39efp-1, 0x1.abc9e3b39803fp-56};
-  double k = rint(z.hi * 0x1.71547652b82fep+0);
-  D2 r = add(z, mul_d({-ln2.hi, -ln2.lo}, k));
-  r.hi *= 0x1p-8; r.lo *= 0x1p-8;
+  double kk = rint(z.hi * 0x1.71547652b82fep+6);   // 64 / log 2
+  int n = (int)kk;
+  int j = n & 63;
+  D2 r = add(z, mul_d({-gh_ln2[0] * 0x1p-6, -gh_ln2[1] * 0x1p-6}, kk));
   double rh = r.hi;
-  double q = 1. / 40320.;
-  q = 1. / 5040. + rh * q; q = 1. / 720. + rh * q; q = 1. / 120. + rh * q; q = 1. / 24. + rh * q;
-  D2 E = add({0x1.5555555555555p-3, 0x1.5555555555555p-57}, mul_d(r, q));      // 1/6 + r q
-  E = add({0.5, 0.}, mul(r, E));
-  E = add({1., 0.}, mul(r, E));
-  E = add({1., 0.}, mul(r, E));
-  for (int n = 0; n < 8; ++n) E = mul(E, E);
-  *k2 = (int)k;
-  return E;
+  double q = 1. / 362880.;                           // 1/24 + r/120 + ... + r^5/9!
+  q = fma(q, rh, 1. / 40320.); q = fma(q, rh, 1. / 5040.); q = fma(q, rh, 1. / 720.);
+  q = fma(q, rh, 1. / 120.); q = fma(q, rh, 1. / 24.);
+  D2 r2 = mul(r, r);
+  D2 r3 = mul(r2, r);
+  D2 s = add(mul(r3, {0x1.5555555555555p-3, 0x1.5555555555555p-57}), {(r2.hi * r2.hi) * q, 0.});
+  s = add({0.5 * r2.hi, 0.5 * r2.lo}, s);
+  s = add(r, s);
+  s = add({1., 0.}, s);
+  *k2 = (n - j) / 64;
+  return mul({gh_exp2j[j][0], gh_exp2j[j][1]}, s);
 }
 
 // exp of a double-double, rounded to double

@@ -4,9 +4,9 @@
  * glibc: its pow rounds differently in about a quarter of all cases.  Both
  * sides therefore evaluate powers with the same double-double algorithm
  * written with IEEE-exact operations only (+, -, *, /, fma, frexp, ldexp,
- * rint): x^y = exp(y log x), log by 2 atanh((m-1)/(m+1)) on m in
- * [sqrt(1/2), sqrt(2)), exp by exp(r/256)^256 with |r| <= log(2)/2 (Dekker
- * 1971 double-double arithmetic).  This file restates that algorithm in C,
+ * rint): x^y = exp(y log x), log and exp table-driven (Tang 1989, 1990) on
+ * the 80-digit double-double tables of og_tables.h (Dekker 1971 double-double
+ * arithmetic).  This file restates that algorithm in C,
  * operation for operation; tests/test_oracle_pins.py checks it against the C
  * library's pow (equal in all but about 1 case in 1200, never more than one
  * unit in the last place apart) and against correctly rounded values.
@@ -14,6 +14,7 @@
 #ifndef OG_POW_H
 #define OG_POW_H
 #include <math.h>
+#include "og_tables.h"
 
 typedef struct { double hi, lo; } og_d2;
 
@@ -49,40 +50,50 @@ static inline og_d2 og_dd_div(og_d2 a, og_d2 b) {
   double q2 = r.hi / b.hi;
   return og_fast_two_sum(q1, q2);
 }
+/* log x = e log 2 + log c + log1p(z): x = m 2^e, m in [1, 2), c = 1 + i/128
+ * nearest m, z = (m - c)(1/c), series to z^10 (terms through z^3 in
+ * double-double); tables og_tables.h */
 static inline og_d2 og_log_dd(double x) {
   int e;
   double m = frexp(x, &e);
-  if (m < 0x1.6a09e667f3bcdp-1) { m *= 2.; e -= 1; }
-  og_d2 t = og_dd_div(og_mk(m - 1., 0.), og_two_sum(m, 1.));
-  og_d2 u = og_dd_mul(t, t);
-  double uh = u.hi;
-  double r = 1. / 31.;
-  r = 1. / 29. + uh * r; r = 1. / 27. + uh * r; r = 1. / 25. + uh * r; r = 1. / 23. + uh * r;
-  r = 1. / 21. + uh * r; r = 1. / 19. + uh * r; r = 1. / 17. + uh * r; r = 1. / 15. + uh * r;
-  r = 1. / 13. + uh * r; r = 1. / 11. + uh * r; r = 1. / 9. + uh * r;
-  og_d2 s = og_dd_add(og_mk(0x1.2492492492492p-3, 0x1.2492492492492p-57), og_dd_mul_d(u, r));
-  s = og_dd_add(og_mk(0x1.999999999999ap-3, -0x1.999999999999ap-57), og_dd_mul(u, s));
-  s = og_dd_add(og_mk(0x1.5555555555555p-2, 0x1.5555555555555p-56), og_dd_mul(u, s));
-  s = og_dd_add(og_mk(1., 0.), og_dd_mul(u, s));
-  og_d2 lm = og_dd_mul(og_dd_mul_d(t, 2.), s);
-  og_d2 ln2 = og_mk(0x1.62e42fefa39efp-1, 0x1.abc9e3b39803fp-56);
-  return og_dd_add(og_dd_mul_d(ln2, (double)e), lm);
+  m *= 2.;
+  e -= 1;
+  int i = (int)rint((m - 1.) * 128.);
+  double c = (double)(128 + i) * 0x1p-7;
+  const double *T = og_logc[i];
+  og_d2 z = og_dd_mul_d(og_mk(T[2], T[3]), m - c);
+  double zh = z.hi;
+  double q = -1. / 10.;
+  q = fma(q, zh, 1. / 9.); q = fma(q, zh, -1. / 8.); q = fma(q, zh, 1. / 7.);
+  q = fma(q, zh, -1. / 6.); q = fma(q, zh, 1. / 5.); q = fma(q, zh, -1. / 4.);
+  og_d2 z2 = og_dd_mul(z, z);
+  og_d2 z3 = og_dd_mul(z2, z);
+  og_d2 s = og_dd_add(og_dd_mul(z3, og_mk(0x1.5555555555555p-2, 0x1.5555555555555p-56)),
+                      og_mk((z2.hi * z2.hi) * q, 0.));
+  s = og_dd_add(og_mk(-0.5 * z2.hi, -0.5 * z2.lo), s);
+  s = og_dd_add(z, s);
+  return og_dd_add(og_dd_add(og_dd_mul_d(og_mk(og_ln2[0], og_ln2[1]), (double)e), og_mk(T[0], T[1])), s);
 }
+/* exp z = 2^k 2^(j/64) exp(r): z = (64 k + j) log(2)/64 + r, |r| <= log(2)/128,
+ * Taylor series to r^9 (terms through r^3 in double-double); E in [1, 2.02) */
 static inline og_d2 og_exp_dd2(og_d2 z, int *k2) {
-  og_d2 ln2 = og_mk(0x1.62e42fefa39efp-1, 0x1.abc9e3b39803fp-56);
-  double k = rint(z.hi * 0x1.71547652b82fep+0);
-  og_d2 r = og_dd_add(z, og_dd_mul_d(og_mk(-ln2.hi, -ln2.lo), k));
-  r.hi *= 0x1p-8; r.lo *= 0x1p-8;
+  double kk = rint(z.hi * 0x1.71547652b82fep+6);
+  int n = (int)kk;
+  int j = n & 63;
+  og_d2 r = og_dd_add(z, og_dd_mul_d(og_mk(-og_ln2[0] * 0x1p-6, -og_ln2[1] * 0x1p-6), kk));
   double rh = r.hi;
-  double q = 1. / 40320.;
-  q = 1. / 5040. + rh * q; q = 1. / 720. + rh * q; q = 1. / 120. + rh * q; q = 1. / 24. + rh * q;
-  og_d2 E = og_dd_add(og_mk(0x1.5555555555555p-3, 0x1.5555555555555p-57), og_dd_mul_d(r, q));
-  E = og_dd_add(og_mk(0.5, 0.), og_dd_mul(r, E));
-  E = og_dd_add(og_mk(1., 0.), og_dd_mul(r, E));
-  E = og_dd_add(og_mk(1., 0.), og_dd_mul(r, E));
-  for (int n = 0; n < 8; ++n) E = og_dd_mul(E, E);
-  *k2 = (int)k;
-  return E;
+  double q = 1. / 362880.;
+  q = fma(q, rh, 1. / 40320.); q = fma(q, rh, 1. / 5040.); q = fma(q, rh, 1. / 720.);
+  q = fma(q, rh, 1. / 120.); q = fma(q, rh, 1. / 24.);
+  og_d2 r2 = og_dd_mul(r, r);
+  og_d2 r3 = og_dd_mul(r2, r);
+  og_d2 s = og_dd_add(og_dd_mul(r3, og_mk(0x1.5555555555555p-3, 0x1.5555555555555p-57)),
+                      og_mk((r2.hi * r2.hi) * q, 0.));
+  s = og_dd_add(og_mk(0.5 * r2.hi, 0.5 * r2.lo), s);
+  s = og_dd_add(r, s);
+  s = og_dd_add(og_mk(1., 0.), s);
+  *k2 = (n - j) / 64;
+  return og_dd_mul(og_mk(og_exp2j[j][0], og_exp2j[j][1]), s);
 }
 static inline double og_exp_dd(og_d2 z) {
   if (z.hi > 709.8) return HUGE_VAL;
