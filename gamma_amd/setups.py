@@ -341,6 +341,41 @@ def ensemble_1d(n_runs: int = 256, sizes=(512, 16384), lfacs=(2., 300.), seed=DE
     return ens, members
 
 
+def riemann_2d_uneven(nx: int = 96, ny: int = 16, gamma=5. / 3., seed=DEFAULT_SEED) -> Setup:
+    """Test configuration (not a benchmark): the 2D Riemann problem with
+    tracks of different lengths, track a holding n_a = nx - (37 a mod nx/2)
+    uniform cells over [0, 1] (between nx/2 and nx), so that neighbouring
+    tracks' faces are misaligned from the start and a cell's F1 neighbours sit
+    at indices far from its own (up to half the track away): lists of one to
+    three cells per side, neighbour searches that leave the cell's
+    neighbourhood, incoming faces from several lower cells.  Same states,
+    perturbation, boundaries and AR2D rule as riemann_2d; dy = 1/nx."""
+    n = np.array([nx - (37 * a) % (nx // 2) for a in range(ny)], dtype=np.int64)
+    cfg = RunConfig(nd=2, geom=GEOM_CART, nmax=(nx * 5) // 4, ncell_mv=nx,
+                    ncell_f1=ny, gamma=gamma, kin_mode=KIN_FIXED, v_in=0., v_out=0.,
+                    rg_mode=RG_AR2D, rg=(3., 0.2, 0., 0., 0., 0.))
+    dyc = 1. / nx
+    y = (np.arange(ny) + 0.5) * dyc
+    x = np.zeros((ny, nx))
+    dx = np.ones((ny, nx))
+    prim = np.zeros((cfg.nq, ny, nx))
+    xi = splitmix_uniform(seed, np.arange(nx * ny, dtype=np.uint64)).reshape(ny, nx)
+    for a in range(ny):
+        m = int(n[a])
+        xa = (np.arange(m) + 0.5) / m
+        x[a, :m] = xa
+        dx[a, :m] = 1. / m
+        x[a, m:] = xa[-1]
+        left = xa <= 0.5
+        prim[RHO, a, :m] = np.where(left, 10., 1.) * (1. + 1.e-3 * xi[a, :m])
+        prim[PPP, a, :m] = np.where(left, 13.33, 1.e-6)
+        prim[TR1, a, :m] = np.where(left, 1., 2.)
+        prim[RHO, a, m:] = 1.
+        prim[PPP, a, m:] = 1.e-6
+    return Setup("riemann_2d_uneven", cfg, 0., x, dx, y, np.full(ny, dyc), prim,
+                 meta=dict(seed=seed), ncells=n)
+
+
 def riemann_2d(nx: int = 2048, ny: int = 2048, rho_l=10., p_l=13.33, rho_r=1.,
                p_r=1.e-6, gamma=5. / 3., amp=1.e-3, seed=DEFAULT_SEED,
                nmax=None, regrid=True, ly: float = 1.) -> Setup:

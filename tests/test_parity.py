@@ -31,6 +31,7 @@ CASES = {
                                                    geom=setups.GEOM_SPH, shock=1), 80),
     "blandford_mckee": (lambda: setups.blandford_mckee_1d(400), 60),
     "riemann_2d": (lambda: setups.riemann_2d(64, 16), 60),
+    "riemann_2d_uneven": (lambda: setups.riemann_2d_uneven(96, 16), 60),
     # edge cells grow, split next to the ghosts, which must follow their faces
     "expanding_tube": (lambda: setups.expanding_tube_1d(200), 300),
     # the same with six spare slots: tracks fill their storage and splits wait
@@ -362,7 +363,7 @@ def test_gpu_pow_equals_oracle_bitwise():
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("case", ["shock_tube_cart", "riemann_2d", "bm_jet_2d"])
+@pytest.mark.parametrize("case", ["shock_tube_cart", "riemann_2d", "riemann_2d_uneven", "bm_jet_2d"])
 @pytest.mark.parametrize("opts", [
     dict(block=128, remap=1, graph=1, persist=0),
     dict(block=256, remap=0, graph=0, persist=4, f1fuse=1),
@@ -398,7 +399,7 @@ def test_gpu_parity_launch_options(case, opts):
     print(f"[gpu parity options] {case} {opts}: {res}", flush=True)
 
 
-@pytest.mark.parametrize("case", ["riemann_2d", "bm_jet_2d"])
+@pytest.mark.parametrize("case", ["riemann_2d", "riemann_2d_uneven", "bm_jet_2d"])
 @pytest.mark.parametrize("fuse", [2, 4, 6])
 def test_emu_fused_options_are_bitwise(emu_lib, case, fuse):
     """The F1 passes recomputing MV gradients from the primitives
