@@ -16,7 +16,10 @@
   128 x 32 without regrid, 2e-16 without the perturbation, where the
   full-size run is checked against the oracle's strip instead).
 * C3, the 2D spherical jet (1024 theta tracks of 500-4000 cells): the packed
-  launch reproduces the plain one bit for bit.
+  launch reproduces the plain one bit for bit; its 48 tracks nearest the axis
+  (about 4000 cells each) run on their own equal the oracle bit for bit.
+* C2's problem with the perturbation at 512 x 128 equals the oracle bit for
+  bit (the full size is checked against the oracle's strip without it).
 * A state restored into a packed grid continues as the unsplit run (the tile
   list follows the restored track lengths).
 """
@@ -136,6 +139,56 @@ def test_gpu_jet_1024_tracks_packed_equals_plain():
     assert np.array_equal(sa, sb) and np.array_equal(ma, mb)
     assert sa.sum() + ma.sum() > 0
     print(f"[gpu configs] C3 bitwise; splits {sa.sum()} merges {ma.sum()}", flush=True)
+
+
+def _equals_oracle(s, nsteps, label):
+    """The default GPU launch of s against the oracle, bit for bit: every field
+    of every active cell, the cells' positions and widths, shock flags, track
+    lengths, regrid counts and time.  Returns (splits, merges)."""
+    print(f"[gpu configs] {label}: oracle, {nsteps} steps", flush=True)
+    o = oc.OracleGrid(s)
+    o.run(nsteps)
+    print(f"[gpu configs] {label}: GPU", flush=True)
+    g = hip.Grid(s)
+    g.step(nsteps)
+    assert np.array_equal(o.tracks()["ntrack"], g.tracks()["ntrack"]), "track lengths differ"
+    so, mo = o.regrid_counts()
+    sg, mg = g.regrid_counts()
+    assert np.array_equal(so, sg) and np.array_equal(mo, mg), "regrid counts differ"
+    fields = [(q, q) for q in range(s.cfg.nq)] + [(hip.F_X, oc.F_X), (hip.F_DX, oc.F_DX)]
+    if s.cfg.shock:
+        fields.append((hip.F_SHOCKED, oc.F_SHOCKED))
+    for fg, fo in fields:
+        for j, (a, b) in enumerate(zip(g.active(fg), o.active(fo))):
+            assert np.array_equal(a, b, equal_nan=True), f"field {fg} differs on active track {j}"
+    assert g.time()[0][0] == o.t
+    st = g.stats()
+    for k in ("c2p_nobracket", "c2p_maxiter", "nan_flux", "negative_dt", "f1_overflow"):
+        assert st[k] == 0, (k, st)
+    print(f"[gpu configs] {label}: bit for bit; splits {sg.sum()} merges {mg.sum()}", flush=True)
+    return int(sg.sum()), int(mg.sum())
+
+
+@pytest.mark.gpu
+def test_gpu_jet_core_tracks_full_length_equal_the_oracle():
+    """C3 at its real track lengths: the 48 tracks nearest the axis of the
+    1024-track jet (same theta, widths and seeded lengths, about 4000 cells
+    each, 189 k cells), 20 steps with shock detection, synchrotron particles
+    and the BM1D rule splitting behind the shock, bit for bit the oracle."""
+    s = setups.bm_jet_2d(ntheta=48, theta_max=0.5 * 48 / 1024)
+    splits, merges = _equals_oracle(s, 20, "C3 core: 48 theta tracks of the 1024-track jet")
+    assert splits > 0
+
+
+@pytest.mark.gpu
+def test_gpu_riemann_perturbed_512_by_128_equals_the_oracle():
+    """C2's problem with its seeded perturbation and AR2D regrid on square
+    cells (512 x 128 over [0, 1] x [0, 0.25]), 100 steps, bit for bit the
+    oracle: the perturbation makes every track's F1 lists and transverse
+    gradients differ, unlike the uniform full-size check."""
+    s = setups.riemann_2d(512, 128, ly=0.25)
+    splits, merges = _equals_oracle(s, 100, "C2 perturbed 512 x 128")
+    assert splits + merges > 0
 
 
 @pytest.mark.gpu
