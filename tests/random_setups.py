@@ -22,13 +22,14 @@ def _widths(rng, n, length):
     return w * (length / w.sum())
 
 
-def _states(rng, nd):
+def _states(rng, nd, strong=False):
     """Two primitive states (rho, p, v1, v2, tracer)."""
     out = []
     for side in range(2):
-        rho = 10. ** rng.uniform(-1., 1.)
-        p = 10. ** rng.uniform(-2., 1.)
-        speed = rng.uniform(0., 0.6)
+        rho = 10. ** rng.uniform(-2., 2.) if strong else 10. ** rng.uniform(-1., 1.)
+        p = 10. ** rng.uniform(-6., 1.) if strong else 10. ** rng.uniform(-2., 1.)
+        # strong: Lorentz factors up to 20
+        speed = np.sqrt(1. - 1. / rng.uniform(1., 20.) ** 2) if strong else rng.uniform(0., 0.6)
         ang = rng.uniform(0., 2. * np.pi)
         v1 = speed * np.cos(ang)
         v2 = speed * np.sin(ang) if nd == 2 else 0.
@@ -36,7 +37,7 @@ def _states(rng, nd):
     return out
 
 
-def random_setup(seed: int) -> S.Setup:
+def random_setup(seed: int, strong: bool = False) -> S.Setup:
     rng = np.random.default_rng(seed)
     nd = 2 if rng.uniform() < 0.75 else 1
     geom = S.GEOM_SPH if rng.uniform() < 0.4 else S.GEOM_CART
@@ -68,7 +69,7 @@ def random_setup(seed: int) -> S.Setup:
     x = np.zeros((ntr, nmv))
     dx = np.ones((ntr, nmv))
     prim = np.zeros((cfg.nq, ntr, nmv))
-    (rl, pl, ul, wl, tl), (rr, pr, ur, wr, tr) = _states(rng, nd)
+    (rl, pl, ul, wl, tl), (rr, pr, ur, wr, tr) = _states(rng, nd, strong)
     xd = x0 + rng.uniform(0.3, 0.7) * length
     for a in range(ntr):
         m = int(lens[a])
