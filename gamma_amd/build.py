@@ -9,6 +9,10 @@ next to their sources, git-ignored but shipped with the tree).
 * ``gamma_amd/host/shock_tube`` (and ``tests/shock_tube_emu``) -- a C++
   caller of the hot path through ``gamma_amd/host/gammahip_grid.hpp``, linked
   against the HIP library (and against the CPU build, for tests).
+
+The user setup's hooks come from ``gamma_amd/csrc/user_hooks.h``, or from the
+file the environment variable GAMMAHIP_USER_HOOKS names; the kernels (HIP and
+CPU builds) and the oracle are built with the same file.
 """
 from __future__ import annotations
 
@@ -21,7 +25,7 @@ ROOT = os.path.dirname(HERE)
 SRC = os.path.join(HERE, "csrc", "gammahip.cpp")
 DEPS = [SRC, os.path.join(HERE, "csrc", "hydro.hpp"), os.path.join(HERE, "csrc", "mlp.hpp"),
         os.path.join(HERE, "csrc", "ddpow.hpp"), os.path.join(HERE, "csrc", "ddtables.h"),
-        os.path.join(ROOT, "include", "gammahip.h")]
+        os.path.join(HERE, "csrc", "user_hooks.h"), os.path.join(ROOT, "include", "gammahip.h")]
 
 HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
 # -ffp-contract=off: no fused multiply-add, every expression rounds as written
@@ -33,6 +37,22 @@ EMU_FLAGS = ["-O2", "-std=c++17", "-DGH_EMU", "-x", "c++", "-ffp-contract=off", 
              "-shared"]
 
 
+def user_hooks() -> str | None:
+    """The user hooks file named by GAMMAHIP_USER_HOOKS (absolute), or None."""
+    p = os.environ.get("GAMMAHIP_USER_HOOKS")
+    if not p:
+        return None
+    p = os.path.abspath(p)
+    if not os.path.isfile(p):
+        raise RuntimeError(f"GAMMAHIP_USER_HOOKS={p}: no such file")
+    return p
+
+
+def _hooks_flags() -> list:
+    p = user_hooks()
+    return [f'-DGH_USER_HOOKS="{p}"'] if p else []
+
+
 def _run(cmd):
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
@@ -42,7 +62,7 @@ def _run(cmd):
 def build_hip(out: str | None = None) -> str:
     out = out or os.path.join(HERE, "libgammahip.so")
     tmp = out + ".tmp"
-    _run([HIPCC, *HIP_FLAGS, SRC, "-o", tmp])
+    _run([HIPCC, *HIP_FLAGS, *_hooks_flags(), SRC, "-o", tmp])
     os.replace(tmp, out)
     return out
 
@@ -50,14 +70,15 @@ def build_hip(out: str | None = None) -> str:
 def build_emu(out: str | None = None) -> str:
     out = out or os.path.join(ROOT, "tests", "libgammahip_emu.so")
     tmp = out + ".tmp"
-    _run(["g++", *EMU_FLAGS, SRC, "-o", tmp])
+    _run(["g++", *EMU_FLAGS, *_hooks_flags(), SRC, "-o", tmp])
     os.replace(tmp, out)
     return out
 
 
 def build_oracle() -> str:
     d = os.path.join(ROOT, "oracle")
-    _run(["make", "-s", "-B", "-C", d])
+    p = user_hooks()
+    _run(["make", "-s", "-B", "-C", d] + ([f"USER_HOOKS={p}"] if p else []))
     return os.path.join(d, "libgamma_oracle.so")
 
 

@@ -14,6 +14,17 @@
 #include "hydro.hpp"
 #include "mlp.hpp"
 #include "../../include/gammahip.h"
+// the user setup's hooks (GAMMAHIP_USER_HOOKS names another file at build time)
+#ifdef GH_EMU
+#define GH_USER_FN static inline
+#else
+#define GH_USER_FN static inline __host__ __device__
+#endif
+#ifdef GH_USER_HOOKS
+#include GH_USER_HOOKS
+#else
+#include "user_hooks.h"
+#endif
 
 #include <stdio.h>
 #include <stdlib.h>
@@ -51,9 +62,9 @@ enum { CNT_C2P_CALLS = 0, CNT_C2P_ITERS, CNT_NOBRACKET, CNT_MAXITER, CNT_PHICAP,
 // each wavefront (the update counts every cons2prim) a queue of ~10^5
 // same-address atomics per launch.
 constexpr int CNT_PARTS = 1024;
-enum { KIN_NONE = 0, KIN_FIXED = 1 };
-enum { BND_NONE = 0, BND_BM = 1 };
-enum { RG_NONE = 0, RG_AR2D = 1, RG_BM1D = 2 };
+enum { KIN_NONE = 0, KIN_FIXED = 1, KIN_USER = 2 };
+enum { BND_NONE = 0, BND_BM = 1, BND_USER = 2 };
+enum { RG_NONE = 0, RG_AR2D = 1, RG_BM1D = 2, RG_USER = 3 };
 
 // ---------------------------------------------------------------------------
 // device view (passed by value to every kernel)
@@ -94,9 +105,18 @@ struct Dev {
   int* tiles;              // [cap][2] = (track, chunk)
   int* ntiles;             // [1]
   double* ttrig;           // [nj][7] Trig of each track (2D spherical geometry)
+  double* upar;            // [GH_USER_NPAR] parameters of the user hooks
 };
 
 GH_HD long long CI(const Dev& d, int j, int i) { return (long long)j * d.nax + i; }
+
+// the track as the user hooks see it
+GH_HD gh_user_track user_track(const Dev& d, int j) {
+  gh_user_track T;
+  T.j = j; T.ntrack = d.ntrack[j]; T.ngst = d.ngst; T.geom = d.tgeom[j]; T.t = d.tt[j];
+  T.y = d.ty[j]; T.dy = d.tdy[j]; T.par = d.upar;
+  return T;
+}
 
 // this workgroup's copy of counter id
 GH_HD unsigned long long* cnt_at(const Dev& d, int id) {
@@ -209,6 +229,17 @@ GH_HD void body_ghost_tracks(const Dev& d, int g, int i) {
   if (i < nts - 1) d.fx[CI(d, j, i)] = d.fx[CI(d, js, i)];
 }
 
+// userBoundaries of a user setup on ghost cell (j, i): its primitives after
+// the outflow copy, in the reference's slots, through gh_user_ghost
+template <int ND, int NQ>
+GH_HD void user_ghost(const Dev& d, int j, int i, long long kc) {
+  double p[NQ];
+  load_v<ND, NQ>(d, d.prim, kc, p);
+  gh_user_track T = user_track(d, j);
+  gh_user_ghost(&T, i, d.cx[kc], d.cdx[kc], p);
+  store_v<ND, NQ>(d, d.prim, kc, p);
+}
+
 // ghost cells and faces along the track + userBoundaries (BM1D.cpp:254-274)
 template <int ND, int NQ>
 GH_HD void body_ghost_mv(const Dev& d, int j, int kk) {
@@ -230,6 +261,7 @@ GH_HD void body_ghost_mv(const Dev& d, int j, int kk) {
       d.prim[PPP * d.cs + kc] = p;
       d.prim[sq<ND>(TR1) * d.cs + kc] = 2.;
     }
+    if (d.tbnd[j] == BND_USER) user_ghost<ND, NQ>(d, j, i, kc);
   } else {
     int i = iR + (kk - d.ngst);
     long long ke = CI(d, j, iR - 1), kc = CI(d, j, i);
@@ -237,6 +269,7 @@ GH_HD void body_ghost_mv(const Dev& d, int j, int kk) {
     copy_cell<ND>(d, kc, ke);
     d.cx[kc] = d.cx[ke] + (i - iR + 1) * dxe;
     if (i <= nt - 2) d.fx[kc] = d.fx[ke] + (i - iR) * dxe;   // (i-iRbnd): 1d.cpp:241
+    if (d.tbnd[j] == BND_USER) user_ghost<ND, NQ>(d, j, i, kc);
   }
 }
 
@@ -293,6 +326,9 @@ GH_HD void body_faces_s(const Dev& d, int j, int i) {
       if (i == n) v = d.tvin[j];
       if (i == nt - 2 - n) v = d.tvout[j];
     }
+  } else if (d.tkin[j] == KIN_USER) {
+    gh_user_track T = user_track(d, j);
+    v = gh_user_face_speed(&T, i, xf, v);
   }
   Trig T = trig_at<ND>(d, j);
   double dA = face_area<ND>(0, xf, d.tdy[j], geom, T.s_y, T.s_hdy);
@@ -575,7 +611,7 @@ GH_HD void f1face_at(const Dev& d, int j, int i, int s, int tj, int in, const do
   hllc_flux_lazy<2, NQ>(pL, cL, pR, cR, w, 0., dA, 1, xm, geom, [&](bool right, double* f) {
     if (right) state2flux<2, NQ>(pR, cR, f, 1, rR, geom);
     else state2flux<2, NQ>(pL, cL, f, 1, rL, geom);
-  }, fl);
+  }, fl, 0.);
   long long fk = (long long)j * d.fcap + s;
   for (int q = 0; q < NQ; ++q) d.ff1[q * d.fs1 + fk] = fl[q];
   if (SHK) {
@@ -999,6 +1035,13 @@ GH_HD int rg_check(const Dev& d, int j, int i, double iS) {
     if (ar > d.rg[0] * target / gh_pow(lf, 3. / 2.)) return RG_SPLIT;
     if (ar < d.rg[1] * target / gh_pow(lf, 3. / 2.)) return RG_MERGE;
     return RG_SKIP;
+  }
+  if (mode == RG_USER) {
+    double p[NQ];
+    load_v<ND, NQ>(d, d.prim, k, p);
+    gh_user_track T = user_track(d, j);
+    int a = gh_user_regrid(&T, i, d.cx[k], d.cdx[k], p);
+    return a == 2 ? RG_SPLIT : a == 1 ? RG_MERGE : RG_SKIP;
   }
   return RG_SKIP;
 }
@@ -2625,7 +2668,7 @@ int gammahip_create(const gammahip_config* cfg, gammahip_ctx** out) {
   if (cfg->ncell_mv < 8 || cfg->nmax < cfg->ncell_mv) return fail(c, "bad ncell_mv / nmax");
   if (cfg->ntracks_active < 1) return fail(c, "bad ntracks_active");
   if (cfg->nd == 2 && cfg->geom < 0) return fail(c, "2D needs a single geometry");
-  if (cfg->rg_mode < RG_NONE || cfg->rg_mode > RG_BM1D) return fail(c, "unknown rg_mode");
+  if (cfg->rg_mode < RG_NONE || cfg->rg_mode > RG_USER) return fail(c, "unknown rg_mode");
   if (cfg->timestep != 0 && cfg->timestep != 1) return fail(c, "timestep must be 0 (RK3) or 1 (Euler)");
   if (cfg->recon != 0 && cfg->recon != 1) return fail(c, "recon must be 0 (piecewise linear) or 1 (piecewise constant)");
   if (cfg->eos != 0 && cfg->eos != 1) return fail(c, "eos must be 0 (ideal) or 1 (Synge)");
@@ -2675,6 +2718,7 @@ int gammahip_create(const gammahip_config* cfg, gammahip_ctx** out) {
   int e = 0;
   e |= dalloc(c, &d.tiles, (size_t)2 * nj * ((d.nax + 63) / 64)); e |= dalloc(c, &d.ntiles, 1);
   e |= dalloc(c, &d.ttrig, (size_t)7 * nj);
+  e |= dalloc(c, &d.upar, (size_t)GH_USER_NPAR);
   e |= dalloc(c, &d.ntrack, nj); e |= dalloc(c, &d.trg, nj); e |= dalloc(c, &d.tgeom, nj); e |= dalloc(c, &d.tkin, nj);
   e |= dalloc(c, &d.tbnd, nj); e |= dalloc(c, &d.tfirst, nj); e |= dalloc(c, &d.tiS, nj);
   e |= dalloc(c, &d.tvin, nj); e |= dalloc(c, &d.tvout, nj); e |= dalloc(c, &d.tbm, 4 * nj);
@@ -2740,7 +2784,9 @@ int gammahip_load(gammahip_ctx* c, const double* x, const double* dx, const doub
     const gammahip_track_hooks& h = hooks[a];
     int nc = h.ncell > 0 ? h.ncell : nm;
     if (nc < 2 || nc > nm) return fail(c, "track " + std::to_string(a) + ": ncell outside 2..ncell_mv");
-    if (h.rg_mode < -1 || h.rg_mode > RG_BM1D) return fail(c, "track " + std::to_string(a) + ": unknown rg_mode");
+    if (h.rg_mode < -1 || h.rg_mode > RG_USER) return fail(c, "track " + std::to_string(a) + ": unknown rg_mode");
+    if (h.kin_mode < KIN_NONE || h.kin_mode > KIN_USER) return fail(c, "track " + std::to_string(a) + ": unknown kin_mode");
+    if (h.bnd_mode < BND_NONE || h.bnd_mode > BND_USER) return fail(c, "track " + std::to_string(a) + ": unknown bnd_mode");
     ntrack[j] = nc + 2 * ng;
     trg[j] = h.rg_mode >= 0 ? h.rg_mode : cf.rg_mode;
     d.rg_mode |= trg[j] != RG_NONE ? (1 << trg[j]) : 0;
@@ -2862,6 +2908,18 @@ int gammahip_mlp_load(gammahip_ctx* c, const float* params) {
   d.guess = 1;
   return 0;
 #endif
+}
+
+int gammahip_set_user_params(gammahip_ctx* c, const double* par, int n) {
+  if (!c) return -1;
+  if (n < 0 || n > GH_USER_NPAR || (n > 0 && !par))
+    return fail(c, "user parameters: n must be in 0.." + std::to_string((int)GH_USER_NPAR));
+  double v[GH_USER_NPAR] = {0.};
+  for (int q = 0; q < n; ++q) v[q] = par[q];
+#ifndef GH_EMU
+  HIPCHK(hipStreamSynchronize(c->stream));
+#endif
+  return h2d(c, c->d.upar, v, sizeof(v));
 }
 
 int gammahip_set_option(gammahip_ctx* c, int key, long long value) {

@@ -579,18 +579,24 @@ GH_HD void hllc_flux(const double* pL, const double* cL, const double* fL,
 // The same flux with the physical flux formed only for the side the face's
 // wave region uses: side(right, f) writes the flux of the left (false) or
 // right (true) state.  Same arithmetic as hllc_flux; one state2flux instead of
-// two and six fewer live values per state in the face kernels.
+// two and six fewer live values per state in the face kernels.  Where no
+// region matches (a NaN wave speed: a reconstructed state without a sound
+// speed) the flux is `none`: NaN on MV faces (counted by the caller), 0 on F1
+// faces -- the reference forms each F1 face in a fresh local Interface
+// (2d.cpp:923) whose flux the constructor leaves unset (interface.cpp:10-17)
+// and computeFlux then leaves untouched; the zero-initialised face is the
+// reading taken.
 template <int ND, int NQ, class Side>
 GH_HD void hllc_flux_lazy(const double* pL, const double* cL, const double* pR, const double* cR,
                           const Waves& w, double v, double dA, int dim, double rface, int geom, Side side,
-                          double* flux) {
+                          double* flux, double none = __builtin_nan("")) {
   int region = -1;
   if (v < w.lL) region = 0;
   if (w.lL <= v && v < w.lS) region = 1;
   if (w.lS <= v && v < w.lR) region = 2;
   if (w.lR <= v) region = 3;
   if (region < 0) {
-    for (int q = 0; q < NQ; ++q) flux[q] = __builtin_nan("");
+    for (int q = 0; q < NQ; ++q) flux[q] = none;
     return;
   }
   bool right = region >= 2;

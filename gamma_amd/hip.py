@@ -82,6 +82,7 @@ def load_library(path: str = LIB_PATH):
         "gammahip_time": (I, [P, P, P]), "gammahip_stats_get": (I, [P, C.POINTER(Stats)]),
         "gammahip_regrid_counts": (I, [P, P, P]), "gammahip_kernel_times": (I, [P, P, P]),
         "gammahip_sync": (I, [P]), "gammahip_set_option": (I, [P, I, C.c_longlong]),
+        "gammahip_set_user_params": (I, [P, P, I]),
         "gammahip_mlp_nparams": (I, []),
         "gammahip_mlp_train": (I, [P, C.c_longlong, P, I, I, C.c_float, C.c_ulonglong, P]),
         "gammahip_mlp_grad": (I, [P, P, I, P, P]),
@@ -175,6 +176,8 @@ class Grid:
             self.L.gammahip_destroy(h)
             self.h = None
             raise RuntimeError("gammahip_create: " + msg)
+        if rc.upar:
+            self.set_user_params(rc.upar)       # before load: its first ghost update calls the hooks
         if hooks_per_track is None:
             hooks_per_track = setup_hooks(setup)
         arr = (TrackHooks * len(hooks_per_track))(*hooks_per_track)
@@ -188,6 +191,11 @@ class Grid:
     def _check(self, r):
         if r != 0:
             raise RuntimeError(self.L.gammahip_last_error(self.h).decode())
+
+    def set_user_params(self, par):
+        """Parameters of the user setup's hooks (gammahip_set_user_params)."""
+        a = np.ascontiguousarray(par, dtype=np.float64)
+        self._check(self.L.gammahip_set_user_params(self.h, _p(a), int(a.size)))
 
     def close(self):
         if getattr(self, "h", None):

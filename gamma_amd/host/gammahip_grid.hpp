@@ -45,6 +45,12 @@ class Grid {
     check(gammahip_load(ctx_, x.data(), dx.data(), y.data(), dy.data(), prim.data(), hooks.data(), t0));
   }
 
+  // parameters of the user setup's hooks (gamma_amd/csrc/user_hooks.h), set
+  // before prepForRun: its first ghost update calls them
+  void setUserParams(const std::vector<double>& par) {
+    check(gammahip_set_user_params(ctx_, par.data(), (int)par.size()));
+  }
+
   void regrid() { check(gammahip_regrid(ctx_)); }                    // 1d.cpp:484, 2d.cpp:609
   void updateGhosts() { check(gammahip_update_ghosts(ctx_)); }       // 1d.cpp:217, 2d.cpp:241
   void prepForUpdate() { check(gammahip_prep_for_update(ctx_)); }    // grid.cpp:28-37

@@ -31,9 +31,10 @@ GEOM_CART, GEOM_SPH = 0, 1
 TS_RK3, TS_EULER = 0, 1
 RECON_PLM, RECON_PC = 0, 1
 EOS_IDEAL, EOS_SYNGE = 0, 1
-KIN_NONE, KIN_FIXED = 0, 1
-BND_NONE, BND_BM = 0, 1
-RG_NONE, RG_AR2D, RG_BM1D = 0, 1, 2
+# hook kinds; the USER kinds call the user setup's functions (gamma_amd/csrc/user_hooks.h)
+KIN_NONE, KIN_FIXED, KIN_USER = 0, 1, 2
+BND_NONE, BND_BM, BND_USER = 0, 1, 2
+RG_NONE, RG_AR2D, RG_BM1D, RG_USER = 0, 1, 2, 3
 
 # physical constants (reference constants.h:8-28)
 MP = 1.6726219e-24
@@ -82,6 +83,7 @@ class RunConfig:
     vpspec: int = 0               # VARIABLE_PSPEC_
     eos: int = EOS_IDEAL          # EOS_
     circ_regrid: int = 0          # CIRC_REGRID_
+    upar: tuple = ()              # parameters of the user setup's hooks (up to 16)
 
     @property
     def nq(self) -> int:          # NUM_Q = NUM_C(4) + NUM_TR(2) + NUM_S
@@ -400,3 +402,39 @@ def riemann_2d(nx: int = 2048, ny: int = 2048, rho_l=10., p_l=13.33, rho_r=1.,
     prim[TR1] = np.where(left, 1., 2.)
     return Setup("riemann_2d", cfg, 0., x, np.full((ny, nx), dxc), y, np.full(ny, dyc), prim,
                  meta=dict(seed=seed, amp=amp))
+
+
+def reflecting_box(nx: int = 200, ny: int = 0, mesh: float = 0.5, rho0: float = 1., p0: float = 0.01,
+                   p_hot: float = 10., r_hot: float = 0.1, gamma=4. / 3., shock: int = 1,
+                   regrid: bool = True, seed=DEFAULT_SEED) -> Setup:
+    """A setup written against the user hooks as shipped (gamma_amd/csrc/
+    user_hooks.h): a hot region of radius r_hot at the centre of the box
+    [0, 1] (1D, ny = 0) or [0, 1] x [0, ny/nx] (2D, ny x-tracks) expands
+    against reflecting walls at the track ends (gh_user_ghost: the edge
+    cell's state with the normal velocity reversed; gh_user_face_speed: the
+    end faces still); the other faces move at `mesh` times the solver's face
+    speeds; with `regrid`, cells wider than 1.5/nx split (where the density is
+    above 0.1 rho0) and cells narrower than 0.3/nx merge (gh_user_regrid).  The 2D density
+    carries a 1e-3 SplitMix64 perturbation per cell (counter j*nx + i)."""
+    nd = 2 if ny else 1
+    ntr = ny if ny else 1
+    upar = (mesh, 1.5 / nx if regrid else 0., 0.3 / nx if regrid else 0., 0.1 * rho0)
+    cfg = RunConfig(nd=nd, geom=GEOM_CART, shock=shock, nmax=(nx * 3) // 2, ncell_mv=nx, ncell_f1=ntr,
+                    gamma=gamma, kin_mode=KIN_USER, bnd_mode=BND_USER, rg_mode=RG_USER, upar=upar)
+    dxc = 1. / nx
+    x1 = (np.arange(nx) + 0.5) * dxc
+    y = (np.arange(ntr) + 0.5) * dxc if nd == 2 else np.zeros(1)
+    dy = np.full(ntr, dxc) if nd == 2 else np.ones(1)
+    x = np.broadcast_to(x1, (ntr, nx)).copy()
+    yc = 0.5 * ntr * dxc if nd == 2 else 0.
+    r = np.sqrt((x - 0.5) ** 2 + ((y[:, None] - yc) ** 2 if nd == 2 else 0.))
+    hot = r < r_hot
+    prim = np.zeros((cfg.nq, ntr, nx))
+    prim[RHO] = rho0
+    if nd == 2:
+        prim[RHO] *= 1. + 1.e-3 * splitmix_uniform(seed, np.arange(nx * ntr, dtype=np.uint64)).reshape(ntr, nx)
+    prim[RHO] = np.where(hot, 2.5 * prim[RHO], prim[RHO])
+    prim[PPP] = np.where(hot, p_hot, p0)
+    prim[TR1] = np.where(hot, 2., 1.)
+    return Setup("reflecting_box", cfg, 0., x, np.full((ntr, nx), dxc), y, dy, prim,
+                 meta=dict(seed=seed, mesh=mesh))

@@ -43,8 +43,8 @@ typedef struct gammahip_config {
   double gamma, cfl, vi, shock_thr, gamma_max_init;
   double Nalpha, Nmp, Nme;               /* normalizeConstants (constants.cpp:12-20)  */
   int    rg_mode;     /* checkCellForRegrid: 0 none, 1 dx/dy aspect ratio (2DST.cpp:  */
-                      /* 304-322), 2 Blandford-McKee rule (BM1D.cpp:277-319); the     */
-                      /* rule of tracks whose hooks.rg_mode is -1                     */
+                      /* 304-322), 2 Blandford-McKee rule (BM1D.cpp:277-319), 3 the  */
+                      /* user setup's; the rule of tracks whose hooks.rg_mode is -1  */
   double rg[6];       /* rule parameters {split, merge, dth, eta, target, -}          */
   int    c2p_solver;  /* 0 Brent (reference algorithm), 1 safeguarded Newton          */
   int    c2p_guess;   /* 0 previous pressure (reference), 1 learned MLP guess         */
@@ -70,13 +70,21 @@ typedef struct gammahip_config {
  * and sizes, or start a 2D grid from uneven tracks. */
 typedef struct gammahip_track_hooks {
   int    geom;        /* used when config.geom == -1                                  */
-  int    kin_mode;    /* 0 none, 1 fixed boundary-face speeds (BM1D.cpp:239-244)      */
+  int    kin_mode;    /* 0 none, 1 fixed boundary-face speeds (BM1D.cpp:239-244),
+                         2 the user setup's gh_user_face_speed                        */
   double v_in, v_out;
-  int    bnd_mode;    /* 0 outflow only, 1 Blandford-McKee inflow (BM1D.cpp:254-274)  */
+  int    bnd_mode;    /* 0 outflow only, 1 Blandford-McKee inflow (BM1D.cpp:254-274),
+                         2 the user setup's gh_user_ghost                             */
   double bm[4];       /* {lfacShock2(t0), t0, rho_ambient, eta}                       */
-  int    rg_mode;     /* this track's checkCellForRegrid rule; -1: config.rg_mode    */
+  int    rg_mode;     /* this track's checkCellForRegrid rule (0 none, 1 aspect ratio
+                         2DST.cpp:304-322, 2 BM1D.cpp:277-319, 3 the user setup's
+                         gh_user_regrid); -1: config.rg_mode                          */
   int    ncell;       /* initial active cells of this track (2..ncell_mv); 0: ncell_mv */
 } gammahip_track_hooks;
+/* hook kinds (kin_mode, bnd_mode, rg_mode) */
+enum { GAMMAHIP_KIN_NONE = 0, GAMMAHIP_KIN_FIXED = 1, GAMMAHIP_KIN_USER = 2 };
+enum { GAMMAHIP_BND_OUTFLOW = 0, GAMMAHIP_BND_BM = 1, GAMMAHIP_BND_USER = 2 };
+enum { GAMMAHIP_RG_NONE = 0, GAMMAHIP_RG_AR2D = 1, GAMMAHIP_RG_BM1D = 2, GAMMAHIP_RG_USER = 3 };
 
 typedef struct gammahip_stats {
   unsigned long long c2p_calls, c2p_iters, c2p_nobracket, c2p_maxiter, c2p_phicap;
@@ -203,6 +211,15 @@ enum { GAMMAHIP_OPT_BLOCK = 0, GAMMAHIP_OPT_REMAP = 1, GAMMAHIP_OPT_GRAPH = 2,
  * reading them; bit 2 forms the F1 faces with one thread per owner cell
  * instead of one per face slot.  Every combination gives the same bits. */
 int  gammahip_set_option(gammahip_ctx* ctx, int key, long long value);
+
+/* Parameters of the user setup's hooks (gamma_amd/csrc/user_hooks.h, or the
+ * file named by GAMMAHIP_USER_HOOKS at build time): the link-time setup file
+ * of the reference (a file of src/Initial/Tests/ defining userKinematics,
+ * userBoundaries and checkCellForRegrid, selected by Makefile:82-92), for
+ * tracks whose kin_mode, bnd_mode or rg_mode selects the user kind.  par[0..n)
+ * with n <= 16; the rest read 0.  Synchronous; keeps its values until the
+ * next call (0 until the first). */
+int  gammahip_set_user_params(gammahip_ctx* ctx, const double* par, int n);
 
 /* ---- learned cons2prim guess (no counterpart in the reference) ----
  * A 3 -> 64 -> 64 -> 1 MLP predicting log p of a conserved state from

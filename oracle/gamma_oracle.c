@@ -38,6 +38,12 @@
 #endif
 #include "gamma_oracle.h"
 #include "og_pow.h"
+/* the user setup's hooks: the same file the kernels are built with */
+#ifdef OG_USER_HOOKS
+#include OG_USER_HOOKS
+#else
+#include "../gamma_amd/csrc/user_hooks.h"
+#endif
 #include <math.h>
 #ifdef _OPENMP
 #include <omp.h>
@@ -719,8 +725,29 @@ static void grid_state2flux(og_grid *g) {
     for (int i = 0; i < g->ntrack[j]; ++i) { Cell *cl = CELL(g, j, i); fs_state2flux(&g->c, &cl->S, sweep_r(g, cl)); }
 }
 
-/* userKinematics (setup hook): BM1D.cpp:239-244, 2DST.cpp:267-276 */
+/* track j as the user setup's hooks see it (F1 centre and width 0 in 1D) */
+static gh_user_track og_user_track(og_grid *g, int j) {
+  gh_user_track T;
+  T.j = j; T.ntrack = g->ntrack[j]; T.ngst = g->c.ngst; T.geom = g->c.geom; T.t = g->t;
+  T.y = g->c.nd == 2 ? CELL(g, j, 0)->G.x[F1] : 0.;
+  T.dy = g->c.nd == 2 ? CELL(g, j, 0)->G.dx[F1] : 0.;
+  T.par = g->c.upar;
+  return T;
+}
+
+/* userKinematics (setup hook): BM1D.cpp:239-244, 2DST.cpp:267-276; the user
+ * setup's gh_user_face_speed on every face */
 static void user_kinematics(og_grid *g) {
+  if (g->c.kin_mode == OG_KIN_USER) {
+    for (int j = 0; j < g->nj; ++j) {
+      gh_user_track T = og_user_track(g, j);
+      for (int i = 0; i < g->ntrack[j] - 1; ++i) {
+        Face *f = FACE(g, j, i);
+        f->v = gh_user_face_speed(&T, i, f->x[0], f->v);
+      }
+    }
+    return;
+  }
   if (g->c.kin_mode != OG_KIN_FIXED) return;
   for (int j = 0; j < g->nj; ++j)
     for (int n = 0; n < g->c.ngst; ++n) {
@@ -750,8 +777,20 @@ static void bm_profile(const og_config *c, const double *bm, double r, double t,
   *u = sqrt(1. - 1. / (lfac * lfac)) * lfac;
 }
 
-/* userBoundaries (setup hook): BM1D.cpp:254-274 */
+/* userBoundaries (setup hook): BM1D.cpp:254-274; the user setup's
+ * gh_user_ghost on the ghost cells at both ends of every track */
 static void user_boundaries(og_grid *g) {
+  if (g->c.bnd_mode == OG_BND_USER) {
+    for (int j = 0; j < g->nj; ++j) {
+      gh_user_track T = og_user_track(g, j);
+      for (int i = 0; i < g->ntrack[j]; ++i) {
+        if (i > g->iLbnd[j] && i < g->iRbnd[j]) continue;
+        Cell *cl = CELL(g, j, i);
+        gh_user_ghost(&T, i, cl->G.x[0], cl->G.dx[0], cl->S.prim);
+      }
+    }
+    return;
+  }
   if (g->c.bnd_mode != OG_BND_BM) return;
   for (int j = 0; j < g->nj; ++j)
     for (int i = 0; i <= g->iLbnd[j]; ++i) {
@@ -1387,6 +1426,11 @@ static int regrid_check(og_grid *g, int j, int i, double *iS_cache, int *iS_vali
     if (ar > c->rg[0] * target / og_pow(lfac, 3. / 2.)) return SPLIT;
     if (ar < c->rg[1] * target / og_pow(lfac, 3. / 2.)) return MERGE;
     return SKIP;
+  }
+  if (c->rg_mode == OG_RG_USER) {
+    gh_user_track T = og_user_track(g, j);
+    int a = gh_user_regrid(&T, i, cl->G.x[0], cl->G.dx[0], cl->S.prim);
+    return a == 2 ? SPLIT : a == 1 ? MERGE : SKIP;
   }
   return SKIP;
 }

@@ -26,9 +26,11 @@ enum { OG_DEN = 0, OG_TAU = 1, OG_SS1 = 2, OG_SS2 = 3 };
 enum { OG_TR1 = 4, OG_GMN = 6, OG_GMX = 7, OG_GMD = 8, OG_PSN = 9 };
 
 enum { OG_GEOM_CART = 0, OG_GEOM_SPH = 1 };
-enum { OG_KIN_NONE = 0, OG_KIN_FIXED = 1 };      /* userKinematics kinds   */
-enum { OG_BND_NONE = 0, OG_BND_BM = 1 };         /* userBoundaries kinds   */
-enum { OG_RG_NONE = 0, OG_RG_AR2D = 1, OG_RG_BM1D = 2 }; /* checkCellForRegrid */
+/* hook kinds; the USER kinds call the user setup's functions
+   (gamma_amd/csrc/user_hooks.h, or the file named by OG_USER_HOOKS) */
+enum { OG_KIN_NONE = 0, OG_KIN_FIXED = 1, OG_KIN_USER = 2 };   /* userKinematics kinds   */
+enum { OG_BND_NONE = 0, OG_BND_BM = 1, OG_BND_USER = 2 };      /* userBoundaries kinds   */
+enum { OG_RG_NONE = 0, OG_RG_AR2D = 1, OG_RG_BM1D = 2, OG_RG_USER = 3 }; /* checkCellForRegrid */
 
 typedef struct {
   int    nd;        /* NUM_D: 1 or 2 */
@@ -71,6 +73,7 @@ typedef struct {
   double dt_fixed;    /* > 0: every step advances by this dt instead of CFL_ * collect_dt
                          (collect_dt still runs and resets dt_loc); pins the time
                          integrator's order at a fixed step.  0: the reference's rule */
+  double upar[16];    /* parameters of the user setup's hooks */
 } og_config;
 
 typedef struct og_grid og_grid;

@@ -31,6 +31,7 @@ class OgConfig(C.Structure):
         ("c2p_maxiter", C.c_int), ("timestep", C.c_int), ("recon", C.c_int), ("vpspec", C.c_int),
         ("eos", C.c_int), ("circ_regrid", C.c_int),
         ("c2p_tol", C.c_double), ("c2p_form", C.c_int), ("dt_fixed", C.c_double),
+        ("upar", C.c_double * 16),
     ]
 
 
@@ -53,6 +54,8 @@ def make_config(rc, c2p_tol=1e-13, c2p_form=0, dt_fixed=0.) -> OgConfig:
     c.c2p_tol = c2p_tol
     c.c2p_form = c2p_form
     c.dt_fixed = dt_fixed
+    for k, v in enumerate(rc.upar):
+        c.upar[k] = v
     return c
 
 

@@ -61,6 +61,12 @@ CASES = {
     # with the track's smallest / biggest cell
     "riemann_2d_circ": (lambda: _variant(setups.riemann_2d(64, 16), circ_regrid=1), 60),
     "expanding_tube_circ": (lambda: _variant(setups.expanding_tube_1d(200), circ_regrid=1), 300),
+    # a setup written against the user hooks (gamma_amd/csrc/user_hooks.h): a
+    # blast in a reflecting box, the mesh at a fraction of the solver's face
+    # speeds, the user's split/merge rule; in 2D PLM reconstructs negative
+    # pressures at some F1 faces (no wave speed: the zero-flux reading)
+    "reflecting_box_1d": (lambda: setups.reflecting_box(200, mesh=1.), 150),
+    "reflecting_box_2d": (lambda: setups.reflecting_box(48, 16), 60),
 }
 
 
