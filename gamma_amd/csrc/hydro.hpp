@@ -194,8 +194,10 @@ enum { C2P_BRENT = 0, C2P_NEWTON = 1 };
 // a second start); used: calls that started from the second start
 struct C2PStat { int iters; int fail; int evals; int used; };
 
-// Brent's method (Brent 1973, ch. 4) with the GSL fsolver contract used by
-// rel_sph.cpp:273-285: root = best iterate b, bracket = [min(b,c), max(b,c)],
+// Brent's method (Brent 1973, ch. 4) in the iteration form of GSL's brent
+// fsolver (roots/brent.c: the interpolation test min(3 m q - |tol q|, |e q|),
+// the bisection fallback and the bracket bookkeeping, step for step), with
+// the GSL fsolver contract used by rel_sph.cpp:273-285: root = best iterate b, bracket = [min(b,c), max(b,c)],
 // stop on |hi-lo| < 1e-13 * min(|lo|,|hi|) (0 if the bracket straddles 0).
 // flo, fhi: the residual at lo and hi, which the caller has already evaluated
 // while bracketing (the fsolver's set call evaluates them again; the residual

@@ -13,8 +13,10 @@
  * written in the reference's evaluation order.
  *
  * cons2prim uses Brent's method (Brent 1973, "Algorithms for Minimization
- * without Derivatives", ch. 4) with the GSL root-solver contract the reference
- * relies on (gsl_root_fsolver_brent: root = best iterate, [x_lower, x_upper] =
+ * without Derivatives", ch. 4) in the iteration form of GSL's brent fsolver
+ * (roots/brent.c: the interpolation test min(3 m q - |tol q|, |e q|), the
+ * bisection fallback and the bracket bookkeeping after each step, step for
+ * step), with the GSL root-solver contract the reference relies on (gsl_root_fsolver_brent: root = best iterate, [x_lower, x_upper] =
  * current bracket; gsl_root_test_interval: |hi-lo| < epsabs + epsrel*min(|lo|,|hi|),
  * min taken as 0 when the bracket straddles 0).  GSL is not in the image and its
  * version is unpinned by the reference Makefile, so Brent iterates are not
