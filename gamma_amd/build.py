@@ -6,6 +6,10 @@ next to their sources, git-ignored but shipped with the tree).
   (GH_EMU); a test tool that checks indexing and arithmetic against the oracle
   without a GPU.  Never loaded by the package.
 * ``oracle/libgamma_oracle.so`` -- the CPU oracle (test infrastructure).
+* ``tests/libgammahip_piston.so``, ``tests/libgamma_oracle_piston.so`` -- test
+  tools: the HIP library and the oracle built with the test hooks file
+  ``tests/user_hooks_piston.h``, so that the GPU tests can check a hooks file
+  given at build time on both sides.
 * ``gamma_amd/host/shock_tube`` (and ``tests/shock_tube_emu``) -- a C++
   caller of the hot path through ``gamma_amd/host/gammahip_grid.hpp``, linked
   against the HIP library (and against the CPU build, for tests).
@@ -82,6 +86,21 @@ def build_oracle() -> str:
     return os.path.join(d, "libgamma_oracle.so")
 
 
+PISTON_HOOKS = os.path.join(ROOT, "tests", "user_hooks_piston.h")
+
+
+def build_hooks_variant() -> list:
+    """The HIP library and the oracle built with tests/user_hooks_piston.h."""
+    hip_out = os.path.join(ROOT, "tests", "libgammahip_piston.so")
+    _run([HIPCC, *HIP_FLAGS, f'-DGH_USER_HOOKS="{PISTON_HOOKS}"', SRC, "-o", hip_out + ".tmp"])
+    os.replace(hip_out + ".tmp", hip_out)
+    orc_out = os.path.join(ROOT, "tests", "libgamma_oracle_piston.so")
+    _run(["gcc", "-O2", "-std=gnu11", "-fPIC", "-shared", "-ffp-contract=off", "-fno-fast-math", "-fopenmp",
+          f'-DOG_USER_HOOKS="{PISTON_HOOKS}"', os.path.join(ROOT, "oracle", "gamma_oracle.c"),
+          "-o", orc_out, "-lm"])
+    return [hip_out, orc_out]
+
+
 def build_host() -> list:
     src = os.path.join(HERE, "host", "shock_tube.cpp")
     outs = []
@@ -99,6 +118,7 @@ def build_all() -> None:
     build_emu()
     build_oracle()
     build_host()
+    build_hooks_variant()
 
 
 if __name__ == "__main__":

@@ -14,17 +14,6 @@
 #include "hydro.hpp"
 #include "mlp.hpp"
 #include "../../include/gammahip.h"
-// the user setup's hooks (GAMMAHIP_USER_HOOKS names another file at build time)
-#ifdef GH_EMU
-#define GH_USER_FN static inline
-#else
-#define GH_USER_FN static inline __host__ __device__
-#endif
-#ifdef GH_USER_HOOKS
-#include GH_USER_HOOKS
-#else
-#include "user_hooks.h"
-#endif
 
 #include <stdio.h>
 #include <stdlib.h>
@@ -106,6 +95,7 @@ struct Dev {
   int* ntiles;             // [1]
   double* ttrig;           // [nj][7] Trig of each track (2D spherical geometry)
   double* upar;            // [GH_USER_NPAR] parameters of the user hooks
+  int ucell;               // per-cell user hooks: bit 0 userSourceTerms, bit 1 cons2prim_user
 };
 
 GH_HD long long CI(const Dev& d, int j, int i) { return (long long)j * d.nax + i; }
@@ -903,6 +893,10 @@ GH_HD void body_update_g(const Dev& d, int j, int i, int stage, bool valid, cons
       for (int q = 0; q < NQ; ++q) U[q] += f1[q] * dt;
     }
     sources<ND, NQ>(p, U, d.cx[k], d.cdx[k], trig_at_g<ND>(d, j, geom), g.dV, dt, geom, G, d.P.synge);
+    if (d.ucell & 1) {       // userSourceTerms (cell.cpp:112)
+      gh_user_track T = user_track(d, j);
+      gh_user_sources(&T, i, d.cx[k], d.cdx[k], g.dV, dt, p, U);
+    }
     if (NQ == 10 && d.sync) radiative(p, U, g.dV, dt, d.P);
     cxn = (xnR + xnL) / 2.;
     cdxn = (xnR - xnL);
@@ -918,7 +912,7 @@ GH_HD void body_update_g(const Dev& d, int j, int i, int stage, bool valid, cons
 #ifdef GH_EMU
     double pprev = p[PPP];
 #endif
-    if (cell) cons2prim<ND, NQ>(p, U, r1, geom, G, SV, gs, d.maxit, &st, d.P.synge);
+    if (cell) cons2prim<ND, NQ>(p, U, r1, geom, G, SV, gs, d.maxit, &st, d.P.synge, (d.ucell & 2) ? d.upar : nullptr);
 #ifdef GH_EMU
     if (cell) capture_c2p(D, S, tau, pprev, p[PPP], d.shock ? d.shk[k] : 0, stage);
 #endif
@@ -943,7 +937,7 @@ GH_HD void body_update_g(const Dev& d, int j, int i, int stage, bool valid, cons
 #ifdef GH_EMU
     double pprev = p[PPP];
 #endif
-    if (cell) cons2prim<ND, NQ>(p, U, cxn, geom, G, SV, gs, d.maxit, &st, d.P.synge);
+    if (cell) cons2prim<ND, NQ>(p, U, cxn, geom, G, SV, gs, d.maxit, &st, d.P.synge, (d.ucell & 2) ? d.upar : nullptr);
 #ifdef GH_EMU
     if (cell) capture_c2p(D, S, tau, pprev, p[PPP], d.shock ? d.shk[k] : 0, stage);
 #endif
@@ -1204,7 +1198,8 @@ GH_HD bool merge_fill(const Dev& d, int j, int i) {
   d.cdx[b + i] = dxnew;
   Geo g2 = cell_geo<ND>(xnew, dxnew, d.ty[j], d.tdy[j], geom, trig_at<ND>(d, j));
   C2PStat st = {0, 0, 0, 0};
-  cons2prim<ND, NQ>(pc, cc, ND == 1 ? g2.cen0 : xnew, geom, G, d.solver, -1., d.maxit, &st, d.P.synge);
+  cons2prim<ND, NQ>(pc, cc, ND == 1 ? g2.cen0 : xnew, geom, G, d.solver, -1., d.maxit, &st, d.P.synge,
+                    (d.ucell & 2) ? d.upar : nullptr);
   store_v<ND, NQ>(d, d.prim, b + i, pc);
   return left;
 }
@@ -2957,6 +2952,9 @@ int gammahip_set_option(gammahip_ctx* c, int key, long long value) {
       c->remap = value;
       return 0;
     case GAMMAHIP_OPT_GRAPH: c->use_graph = value ? 1 : 0; return 0;
+    case GAMMAHIP_OPT_USER_CELL:
+      if (value < 0 || value > 3) return fail(c, "user cell hooks option must be in 0..3");
+      c->d.ucell = (int)value; return 0;
     case GAMMAHIP_OPT_MLP_THR:
       if (value < 0 || value > 16) return fail(c, "network threshold exponent must be in 0..16");
       c->d.mlp_thr = pow(10., -(double)value); return 0;

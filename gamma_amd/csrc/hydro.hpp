@@ -24,6 +24,14 @@
 
 #define GH_HD __host__ __device__ __forceinline__
 
+// the user setup's hooks (GAMMAHIP_USER_HOOKS names another file at build time)
+#define GH_USER_FN static inline __host__ __device__
+#ifdef GH_USER_HOOKS
+#include GH_USER_HOOKS
+#else
+#include "user_hooks.h"
+#endif
+
 namespace gh {
 
 enum { RHO = 0, PPP = 1, UU1 = 2, UU2 = 3 };
@@ -406,7 +414,8 @@ GH_HD void c2p_inputs(const double* cons, double r, int geom, double* D, double*
 // cons on exit = prim2cons(prim) (rel_sph.cpp:318).
 template <int ND, int NQ>
 GH_HD void cons2prim(double* prim, double* cons, double r, int geom, double G, int solver,
-                     double guess_override, int maxit, C2PStat* st, bool synge = false) {
+                     double guess_override, int maxit, C2PStat* st, bool synge = false,
+                     const double* upar = nullptr) {
   double D = cons[DEN], tau = cons[TAU], E = D + tau;
   double ss[2] = {cons[SS1], ND == 2 ? cons[SS2] : 0.};
   if (ND == 2 && geom == SPH) ss[1] /= r;
@@ -464,12 +473,16 @@ GH_HD void cons2prim(double* prim, double* cons, double r, int geom, double G, i
   double v2 = S2 / Ep2;
   double lf = 1. / sqrt(fabs(1. - v2));
   double rho = D / lf;
+  double uu[ND];
   if (fabs(S) == 0.) {
-    for (int d = 0; d < ND; ++d) prim[UU1 + d] = 0;
+    for (int d = 0; d < ND; ++d) uu[d] = 0;
   } else {
     double v = sqrt(1. - 1. / (lf * lf));
-    for (int d = 0; d < ND; ++d) prim[UU1 + d] = lf * v * (ss[d] / S);
+    for (int d = 0; d < ND; ++d) uu[d] = lf * v * (ss[d] / S);
   }
+  // cons2prim_user (rel_sph.cpp:304): the user setup's, when the context asks
+  if (upar) gh_user_c2p(upar, ND, &rho, &p, uu);
+  for (int d = 0; d < ND; ++d) prim[UU1 + d] = uu[d];
   if (geom == SPH) p = fmax(p, kPFloor);
   prim[PPP] = p;
   prim[RHO] = rho;

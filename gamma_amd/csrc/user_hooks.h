@@ -18,6 +18,10 @@
  * tracers, synchrotron tracers; u2 reads 0 in 1D and is not kept there),
  * velocities as four-velocity components as the solver stores them.
  *
+ * Two more hooks run per cell when the context's GAMMAHIP_OPT_USER_CELL bits
+ * ask for them (the reference calls them in every run; its shipped setups
+ * leave them empty): userSourceTerms (bit 0) and cons2prim_user (bit 1).
+ *
  * The shipped functions define a reflecting box on a slowly moving mesh:
  *   userKinematics      the ngst faces at each end of a track stay still (the
  *                       walls); the others move at par[0] times the speed the
@@ -27,7 +31,9 @@
  *                       before the hook) with the normal velocity reversed;
  *   checkCellForRegrid  split a cell wider than par[1] whose density exceeds
  *                       par[3], merge a cell narrower than par[2] (no rule
- *                       where par[1] or par[2] is 0).
+ *                       where par[1] or par[2] is 0);
+ *   userSourceTerms,    none (empty, as in the reference's setups).
+ *   cons2prim_user
  * par[] are the context's user parameters (gammahip_set_user_params). */
 #ifndef GH_USER_HOOKS_H
 #define GH_USER_HOOKS_H
@@ -73,6 +79,21 @@ GH_USER_FN int gh_user_regrid(const gh_user_track* T, int i, double x, double dx
   if (p[1] > 0. && dx > p[1] && prim[0] > p[3]) return 2;
   if (p[2] > 0. && dx < p[2]) return 1;
   return 0;
+}
+
+/* userSourceTerms (cell.cpp:112, after the geometric source terms): add the
+ * sources of active cell i over dt to cons[] (conserved variables times the
+ * cell volume, as Cell::update holds them there), given the stage's starting
+ * primitives prim[], the cell's centre x, width dx and volume dV */
+GH_USER_FN void gh_user_sources(const gh_user_track* T, int i, double x, double dx, double dV, double dt,
+                                const double* prim, double* cons) {
+  (void)T; (void)i; (void)x; (void)dx; (void)dV; (void)dt; (void)prim; (void)cons;
+}
+
+/* cons2prim_user (rel_sph.cpp:304, rel_cart.cpp:216): adjust the recovered
+ * density, pressure and four-velocity u[0..nd) before the pressure floor */
+GH_USER_FN void gh_user_c2p(const double* par, int nd, double* rho, double* p, double* u) {
+  (void)par; (void)nd; (void)rho; (void)p; (void)u;
 }
 
 #endif

@@ -24,7 +24,7 @@ F_GRADMV0, F_GRADF10 = 300, 400
 FACE_X, FACE_V, FACE_LL, FACE_LR, FACE_FLUX0 = 1000, 1001, 1002, 1003, 1010
 RESET_SHOCK, DETECT_SHOCK, INJECT, INIT_GAMMAS, ADVANCE_TIME = range(5)
 C2P_BRENT, C2P_NEWTON = 0, 1
-OPT_BLOCK, OPT_REMAP, OPT_GRAPH, OPT_MLP_THR, OPT_PERSIST, OPT_OCC, OPT_F1FUSE, OPT_SOLVER, OPT_F1CELL, OPT_FUSE = range(10)
+OPT_BLOCK, OPT_REMAP, OPT_GRAPH, OPT_MLP_THR, OPT_PERSIST, OPT_OCC, OPT_F1FUSE, OPT_SOLVER, OPT_F1CELL, OPT_FUSE, OPT_USER_CELL = range(11)
 
 
 class Config(C.Structure):
@@ -178,6 +178,8 @@ class Grid:
             raise RuntimeError("gammahip_create: " + msg)
         if rc.upar:
             self.set_user_params(rc.upar)       # before load: its first ghost update calls the hooks
+        if rc.user_cell:
+            self._check(self.L.gammahip_set_option(self.h, OPT_USER_CELL, int(rc.user_cell)))
         if hooks_per_track is None:
             hooks_per_track = setup_hooks(setup)
         arr = (TrackHooks * len(hooks_per_track))(*hooks_per_track)

@@ -84,6 +84,7 @@ class RunConfig:
     eos: int = EOS_IDEAL          # EOS_
     circ_regrid: int = 0          # CIRC_REGRID_
     upar: tuple = ()              # parameters of the user setup's hooks (up to 16)
+    user_cell: int = 0            # the user setup's per-cell hooks: 1 userSourceTerms, 2 cons2prim_user
 
     @property
     def nq(self) -> int:          # NUM_Q = NUM_C(4) + NUM_TR(2) + NUM_S

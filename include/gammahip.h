@@ -204,12 +204,18 @@ int  gammahip_sync(gammahip_ctx* ctx);
 enum { GAMMAHIP_OPT_BLOCK = 0, GAMMAHIP_OPT_REMAP = 1, GAMMAHIP_OPT_GRAPH = 2,
        GAMMAHIP_OPT_MLP_THR = 3, GAMMAHIP_OPT_PERSIST = 4, GAMMAHIP_OPT_OCC = 5,
        GAMMAHIP_OPT_F1FUSE = 6, GAMMAHIP_OPT_SOLVER = 7, GAMMAHIP_OPT_F1CELL = 8,
-       GAMMAHIP_OPT_FUSE = 9 };
+       GAMMAHIP_OPT_FUSE = 9, GAMMAHIP_OPT_USER_CELL = 10 };
 /* GAMMAHIP_OPT_FUSE (bit set, 2D grids): bit 0 runs the F1 neighbour search
  * inside the transverse-gradient pass (one kernel instead of two); bit 1 has
  * the F1 passes recompute the MV gradients from the primitives instead of
  * reading them; bit 2 forms the F1 faces with one thread per owner cell
- * instead of one per face slot.  Every combination gives the same bits. */
+ * instead of one per face slot.  Every combination gives the same bits.
+ * GAMMAHIP_OPT_USER_CELL (bit set): the user setup's per-cell hooks, called
+ * where the reference calls them -- bit 0 userSourceTerms (cell.cpp:112, in
+ * every cell update), bit 1 cons2prim_user (rel_sph.cpp:304,
+ * rel_cart.cpp:216, in every cons2prim of the updates and merges); 0 (the
+ * default) leaves them out, as the reference's shipped setups leave them
+ * empty. */
 int  gammahip_set_option(gammahip_ctx* ctx, int key, long long value);
 
 /* Parameters of the user setup's hooks (gamma_amd/csrc/user_hooks.h, or the

@@ -334,6 +334,7 @@ static int fs_cons2prim(const og_config *c, FS *s, double r, long *cnt) {
     double v = sqrt(1. - 1. / (lfac * lfac));
     for (int d = 0; d < c->nd; ++d) uu[d] = lfac * v * (ss[d] / S);
   }
+  if (c->user_cell & 2) gh_user_c2p(c->upar, c->nd, &rho, &p, uu);   /* cons2prim_user, rel_sph.cpp:304 */
   if (sph) p = fmax(p, P_FLOOR);          /* rel_sph.cpp:307, absent in rel_cart */
   s->prim[OG_PPP] = p;
   s->prim[OG_RHO] = rho;
@@ -692,8 +693,10 @@ static void cell_sources(const og_config *c, Cell *cl, double dt) {
   }
 }
 
-/* Cell::update  cell.cpp:102-127 */
-static void cell_update(og_grid *g, Cell *cl, double dt, double xL, double xR) {
+static gh_user_track og_user_track(og_grid *g, int j);
+
+/* Cell::update  cell.cpp:102-127 (cell i of track j) */
+static void cell_update(og_grid *g, int j, int i, Cell *cl, double dt, double xL, double xR) {
   const og_config *c = &g->c;
   int nq = c->nq;
   for (int q = 0; q < nq; ++q) cl->S.cons[q] *= cl->G.dV;
@@ -701,6 +704,10 @@ static void cell_update(og_grid *g, Cell *cl, double dt, double xL, double xR) {
     for (int q = 0; q < nq; ++q)
       cl->S.cons[q] += (cl->flux[0][d][q] - cl->flux[1][d][q]) * dt;
   cell_sources(c, cl, dt);
+  if (c->user_cell & 1) {                 /* userSourceTerms, cell.cpp:112 */
+    gh_user_track T = og_user_track(g, j);
+    gh_user_sources(&T, i, cl->G.x[0], cl->G.dx[0], cl->G.dV, dt, cl->S.prim, cl->S.cons);
+  }
   if (c->sync) cell_radiative(c, cl, dt);
   cell_move(c, cl, xL, xR);
   for (int q = 0; q < nq; ++q) cl->S.cons[q] /= cl->G.dV;
@@ -1211,7 +1218,7 @@ static void grid_update(og_grid *g, double dt) {
     for (int i = g->iLbnd[j] + 1; i <= g->iRbnd[j] - 1; ++i) {
       double xL = FACE(g, j, i - 1)->x[MV], xR = FACE(g, j, i)->x[MV];
       if (xR < xL) OG_ADD(g->cnt[7], 1);
-      cell_update(g, CELL(g, j, i), dt, xL, xR);
+      cell_update(g, j, i, CELL(g, j, i), dt, xL, xR);
     }
   }
 }

@@ -74,6 +74,9 @@ typedef struct {
                          (collect_dt still runs and resets dt_loc); pins the time
                          integrator's order at a fixed step.  0: the reference's rule */
   double upar[16];    /* parameters of the user setup's hooks */
+  int    user_cell;   /* the user setup's per-cell hooks: bit 0 userSourceTerms,
+                         bit 1 cons2prim_user (the reference always calls them;
+                         its shipped setups leave them empty) */
 } og_config;
 
 typedef struct og_grid og_grid;

@@ -31,7 +31,7 @@ class OgConfig(C.Structure):
         ("c2p_maxiter", C.c_int), ("timestep", C.c_int), ("recon", C.c_int), ("vpspec", C.c_int),
         ("eos", C.c_int), ("circ_regrid", C.c_int),
         ("c2p_tol", C.c_double), ("c2p_form", C.c_int), ("dt_fixed", C.c_double),
-        ("upar", C.c_double * 16),
+        ("upar", C.c_double * 16), ("user_cell", C.c_int),
     ]
 
 
@@ -56,6 +56,7 @@ def make_config(rc, c2p_tol=1e-13, c2p_form=0, dt_fixed=0.) -> OgConfig:
     c.dt_fixed = dt_fixed
     for k, v in enumerate(rc.upar):
         c.upar[k] = v
+    c.user_cell = rc.user_cell
     return c
 
 

@@ -14,8 +14,10 @@ the two runs is in test_parity (reflecting_box_1d, reflecting_box_2d).
   normal four-velocity reversed;
 * the split/merge rule acts only when its widths are set;
 * the parameters are checked at the boundary;
-* a hooks file named at build time (a piston, tests/user_hooks_piston.h)
-  replaces the shipped one on both sides.
+* a hooks file named at build time (a piston with a body force and a
+  density floor, tests/user_hooks_piston.h) replaces the shipped one on both
+  sides, the per-cell hooks (userSourceTerms, cons2prim_user) included, on
+  the CPU build and on the GPU.
 """
 from __future__ import annotations
 
@@ -121,6 +123,7 @@ def test_user_params_checked(emu_lib):
 
 # ---- a hooks file given at build time (tests/user_hooks_piston.h) ----
 PISTON = os.path.join(os.path.dirname(os.path.abspath(__file__)), "user_hooks_piston.h")
+PISTON_PAR = (0.05, 0.5, 0.9)          # piston speed, force, density floor
 
 
 @pytest.fixture(scope="module")
@@ -139,23 +142,61 @@ def piston_libs(tmp_path_factory):
     return emu, orc
 
 
-def test_hooks_file_given_at_build_time(piston_libs):
-    """Both sides call the file they were built with: the inner walls move as
-    the piston file says (0.05 t), not as the shipped file's (still), and the
-    two runs agree bit for bit."""
-    emu, orc = piston_libs
-    s = setups.reflecting_box(128, mesh=1., regrid=False)
-    s = dataclasses.replace(s, cfg=s.cfg.replace(upar=(0.05,)))
-    g = hip.Grid(s, lib_path=emu)
+def _piston(lib, orc, ny, user_cell, nsteps):
+    """The piston setup run by the library at `lib` and the oracle at `orc`:
+    bit for bit equal; returns (grid, oracle)."""
+    s = setups.reflecting_box(64 if ny else 128, ny, mesh=1., regrid=False)
+    s = dataclasses.replace(s, cfg=s.cfg.replace(upar=PISTON_PAR, user_cell=user_cell))
+    g = hip.Grid(s, lib_path=lib)
     o = oc.OracleGrid(s, lib_path=orc)
-    w = s.cfg.ngst - 1                       # the inner wall: the face between ghost and active cells
-    x0 = g.faces(1000)[0, w]
-    g.step(60)
-    o.run(60)
+    g.step(nsteps)
+    o.run(nsteps)
     assert np.array_equal(o.tracks()["ntrack"], g.tracks()["ntrack"])
     for f in range(s.cfg.nq):
-        assert np.array_equal(g.active(f)[0], o.active(f)[0]), f
-    nf = g.tracks()["ntrack"][0] - 1
-    assert np.array_equal(g.faces(1000)[0, :nf], o.faces(oc.I_X)[0, :nf])
-    t = o.t
-    np.testing.assert_allclose(g.faces(1000)[0, w] - x0, 0.05 * t, rtol=1e-12)
+        for j, (a, b) in enumerate(zip(g.active(f), o.active(f))):
+            assert np.array_equal(a, b), (f, j)
+    nt = g.tracks()["ntrack"]
+    xg, xo = g.faces(1000), o.faces(oc.I_X)
+    ng = s.cfg.ngst
+    for j in (range(ng, len(nt) - ng) if ny else range(len(nt))):
+        assert np.array_equal(xg[j, :nt[j] - 1], xo[j, :nt[j] - 1]), j
+    assert g.time()[0][0] == o.t
+    return g, o
+
+
+def _piston_checks(lib, orc):
+    for ny in (0, 8):
+        g0, o0 = _piston(lib, orc, ny, 0, 60)
+        # the inner walls (faces between ghost and active cells) move with the piston
+        w = g0.rc.ngst - 1
+        js = range(g0.rc.ngst, g0.nj - g0.rc.ngst) if ny else range(g0.nj)
+        x = g0.faces(1000)
+        x_init = hip.Grid(g0.setup, lib_path=lib).faces(1000)
+        for j in js:
+            np.testing.assert_allclose(x[j, w] - x_init[j, w], PISTON_PAR[0] * o0.t, rtol=1e-12)
+        # the per-cell hooks act: the density floor holds where it did not
+        g3, o3 = _piston(lib, orc, ny, 3, 60)
+        rho0 = np.concatenate(g0.active(0))
+        rho3 = np.concatenate(g3.active(0))
+        assert rho0.min() < PISTON_PAR[2] and rho3.min() >= PISTON_PAR[2]
+
+
+def test_hooks_file_given_at_build_time(piston_libs):
+    """Both sides call the file they were built with, all five hooks (the
+    per-cell ones under GAMMAHIP_OPT_USER_CELL): bit for bit equal runs, the
+    inner walls moving as the piston file says (0.05 t), the density floor
+    holding only with the per-cell hooks on (CPU build)."""
+    _piston_checks(*piston_libs)
+
+
+@pytest.mark.gpu
+def test_gpu_hooks_file_given_at_build_time():
+    """The same on the GPU: the HIP library built with the piston file
+    (tests/libgammahip_piston.so, made by the build) against the oracle built
+    with it."""
+    here = os.path.dirname(os.path.abspath(__file__))
+    lib, orc = (os.path.join(here, n) for n in ("libgammahip_piston.so", "libgamma_oracle_piston.so"))
+    for f in (lib, orc):
+        assert os.path.exists(f), f"{f} is missing: build with __graft_entry__.build()"
+    print("[gpu user hooks] piston hooks file, 1D and 2D, per-cell hooks off and on", flush=True)
+    _piston_checks(lib, orc)

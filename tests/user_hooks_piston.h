@@ -3,7 +3,9 @@
  * stay, the faces between move at the solver's speeds; ghost cells copy the
  * edge cell with the normal four-velocity mirrored about the wall's
  * (u -> 2 u_wall - u, u_wall = par[0] / sqrt(1 - par[0]^2) at the inner end,
- * 0 at the outer); no regrid. */
+ * 0 at the outer); no regrid.  Per cell: a uniform force par[1] along the
+ * track (momentum -par[1] rho dV dt, energy -par[1] rho v dV dt) and a density
+ * floor par[2] after each cons2prim. */
 #ifndef GH_USER_HOOKS_H
 #define GH_USER_HOOKS_H
 #ifndef GH_USER_FN
@@ -36,6 +38,20 @@ GH_USER_FN void gh_user_ghost(const gh_user_track* T, int i, double x, double dx
 GH_USER_FN int gh_user_regrid(const gh_user_track* T, int i, double x, double dx, const double* prim) {
   (void)T; (void)i; (void)x; (void)dx; (void)prim;
   return 0;
+}
+
+GH_USER_FN void gh_user_sources(const gh_user_track* T, int i, double x, double dx, double dV, double dt,
+                                const double* prim, double* cons) {
+  (void)i; (void)x; (void)dx;
+  double g = T->par[1], rho = prim[0], u = prim[2];
+  double v = u / sqrt(1. + u * u);
+  cons[2] -= g * rho * dV * dt;
+  cons[1] -= g * rho * v * dV * dt;
+}
+
+GH_USER_FN void gh_user_c2p(const double* par, int nd, double* rho, double* p, double* u) {
+  (void)nd; (void)p; (void)u;
+  if (*rho < par[2]) *rho = par[2];
 }
 
 #endif
