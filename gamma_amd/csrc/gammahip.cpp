@@ -1513,8 +1513,8 @@ __global__ void __launch_bounds__(KB, W) k_f1own(Dev d, int ni, int remap) {
 // W: minimum wavefronts per SIMD (1 = the compiler's choice; 5 caps the
 // registers at 96 and spills the rest), picked by the launch option; GEO as
 // body_update_g (-1 in 1D).  The 2D Cartesian six-field body fits 96
-// registers without spilling, so its default build runs 5 wavefronts too
-// (0.358 against 0.370 ms on the 2048^2 grid).
+// registers but for 4 spilled dwords, so its default build runs 5 wavefronts
+// too (0.358 against 0.370 ms on the 2048^2 grid).
 template <int ND, int NQ, int W, int GEO>
 constexpr int update_waves() { return (W == 1 && ND == 2 && NQ == 6 && GEO == CART) ? 5 : W; }
 template <bool PK, int ND, int NQ, int SV, int W, int GEO>
