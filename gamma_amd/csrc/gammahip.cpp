@@ -70,6 +70,7 @@ struct Dev {
   double rg[6];
   // per track
   int *ntrack, *tgeom, *tkin, *tbnd, *tfirst, *tiS, *trg;
+  int *tlast;              // last cell of each track the regrid screen flagged
   double *tvin, *tvout, *tbm, *ty, *tdy, *tdt, *tt, *tmin;
   // cells
   double *prim, *cx, *cdx, *dtloc, *U0, *dV0, *gmv, *gf1, *gf1n, *sd, *pspec;
@@ -1209,6 +1210,7 @@ GH_HD bool merge_fill(const Dev& d, int j, int i) {
 GH_HD void body_rg_init(const Dev& d, int j, int i) {
   if (i != 0) return;
   d.tfirst[j] = INT_MAX;
+  d.tlast[j] = -1;
   d.tiS[j] = d.ngst;        // iLbnd + 1
 }
 GH_HD void body_rg_shockpos(const Dev& d, int j, int i) {
@@ -1224,7 +1226,37 @@ GH_HD void body_rg_screen(const Dev& d, int j, int i) {
   if (i < iL + 1 || i > iR - 1) return;
   int a = rg_check<ND, NQ>(d, j, i, (double)d.tiS[j]);
   bool ok = (a == RG_SPLIT && nt < d.nax - 1) || (a == RG_MERGE && nt > 2 * d.ngst + 2);
-  if (ok) GH_ATOMIC_MIN(&d.tfirst[j], i);
+  if (ok) {
+    GH_ATOMIC_MIN(&d.tfirst[j], i);
+    GH_ATOMIC_MAX(&d.tlast[j], i);
+  }
+}
+// The sequential loop may stop at cell i when no cell from i on can act in
+// this pass: every cell from i on is an unmodified cell of the screened track
+// (i past the cells the last split or merge wrote), shifted by the net count
+// dn of actions so far; its rule reads, besides the cell, only the shock
+// index relative to the cell and the outer radius (BM1D) or nothing (aspect
+// ratio), and the guards read the track length; when these are what the
+// screen saw, the cell decides as the screen found, and the screen flagged
+// none past tlast.  (CIRC_REGRID_ and the user rule read more of the track:
+// never stopped early.)
+struct RgStop { int tl, nt0; double iS0, rOut0; bool on; };
+GH_HD RgStop rg_stop_init(const Dev& d, int j, double iS) {
+  RgStop s;
+  s.tl = d.tlast[j];
+  s.nt0 = d.ntrack[j];
+  s.iS0 = iS;
+  s.rOut0 = d.cx[CI(d, j, s.nt0 - d.ngst - 1)];
+  s.on = !d.circ && (d.trg[j] == RG_AR2D || d.trg[j] == RG_BM1D) && s.tl >= 0;
+  return s;
+}
+GH_HD bool rg_stop(const Dev& d, int j, const RgStop& s, int i, int last_mod, double iS) {
+  if (!s.on || i <= last_mod) return false;
+  int nt = d.ntrack[j], dn = nt - s.nt0;
+  if (i <= s.tl + dn) return false;
+  if ((s.nt0 < d.nax - 1) != (nt < d.nax - 1) || (s.nt0 > 2 * d.ngst + 2) != (nt > 2 * d.ngst + 2)) return false;
+  if (d.trg[j] == RG_BM1D && (iS != s.iS0 + dn || d.cx[CI(d, j, nt - d.ngst - 1)] != s.rOut0)) return false;
+  return true;
 }
 // sequential applyRegrid loop of one track from its first flagged cell (the
 // cells before it take no action, so the victims found at the start of the
@@ -1241,7 +1273,10 @@ GH_HD void body_rg_apply(const Dev& d, int j, int i0) {
   double iS = bm ? rg_shock_index(d, j) : 0.;
   int is = -1, ib = -1;
   if (d.circ) rg_victims<ND, NQ>(d, j, &is, &ib);
+  RgStop stop = rg_stop_init(d, j, iS);
+  int last_mod = -1;                       // the last cell a split or merge wrote
   for (; i <= d.ntrack[j] - d.ngst - 1; ++i) {
+    if (rg_stop(d, j, stop, i, last_mod, iS)) break;
     int a = rg_check<ND, NQ>(d, j, i, iS);
     int nt = d.ntrack[j];
     bool acted = false;
@@ -1256,6 +1291,7 @@ GH_HD void body_rg_apply(const Dev& d, int j, int i0) {
       rg_split<ND, NQ>(d, j, isplit);
       d.nsplit[j]++;
       acted = true;
+      last_mod = i + 1;
     }
     if (a == RG_MERGE && nt > 2 * d.ngst + 2) {
       if (d.circ && (ib < 0 || ib == i)) { GH_ATOMIC_ADD(cnt_at(d, CNT_RGCLASH), 1ull); return; }
@@ -1266,6 +1302,7 @@ GH_HD void body_rg_apply(const Dev& d, int j, int i0) {
         d.nsplit[j]++;
       }
       acted = true;
+      last_mod = i;
     }
     if (acted) {
       if (bm) iS = rg_shock_index(d, j);
@@ -1789,10 +1826,13 @@ __global__ void __launch_bounds__(RGB) k_rg_apply(Dev d) {
   double iS = bm ? wg_shock_index(d, j, red) : 0.;
   int is = -1, ib = -1;
   if (d.circ) wg_victims<ND, NQ>(d, j, rv, red, &is, &ib);
+  RgStop stop = rg_stop_init(d, j, iS);
+  int last_mod = -1;                       // the last cell a split or merge wrote
   for (;;) {
     int nt = d.ntrack[j];
     int last = nt - d.ngst - 1;
     if (i > last) break;
+    if (rg_stop(d, j, stop, i, last_mod, iS)) break;
     // The rule of a cell reads that cell, the track's last active cell and the
     // shock index, none of which change until an action is taken: evaluate
     // the next RGB cells at once and take the first action among them, which
@@ -1828,6 +1868,7 @@ __global__ void __launch_bounds__(RGB) k_rg_apply(Dev d) {
         if (i > is) isplit--;
       }
       wg_split<ND, NQ>(d, j, b, isplit);
+      last_mod = i + 1;
     } else {
       if (d.circ && (ib < 0 || ib == i)) {
         if (threadIdx.x == 0) GH_ATOMIC_ADD(cnt_at(d, CNT_RGCLASH), 1ull);
@@ -1835,6 +1876,7 @@ __global__ void __launch_bounds__(RGB) k_rg_apply(Dev d) {
       }
       wg_merge<ND, NQ>(d, j, b, i, &s_act);
       if (d.circ) wg_split<ND, NQ>(d, j, b, ib > i ? ib - 1 : ib);
+      last_mod = i;
     }
     if (bm) iS = wg_shock_index(d, j, red);
     if (d.circ) wg_victims<ND, NQ>(d, j, rv, red, &is, &ib);
@@ -2716,6 +2758,7 @@ int gammahip_create(const gammahip_config* cfg, gammahip_ctx** out) {
   e |= dalloc(c, &d.upar, (size_t)GH_USER_NPAR);
   e |= dalloc(c, &d.ntrack, nj); e |= dalloc(c, &d.trg, nj); e |= dalloc(c, &d.tgeom, nj); e |= dalloc(c, &d.tkin, nj);
   e |= dalloc(c, &d.tbnd, nj); e |= dalloc(c, &d.tfirst, nj); e |= dalloc(c, &d.tiS, nj);
+  e |= dalloc(c, &d.tlast, nj);
   e |= dalloc(c, &d.tvin, nj); e |= dalloc(c, &d.tvout, nj); e |= dalloc(c, &d.tbm, 4 * nj);
   e |= dalloc(c, &d.ty, nj); e |= dalloc(c, &d.tdy, nj); e |= dalloc(c, &d.tdt, nj);
   e |= dalloc(c, &d.tt, nj); e |= dalloc(c, &d.tmin, nj);
