@@ -219,26 +219,41 @@ GH_HD void body_ghost_tracks(const Dev& d, int g, int i) {
   if (i < nts) copy_cell<2>(d, CI(d, j, i), CI(d, js, i));
   if (i < nts - 1) d.fx[CI(d, j, i)] = d.fx[CI(d, js, i)];
 }
+// the same copy without the ghost cells and faces along the track, which
+// body_ghost_mv_src then forms from the edge track directly (one pass for
+// both: the two write disjoint cells and read only the edge track's active
+// cells)
+GH_HD void body_ghost_track_inner(const Dev& d, int g, int i) {
+  int j = g < d.ngst ? g : d.nj - d.ngst + (g - d.ngst);
+  int js = j <= d.jL ? d.jL + 1 : d.jR - 1;
+  int nts = d.ntrack[js];
+  if (i == 0) d.ntrack[j] = nts;
+  if (i >= d.ngst && i < nts - d.ngst) copy_cell<2>(d, CI(d, j, i), CI(d, js, i));
+  if (i >= d.ngst - 1 && i <= nts - d.ngst - 1) d.fx[CI(d, j, i)] = d.fx[CI(d, js, i)];
+}
 
 // userBoundaries of a user setup on ghost cell (j, i): its primitives after
 // the outflow copy, in the reference's slots, through gh_user_ghost
 template <int ND, int NQ>
-GH_HD void user_ghost(const Dev& d, int j, int i, long long kc) {
+GH_HD void user_ghost(const Dev& d, int j, int nt, int i, long long kc) {
   double p[NQ];
   load_v<ND, NQ>(d, d.prim, kc, p);
   gh_user_track T = user_track(d, j);
+  T.ntrack = nt;
   gh_user_ghost(&T, i, d.cx[kc], d.cdx[kc], p);
   store_v<ND, NQ>(d, d.prim, kc, p);
 }
 
-// ghost cells and faces along the track + userBoundaries (BM1D.cpp:254-274)
+// ghost cells and faces along track j + userBoundaries (BM1D.cpp:254-274),
+// formed from the edge cells of track js: j itself, or for a ghost track the
+// edge track it copies (whose active cells it holds)
 template <int ND, int NQ>
-GH_HD void body_ghost_mv(const Dev& d, int j, int kk) {
+GH_HD void body_ghost_mv_src(const Dev& d, int j, int kk, int js) {
   if (kk >= 2 * d.ngst) return;
-  int nt = d.ntrack[j], iL = d.ngst - 1, iR = nt - d.ngst;
+  int nt = d.ntrack[js], iL = d.ngst - 1, iR = nt - d.ngst;
   if (kk < d.ngst) {
     int i = kk;
-    long long ke = CI(d, j, iL + 1), kc = CI(d, j, i);
+    long long ke = CI(d, js, iL + 1), kc = CI(d, j, i);
     double dxe = d.cdx[ke];
     copy_cell<ND>(d, kc, ke);
     d.cx[kc] = d.cx[ke] - (iL - i + 1) * dxe;
@@ -252,17 +267,19 @@ GH_HD void body_ghost_mv(const Dev& d, int j, int kk) {
       d.prim[PPP * d.cs + kc] = p;
       d.prim[sq<ND>(TR1) * d.cs + kc] = 2.;
     }
-    if (d.tbnd[j] == BND_USER) user_ghost<ND, NQ>(d, j, i, kc);
+    if (d.tbnd[j] == BND_USER) user_ghost<ND, NQ>(d, j, nt, i, kc);
   } else {
     int i = iR + (kk - d.ngst);
-    long long ke = CI(d, j, iR - 1), kc = CI(d, j, i);
+    long long ke = CI(d, js, iR - 1), kc = CI(d, j, i);
     double dxe = d.cdx[ke];
     copy_cell<ND>(d, kc, ke);
     d.cx[kc] = d.cx[ke] + (i - iR + 1) * dxe;
     if (i <= nt - 2) d.fx[kc] = d.fx[ke] + (i - iR) * dxe;   // (i-iRbnd): 1d.cpp:241
-    if (d.tbnd[j] == BND_USER) user_ghost<ND, NQ>(d, j, i, kc);
+    if (d.tbnd[j] == BND_USER) user_ghost<ND, NQ>(d, j, nt, i, kc);
   }
 }
+template <int ND, int NQ>
+GH_HD void body_ghost_mv(const Dev& d, int j, int kk) { body_ghost_mv_src<ND, NQ>(d, j, kk, j); }
 
 // ---------------------------------------------------------------------------
 // Grid::movDir_ComputeLambda + updateKinematics + computeFluxes, MV faces
@@ -1478,6 +1495,26 @@ GH_KERNEL2(k_rg_init, body_rg_init)
 GH_KERNEL2P((), k_rg_shockpos, body_rg_shockpos)
 GH_KERNEL2(k_advance_time, body_advance_time)
 template <int ND, int NQ> GH_KERNEL2(k_ghost_mv, (body_ghost_mv<ND, NQ>))
+// updateGhosts of a 2D grid in one launch: workgroups [0, ncopy) copy the
+// ghost tracks' active cells and inner faces from the edge tracks (128 cells
+// each), the rest form the ghost cells along every track, two tracks per
+// workgroup (one per wavefront, 2 ngst lanes busy), ghost tracks from their
+// edge tracks' cells -- the state the copy then the ghost-cell pass leave
+template <int NQ>
+__global__ void __launch_bounds__(128) k_ghosts2d(Dev d, int ncopy) {
+  int b = (int)blockIdx.x, t = (int)threadIdx.x;
+  if (b < ncopy) {
+    long long gi = (long long)b * 128 + t;
+    int g = (int)(gi / d.nax), i = (int)(gi % d.nax);
+    if (g < 2 * d.ngst) body_ghost_track_inner(d, g, i);
+    return;
+  }
+  int j = __builtin_amdgcn_readfirstlane((b - ncopy) * 2 + (t >> 6)), kk = t & 63;
+  if (j < d.nj && kk < 2 * d.ngst) {
+    int js = j <= d.jL ? d.jL + 1 : j >= d.jR ? d.jR - 1 : j;
+    body_ghost_mv_src<2, NQ>(d, j, kk, js);
+  }
+}
 // MV face pass: built for 5 wavefronts per SIMD with six fields and no shock
 // detector (the body then fits 96 registers without spilling; left to itself
 // the compiler hoists loads into more and runs 4)
@@ -2365,8 +2402,17 @@ template <int ND, int NQ>
 struct Pipe {
   static void ghosts(gammahip_ctx* c, bool retile = false) {
     Dev& d = c->d;
-    if (ND == 2) L2(c, KSEL(body_ghost_tracks, k_ghost_tracks), 2 * d.ngst, d.nax);
-    L2(c, KSEL((body_ghost_mv<ND, NQ>), (k_ghost_mv<ND, NQ>)), d.nj, 2 * d.ngst);
+#ifdef GH_EMU
+    if (ND == 2) L2(c, body_ghost_tracks, 2 * d.ngst, d.nax);
+    L2(c, (body_ghost_mv<ND, NQ>), d.nj, 2 * d.ngst);
+#else
+    if (ND == 2) {
+      int ncopy = (int)((2LL * d.ngst * d.nax + 127) / 128);
+      hipLaunchKernelGGL((k_ghosts2d<NQ>), dim3(ncopy + (d.nj + 1) / 2), dim3(128), 0, c->stream, d, ncopy);
+    } else {
+      L2(c, (k_ghost_mv<ND, NQ>), d.nj, 2 * d.ngst);
+    }
+#endif
     if (retile) build_tiles(c);
   }
   // dt = false: the dt candidates of these faces are formed by the update
