@@ -365,17 +365,21 @@ GH_HD void body_faces(const Dev& d, int j, int i) {
 }
 
 // Cell::update_dt, MV candidates (cell.cpp:47-99)
-GH_HD void body_dt(const Dev& d, int j, int i) {
-  int nt = d.ntrack[j];
-  if (i < 1 || i > nt - 2) return;
-  long long k = CI(d, j, i);
+// the smaller of the dt candidates cell k's two MV faces give it
+GH_HD double dt_cand(const Dev& d, long long k) {
   double a = d.cdx[k];
   double l, v, dtL, dtR;
   l = d.flR[k - 1]; v = d.fv[k];
   if (l > v) dtL = a / (l - v); else dtL = 1.e15;
   l = d.flL[k]; v = d.fv[k - 1];
   if (l < v) dtR = a / (v - l); else dtR = 1.e15;
-  double cand = fmin(dtL, dtR);
+  return fmin(dtL, dtR);
+}
+GH_HD void body_dt(const Dev& d, int j, int i) {
+  int nt = d.ntrack[j];
+  if (i < 1 || i > nt - 2) return;
+  long long k = CI(d, j, i);
+  double cand = dt_cand(d, k);
   if (cand < 0) GH_ATOMIC_ADD(cnt_at(d, CNT_NEGDT), 1ull);
   d.dtloc[k] = fmin(d.dtloc[k], cand);
 }
@@ -1631,6 +1635,9 @@ __global__ void __launch_bounds__(1024) k_tiles(Dev d, int bs) {
 }
 
 // per-track minimum of dt_loc with reset (1d.cpp:779-795, 2d.cpp:954-971)
+// FORM: the stage-1 candidates of the faces too (body_dt's, folded into the
+// minimum instead of stored: the step's collect_dt reads dtloc right after)
+template <bool FORM>
 __global__ void __launch_bounds__(KB) k_dt_track_min(Dev d) {
   __shared__ double sm[KB];
   int j = blockIdx.x;
@@ -1638,7 +1645,13 @@ __global__ void __launch_bounds__(KB) k_dt_track_min(Dev d) {
   double m = 1.e15;
   for (int i = threadIdx.x; i < nt; i += KB) {
     long long k = CI(d, j, i);
-    m = fmin(m, d.dtloc[k]);
+    double v = d.dtloc[k];
+    if (FORM && i >= 1 && i <= nt - 2) {
+      double cand = dt_cand(d, k);
+      if (cand < 0) GH_ATOMIC_ADD(cnt_at(d, CNT_NEGDT), 1ull);
+      v = fmin(v, cand);
+    }
+    m = fmin(m, v);
     d.dtloc[k] = 1.e15;
   }
   sm[threadIdx.x] = m;
@@ -2562,9 +2575,12 @@ struct Pipe {
       std::swap(d.gf1, d.gf1n);
     }
   }
-  static void collect_dt(gammahip_ctx* c) {
+  // form = true: the stage-1 dt candidates are formed here (prep ran with
+  // dt = false)
+  static void collect_dt(gammahip_ctx* c, bool form = false) {
     Dev& d = c->d;
 #ifdef GH_EMU
+    if (form) L2P(c, body_dt, k_dt, (), d.nj, d.nax);
     double gm = 1.e15;
     for (int j = 0; j < d.nj; ++j) {
       double m = 1.e15;
@@ -2578,7 +2594,8 @@ struct Pipe {
     }
     for (int j = 0; j < d.nj; ++j) d.tdt[j] = d.P.cfl * (d.nd == 2 ? gm : d.tmin[j]);
 #else
-    hipLaunchKernelGGL(k_dt_track_min, dim3(d.nj), dim3(KB), 0, c->stream, d);
+    if (form) hipLaunchKernelGGL(k_dt_track_min<true>, dim3(d.nj), dim3(KB), 0, c->stream, d);
+    else hipLaunchKernelGGL(k_dt_track_min<false>, dim3(d.nj), dim3(KB), 0, c->stream, d);
     hipLaunchKernelGGL(k_dt_finish, dim3(1), dim3(KB), 0, c->stream, d);
 #endif
   }
@@ -2648,9 +2665,9 @@ struct Pipe {
     regrid(c);
     apply(c, GAMMAHIP_RESET_SHOCK);
     ghosts(c, true);          // track lengths settle here: rebuild the tile list
-    prep(c);
+    prep(c, false);           // the stage-1 dt candidates: formed by collect_dt
     apply(c, GAMMAHIP_DETECT_SHOCK);
-    collect_dt(c);
+    collect_dt(c, true);
     evolve(c);
     apply(c, GAMMAHIP_ADVANCE_TIME);
     apply(c, GAMMAHIP_INJECT);
