@@ -1424,7 +1424,6 @@ __device__ __forceinline__ void tile_of(int remap, int& j, int& i) {
     GH_TILE_LOOP(PK, if (i < ni && j < d.nj) BODY(d, j, i))                      \
   }
 namespace {
-GH_KERNEL2(k_ghost_tracks, body_ghost_tracks)
 GH_KERNEL2P((), k_dt, body_dt)
 // body_nb on a wavefront of 64 consecutive cells of one track: the lower
 // bound of the previous cell's right face (first search of nb_side) is the
