@@ -102,11 +102,34 @@ static inline double og_exp_dd(og_d2 z) {
   og_d2 E = og_exp_dd2(z, &k);
   return ldexp(E.hi, k);
 }
+/* x^(3/4) and x^(3/2) for x in [2^-300, 2^300]: square roots for a first
+ * value y0, then one Newton step on y^4 = x^3 (y^2 = x^3) with the residual
+ * in double-double; y0 minus the step is within about 2^-100 of the power */
+static inline double og_pow34(double x) {
+  double s = sqrt(x), t = sqrt(s);
+  double y0 = s * t;
+  og_d2 y2 = og_two_prod(y0, y0);
+  og_d2 y4 = og_dd_mul(y2, y2);
+  og_d2 x3 = og_dd_mul_d(og_two_prod(x, x), x);
+  og_d2 r = og_dd_add(y4, og_mk(-x3.hi, -x3.lo));
+  return y0 - r.hi / (4. * y0 * y2.hi);
+}
+static inline double og_pow32(double x) {
+  double y0 = x * sqrt(x);
+  og_d2 y2 = og_two_prod(y0, y0);
+  og_d2 x3 = og_dd_mul_d(og_two_prod(x, x), x);
+  og_d2 r = og_dd_add(y2, og_mk(-x3.hi, -x3.lo));
+  return y0 - r.hi / (2. * y0);
+}
 static inline double og_pow(double x, double y) {
   if (y == 0.) return 1.;
   if (!(x >= 0.)) return (x - x) / (x - x);
   if (x == 0.) return y > 0. ? 0. : HUGE_VAL;
   if (x == HUGE_VAL) return y > 0. ? HUGE_VAL : 0.;
+  if (x >= 0x1p-300 && x <= 0x1p+300) {
+    if (y == 0.75) return og_pow34(x);
+    if (y == 1.5) return og_pow32(x);
+  }
   return og_exp_dd(og_dd_mul_d(og_log_dd(x), y));
 }
 /* tanh and log10 the same way (the reference calls the C library's; these
