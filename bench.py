@@ -241,13 +241,16 @@ def _traffic() -> dict:
 
 
 def measured_step_traffic():
-    """Counter bytes of one whole step (every kernel's bytes per launch times
-    its launches per step) from the same summary, or None."""
+    """Counter bytes of one whole step (every solver kernel's bytes per launch
+    times its launches per step) from the same summary, or None.  The
+    runtime's fill and copy kernels in the summary run at set-up (each plane
+    zeroed as it is allocated, the initial state uploaded) and are left out."""
     t = _traffic()
     steps = t.get("steps")
     if not steps:
         return None
-    return sum(v["bytes_per_launch"] * v["dispatches"] for v in t.get("kernels", {}).values()) / steps
+    return sum(v["bytes_per_launch"] * v["dispatches"] for k, v in t.get("kernels", {}).items()
+               if "__amd_rocclr" not in k) / steps
 
 
 def kernel_table(res, nq):
