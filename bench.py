@@ -179,10 +179,10 @@ def run_other(setup, steps, warmup, options=None, solver=None, surrogate=None):
     el = time.perf_counter() - t0
     n1 = track_cells(g)
     st = g.stats()
-    lens = g.tracks()["ntrack"]
+    lens = g.tracks()["ntrack"] - 2 * g.rc.ngst
     g.close()
     return {"value": 0.5 * (n0 + n1) * steps / el, "ms_per_step": 1e3 * el / steps, "cells": n1,
-            "track_cells_min": int(lens.min()) - 4, "track_cells_max": int(lens.max()) - 4,
+            "track_cells_min": int(lens.min()), "track_cells_max": int(lens.max()),
             "c2p_iters_per_call": st["c2p_iters"] / max(st["c2p_calls"], 1),
             "network_starts": st["mlp_used"],
             "failures": {k: st[k] for k in ("c2p_nobracket", "c2p_maxiter", "nan_flux", "negative_dt")},
