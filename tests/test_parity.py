@@ -369,7 +369,8 @@ def test_gpu_pow_equals_oracle_bitwise():
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("case", ["shock_tube_cart", "riemann_2d", "riemann_2d_uneven", "bm_jet_2d"])
+@pytest.mark.parametrize("case", ["shock_tube_cart", "riemann_2d", "riemann_2d_uneven", "bm_jet_2d",
+                                  "reflecting_box_1d", "reflecting_box_2d"])
 @pytest.mark.parametrize("opts", [
     dict(block=128, remap=1, graph=1, persist=0),
     dict(block=256, remap=0, graph=0, persist=4, f1fuse=1),
@@ -391,7 +392,9 @@ def test_gpu_pow_equals_oracle_bitwise():
         "f1-faces-per-owner", "nbgrad-f1-faces-per-owner-capped-graph"])
 def test_gpu_parity_launch_options(case, opts):
     """Launch options change where work runs, never the arithmetic: the
-    trajectory under every option matches the oracle as the default launch does."""
+    trajectory under every option matches the oracle as the default launch does
+    (the reflecting boxes: the user setup's hooks inside the packed, persistent,
+    fused and graph-replayed kernels too)."""
     mk, n = CASES[case]
     s = mk()
     print(f"[gpu parity options] case={case} opts={opts}", flush=True)

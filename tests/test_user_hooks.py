@@ -142,12 +142,14 @@ def piston_libs(tmp_path_factory):
     return emu, orc
 
 
-def _piston(lib, orc, ny, user_cell, nsteps):
-    """The piston setup run by the library at `lib` and the oracle at `orc`:
-    bit for bit equal; returns (grid, oracle)."""
+def _piston(lib, orc, ny, user_cell, nsteps, opts=None):
+    """The piston setup run by the library at `lib` (under launch options
+    `opts`) and the oracle at `orc`: bit for bit equal; returns (grid, oracle)."""
     s = setups.reflecting_box(64 if ny else 128, ny, mesh=1., regrid=False)
     s = dataclasses.replace(s, cfg=s.cfg.replace(upar=PISTON_PAR, user_cell=user_cell))
     g = hip.Grid(s, lib_path=lib)
+    if opts:
+        g.set_options(**opts)
     o = oc.OracleGrid(s, lib_path=orc)
     g.step(nsteps)
     o.run(nsteps)
@@ -200,3 +202,21 @@ def test_gpu_hooks_file_given_at_build_time():
         assert os.path.exists(f), f"{f} is missing: build with __graft_entry__.build()"
     print("[gpu user hooks] piston hooks file, 1D and 2D, per-cell hooks off and on", flush=True)
     _piston_checks(lib, orc)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("opts", [dict(persist=4, graph=1), dict(block=64, remap=1, fuse=3, f1fuse=1, occ=3),
+                                  dict(block=256, remap=0, f1cell=1, fuse=4)],
+                         ids=["packed-graph", "fused-capped", "f1cell-per-owner"])
+def test_gpu_hooks_file_under_launch_options(opts):
+    """The piston file's five hooks inside the packed, persistent, fused and
+    graph-replayed kernels: bit for bit the oracle built with it (1D and 2D,
+    per-cell hooks on)."""
+    here = os.path.dirname(os.path.abspath(__file__))
+    lib, orc = (os.path.join(here, n) for n in ("libgammahip_piston.so", "libgamma_oracle_piston.so"))
+    for f in (lib, orc):
+        assert os.path.exists(f), f"{f} is missing: build with __graft_entry__.build()"
+    print(f"[gpu user hooks] piston hooks file under {opts}", flush=True)
+    for ny in (0, 8):
+        g, _ = _piston(lib, orc, ny, 3, 60, opts)
+        assert np.concatenate(g.active(0)).min() >= PISTON_PAR[2]
