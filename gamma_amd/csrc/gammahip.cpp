@@ -55,6 +55,8 @@ enum { KIN_NONE = 0, KIN_FIXED = 1, KIN_USER = 2 };
 enum { BND_NONE = 0, BND_BM = 1, BND_USER = 2 };
 enum { RG_NONE = 0, RG_AR2D = 1, RG_BM1D = 2, RG_USER = 3 };
 
+constexpr int DT_CH = 1024;   // cells per workgroup of the per-track dt minimum
+
 // ---------------------------------------------------------------------------
 // device view (passed by value to every kernel)
 struct Dev {
@@ -72,6 +74,8 @@ struct Dev {
   int *ntrack, *tgeom, *tkin, *tbnd, *tfirst, *tiS, *trg;
   int *tlast;              // last cell of each track the regrid screen flagged
   double *tvin, *tvout, *tbm, *ty, *tdy, *tdt, *tt, *tmin;
+  double *tpart;           // [nj][dtch] dt minimum of each chunk of DT_CH cells of each track
+  int dtch;                // chunks per track: ceil(nax / DT_CH)
   // cells
   double *prim, *cx, *cdx, *dtloc, *U0, *dV0, *gmv, *gf1, *gf1n, *sd, *pspec;
   int *shk, *nbdj, *nbdlo, *nbdcnt, *nbuj, *nbulo, *nbucnt, *f1off, *f1tot;
@@ -1635,14 +1639,23 @@ __global__ void __launch_bounds__(1024) k_tiles(Dev d, int bs) {
 
 // per-track minimum of dt_loc with reset (1d.cpp:779-795, 2d.cpp:954-971)
 // FORM: the stage-1 candidates of the faces too (body_dt's, folded into the
-// minimum instead of stored: the step's collect_dt reads dtloc right after)
+// minimum instead of stored: the step's collect_dt reads dtloc right after).
+// One workgroup per chunk of DT_CH cells of a track (workgroup j * dtch + ch),
+// so that a long track spreads over many CUs; its minimum goes to tpart and
+// k_dt_finish takes the track's minimum over its chunks (a minimum does not
+// depend on the order it is taken in).
 template <bool FORM>
 __global__ void __launch_bounds__(KB) k_dt_track_min(Dev d) {
   __shared__ double sm[KB];
-  int j = blockIdx.x;
+  int j = (int)blockIdx.x / d.dtch, ch = (int)blockIdx.x - j * d.dtch;
   int nt = d.ntrack[j];
+  int i0 = ch * DT_CH, i1 = min(nt, i0 + DT_CH);
+  if (i0 >= nt) {
+    if (threadIdx.x == 0) d.tpart[blockIdx.x] = 1.e15;
+    return;
+  }
   double m = 1.e15;
-  for (int i = threadIdx.x; i < nt; i += KB) {
+  for (int i = i0 + (int)threadIdx.x; i < i1; i += KB) {
     long long k = CI(d, j, i);
     double v = d.dtloc[k];
     if (FORM && i >= 1 && i <= nt - 2) {
@@ -1659,13 +1672,19 @@ __global__ void __launch_bounds__(KB) k_dt_track_min(Dev d) {
     if ((int)threadIdx.x < s) sm[threadIdx.x] = fmin(sm[threadIdx.x], sm[threadIdx.x + s]);
     __syncthreads();
   }
-  if (threadIdx.x == 0) d.tmin[j] = sm[0];
+  if (threadIdx.x == 0) d.tpart[blockIdx.x] = sm[0];
 }
-// dt = CFL * min: one global value for coupled tracks, per track for independent runs
+// each track's minimum over its chunks; dt = CFL * min: one global value for
+// coupled tracks, per track for independent runs
 __global__ void __launch_bounds__(KB) k_dt_finish(Dev d) {
   __shared__ double sm[KB];
   double m = 1.e15;
-  for (int j = threadIdx.x; j < d.nj; j += KB) m = fmin(m, d.tmin[j]);
+  for (int j = threadIdx.x; j < d.nj; j += KB) {
+    double mj = 1.e15;
+    for (int ch = 0; ch < d.dtch; ++ch) mj = fmin(mj, d.tpart[(long long)j * d.dtch + ch]);
+    d.tmin[j] = mj;
+    m = fmin(m, mj);
+  }
   sm[threadIdx.x] = m;
   __syncthreads();
   for (int s = KB / 2; s > 0; s >>= 1) {
@@ -2593,8 +2612,9 @@ struct Pipe {
     }
     for (int j = 0; j < d.nj; ++j) d.tdt[j] = d.P.cfl * (d.nd == 2 ? gm : d.tmin[j]);
 #else
-    if (form) hipLaunchKernelGGL(k_dt_track_min<true>, dim3(d.nj), dim3(KB), 0, c->stream, d);
-    else hipLaunchKernelGGL(k_dt_track_min<false>, dim3(d.nj), dim3(KB), 0, c->stream, d);
+    dim3 g((unsigned)(d.nj * d.dtch));
+    if (form) hipLaunchKernelGGL(k_dt_track_min<true>, g, dim3(KB), 0, c->stream, d);
+    else hipLaunchKernelGGL(k_dt_track_min<false>, g, dim3(KB), 0, c->stream, d);
     hipLaunchKernelGGL(k_dt_finish, dim3(1), dim3(KB), 0, c->stream, d);
 #endif
   }
@@ -2824,6 +2844,8 @@ int gammahip_create(const gammahip_config* cfg, gammahip_ctx** out) {
   e |= dalloc(c, &d.tvin, nj); e |= dalloc(c, &d.tvout, nj); e |= dalloc(c, &d.tbm, 4 * nj);
   e |= dalloc(c, &d.ty, nj); e |= dalloc(c, &d.tdy, nj); e |= dalloc(c, &d.tdt, nj);
   e |= dalloc(c, &d.tt, nj); e |= dalloc(c, &d.tmin, nj);
+  d.dtch = (d.nax + DT_CH - 1) / DT_CH;
+  e |= dalloc(c, &d.tpart, nj * (size_t)d.dtch);
   e |= dalloc(c, &d.prim, nqs * cs); e |= dalloc(c, &d.cx, cs); e |= dalloc(c, &d.cdx, cs);
   e |= dalloc(c, &d.dtloc, cs); e |= dalloc(c, &d.U0, nqs * cs); e |= dalloc(c, &d.dV0, cs);
   e |= dalloc(c, &d.fx, cs); e |= dalloc(c, &d.fxn, cs); e |= dalloc(c, &d.fx0, cs);
