@@ -284,7 +284,7 @@ def main():
                     help="run only that configuration's plain form for --steps steps (for profilers)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU only; sharding the tracks across GPUs "
+        print(json.dumps({"error": "this library runs on one GPU only; sharding the tracks across GPUs "
                                    "(ghost-track exchange per stage, one dt reduction per step) is "
                                    "designed, not built", "n_gpus": args.gpus}))
         sys.exit(2)
@@ -306,7 +306,7 @@ def main():
     kern = kernel_table(plain, nq)
     dom = max(kern, key=lambda k: kern[k]["ms"] * kern[k]["launches"])
     bcu = b_cu(s.cfg)
-    # the survey's byte model: a fused step moves B_cu per cell-update, a third
+    # the whole-step byte model: a fused step moves B_cu per cell-update, a third
     # of it per stage; the dominant (per-stage) kernel is charged one stage's
     # share of it for the cells it advances
     alg = bcu / 3.0 * plain["cells"]

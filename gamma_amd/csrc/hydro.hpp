@@ -409,7 +409,7 @@ GH_HD void c2p_inputs(const double* cons, double r, int geom, double* D, double*
 // prim[PPP] on entry is the previous pressure (the reference's guess).
 // guess_override > 0 (Newton only) is a second start, e.g. the learned guess:
 // the solve starts from whichever of the two has the smaller residual |f|,
-// so the previous pressure stays the fallback and the judge.  Newton's answer
+// so the previous pressure stays the fallback and the arbiter.  Newton's answer
 // does not depend on its start (see newton).
 // cons on exit = prim2cons(prim) (rel_sph.cpp:318).
 template <int ND, int NQ>

@@ -270,7 +270,7 @@ int  gammahip_pow_batch(const double* x, const double* y, double* out, long long
  * and device.  No counterpart in the reference; for numerics checks. */
 int  gammahip_fn_batch(int fn, const double* x, double* out, long long n);
 /* Seed the context's Newton cons2prim with the network (params = null
- * switches it off).  The previous pressure stays the judge: each solve starts
+ * switches it off).  The previous pressure stays the arbiter: each solve starts
  * from whichever start has the smaller residual; stats.mlp_used counts the
  * solves that started from the network. */
 int  gammahip_mlp_load(gammahip_ctx* ctx, const float* params);

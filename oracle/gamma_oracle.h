@@ -5,7 +5,8 @@
  * algorithm (PLM reconstruction, HLLC on moving faces, Brent cons2prim, RK3
  * with moving faces, CFL dt, shock detection, local synchrotron update, host
  * AMR split/merge).  Only tests/, __graft_entry__.smoke() and bench.py's
- * cpu_baseline leg may load it, and only as the checker or the CPU baseline.
+ * cpu_baseline leg may load it, and only as the reference of the parity
+ * checks or the CPU baseline.
  * The product (gamma_amd/) never links or calls it.
  *
  * Every function in gamma_oracle.c cites the reference file:line it follows.

@@ -1,7 +1,8 @@
 """ctypes binding of the CPU oracle (oracle/libgamma_oracle.so).
 
 Test infrastructure: only tests/, __graft_entry__.smoke() and bench.py's
-cpu_baseline leg import this module, and only as the checker / CPU baseline.
+cpu_baseline leg import this module, and only as the reference of the parity
+checks / CPU baseline.
 """
 from __future__ import annotations
 
