@@ -73,6 +73,7 @@ struct Dev {
   // per track
   int *ntrack, *tgeom, *tkin, *tbnd, *tfirst, *tiS, *trg;
   int *tlast;              // last cell of each track the regrid screen flagged
+  unsigned char* rgflag;   // [nj][nax] the regrid screen's decision per active cell (1: an admissible action)
   double *tvin, *tvout, *tbm, *ty, *tdy, *tdt, *tt, *tmin;
   double *tpart;           // [nj][dtch] dt minimum of each chunk of DT_CH cells of each track
   int dtch;                // chunks per track: ceil(nax / DT_CH)
@@ -1251,6 +1252,7 @@ GH_HD void body_rg_screen(const Dev& d, int j, int i) {
   if (i < iL + 1 || i > iR - 1) return;
   int a = rg_check<ND, NQ>(d, j, i, (double)d.tiS[j]);
   bool ok = (a == RG_SPLIT && nt < d.nax - 1) || (a == RG_MERGE && nt > 2 * d.ngst + 2);
+  d.rgflag[CI(d, j, i)] = ok ? 1 : 0;
   if (ok) {
     GH_ATOMIC_MIN(&d.tfirst[j], i);
     GH_ATOMIC_MAX(&d.tlast[j], i);
@@ -1275,13 +1277,31 @@ GH_HD RgStop rg_stop_init(const Dev& d, int j, double iS) {
   s.on = !d.circ && (d.trg[j] == RG_AR2D || d.trg[j] == RG_BM1D) && s.tl >= 0;
   return s;
 }
-GH_HD bool rg_stop(const Dev& d, int j, const RgStop& s, int i, int last_mod, double iS) {
-  if (!s.on || i <= last_mod) return false;
+// whether the rule inputs besides the cell (guards, shock index relative to
+// the cell, outer radius) are what the screen saw, shifted by dn
+GH_HD bool rg_inputs_same(const Dev& d, int j, const RgStop& s, double iS) {
   int nt = d.ntrack[j], dn = nt - s.nt0;
-  if (i <= s.tl + dn) return false;
   if ((s.nt0 < d.nax - 1) != (nt < d.nax - 1) || (s.nt0 > 2 * d.ngst + 2) != (nt > 2 * d.ngst + 2)) return false;
   if (d.trg[j] == RG_BM1D && (iS != s.iS0 + dn || d.cx[CI(d, j, nt - d.ngst - 1)] != s.rOut0)) return false;
   return true;
+}
+GH_HD bool rg_stop(const Dev& d, int j, const RgStop& s, int i, int last_mod, double iS) {
+  if (!s.on || i <= last_mod) return false;
+  int dn = d.ntrack[j] - s.nt0;
+  if (i <= s.tl + dn) return false;
+  return rg_inputs_same(d, j, s, iS);
+}
+// By the same argument the loop may skip from cell i to the next cell the
+// screen flagged: cell m >= i is the screened cell m - dn, and decides as the
+// screen found (rgflag).  Returns where to continue: i itself when the
+// argument does not hold, the next flagged cell, or -1 when none is left.
+GH_HD int rg_next(const Dev& d, int j, const RgStop& s, int i, int last_mod, double iS) {
+  if (!s.on || i <= last_mod || !rg_inputs_same(d, j, s, iS)) return i;
+  int dn = d.ntrack[j] - s.nt0;
+  long long b = CI(d, j, 0);
+  for (int o = i - dn > 0 ? i - dn : 0; o <= s.tl; ++o)
+    if (d.rgflag[b + o]) return o + dn;
+  return -1;
 }
 // sequential applyRegrid loop of one track from its first flagged cell (the
 // cells before it take no action, so the victims found at the start of the
@@ -1302,6 +1322,8 @@ GH_HD void body_rg_apply(const Dev& d, int j, int i0) {
   int last_mod = -1;                       // the last cell a split or merge wrote
   for (; i <= d.ntrack[j] - d.ngst - 1; ++i) {
     if (rg_stop(d, j, stop, i, last_mod, iS)) break;
+    i = rg_next(d, j, stop, i, last_mod, iS);
+    if (i < 0) break;
     int a = rg_check<ND, NQ>(d, j, i, iS);
     int nt = d.ntrack[j];
     bool acted = false;
@@ -1802,6 +1824,23 @@ __device__ void wg_shift(const Dev& d, long long b, int lo, int hi, int flo, int
     __syncthreads();
   }
 }
+// minimum (MAX = false) or maximum of one int per thread over the workgroup,
+// returned on every thread: shuffles within each wavefront, then the
+// wavefronts' results through red[0 .. RGB/64); the barrier in front lets a
+// previous use of red finish its reads
+template <bool MAX>
+__device__ int wg_reduce(int v, int* red) {
+  for (int o = 32; o > 0; o >>= 1) {
+    int u = __shfl_xor(v, o, 64);
+    v = MAX ? (u > v ? u : v) : (u < v ? u : v);
+  }
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  int r = red[0];
+  for (int w = 1; w < RGB / 64; ++w) r = MAX ? (red[w] > r ? red[w] : r) : (red[w] < r ? red[w] : r);
+  return r;
+}
 // rg_shock_index over the workgroup: the last active cell with p > 1.5 eta
 __device__ double wg_shock_index(const Dev& d, int j, int* red) {
   int nt = d.ntrack[j], iL = d.ngst - 1, iR = nt - d.ngst;
@@ -1809,13 +1848,7 @@ __device__ double wg_shock_index(const Dev& d, int j, int* red) {
   int best = iL + 1;
   for (int ii = iL + 1 + (int)threadIdx.x; ii <= iR - 1; ii += RGB)
     if (d.prim[PPP * d.cs + CI(d, j, ii)] > thr && ii > best) best = ii;
-  red[threadIdx.x] = best;
-  __syncthreads();
-  for (int w = RGB / 2; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w && red[threadIdx.x + w] > red[threadIdx.x]) red[threadIdx.x] = red[threadIdx.x + w];
-    __syncthreads();
-  }
-  double r = red[0];
+  double r = wg_reduce<true>(best, red);
   __syncthreads();
   return r;
 }
@@ -1901,6 +1934,22 @@ __global__ void __launch_bounds__(RGB) k_rg_apply(Dev d) {
     int last = nt - d.ngst - 1;
     if (i > last) break;
     if (rg_stop(d, j, stop, i, last_mod, iS)) break;
+    // rg_next over the workgroup: the next cell the screen flagged, 4 RGB
+    // cells a round
+    if (stop.on && i > last_mod && rg_inputs_same(d, j, stop, iS)) {
+      int dn = nt - stop.nt0, nx = INT_MAX;
+      for (int o = i - dn > 0 ? i - dn : 0; o <= stop.tl; o += 4 * RGB) {
+        int f = INT_MAX;
+        for (int u = 0; u < 4; ++u) {
+          int mm = o + 4 * (int)threadIdx.x + u;
+          if (mm <= stop.tl && d.rgflag[b + mm]) { f = mm; break; }
+        }
+        nx = wg_reduce<false>(f, red);
+        if (nx != INT_MAX) break;
+      }
+      if (nx == INT_MAX) break;
+      i = nx + dn;
+    }
     // The rule of a cell reads that cell, the track's last active cell and the
     // shock index, none of which change until an action is taken: evaluate
     // the next RGB cells at once and take the first action among them, which
@@ -1911,13 +1960,7 @@ __global__ void __launch_bounds__(RGB) k_rg_apply(Dev d) {
       int r = rg_check<ND, NQ>(d, j, m, iS);
       a = (r == RG_SPLIT && nt < d.nax - 1) ? 1 : (r == RG_MERGE && nt > 2 * d.ngst + 2) ? 2 : 0;
     }
-    red[threadIdx.x] = a ? (int)threadIdx.x : RGB;
-    __syncthreads();
-    for (int w = RGB / 2; w > 0; w >>= 1) {
-      if ((int)threadIdx.x < w && red[threadIdx.x + w] < red[threadIdx.x]) red[threadIdx.x] = red[threadIdx.x + w];
-      __syncthreads();
-    }
-    int first = red[0];
+    int first = wg_reduce<false>(a ? (int)threadIdx.x : RGB, red);
     if (first < RGB && (int)threadIdx.x == first) s_act = a;
     __syncthreads();
     if (first == RGB) { i += RGB; continue; }
@@ -2841,6 +2884,7 @@ int gammahip_create(const gammahip_config* cfg, gammahip_ctx** out) {
   e |= dalloc(c, &d.ntrack, nj); e |= dalloc(c, &d.trg, nj); e |= dalloc(c, &d.tgeom, nj); e |= dalloc(c, &d.tkin, nj);
   e |= dalloc(c, &d.tbnd, nj); e |= dalloc(c, &d.tfirst, nj); e |= dalloc(c, &d.tiS, nj);
   e |= dalloc(c, &d.tlast, nj);
+  e |= dalloc(c, &d.rgflag, (size_t)cs);
   e |= dalloc(c, &d.tvin, nj); e |= dalloc(c, &d.tvout, nj); e |= dalloc(c, &d.tbm, 4 * nj);
   e |= dalloc(c, &d.ty, nj); e |= dalloc(c, &d.tdy, nj); e |= dalloc(c, &d.tdt, nj);
   e |= dalloc(c, &d.tt, nj); e |= dalloc(c, &d.tmin, nj);
