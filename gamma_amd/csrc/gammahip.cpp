@@ -1111,6 +1111,20 @@ GH_HD double rg_shock_index(const Dev& d, int j) {
   }
   return iS;
 }
+// rg_shock_index after a split (delta = +1) or merge (delta = -1) that wrote
+// cells up to last_mod and moved every cell after them by delta: when the old
+// index shifted lies past last_mod, its cell is an unchanged candidate and the
+// cells after it are unchanged non-candidates, so it is the new index (the
+// scan's default, iL + 1, never passes this test); otherwise scan again
+GH_HD bool rg_shock_shifts(double iS, int delta, int last_mod) { return iS + delta > last_mod; }
+#ifdef GH_EMU
+// CPU build only: every shifted shock index checked against a new scan --
+// [0] shifts taken, [1] shifts that differ from the scan, [2] actions where
+// the shifted index lands on last_mod and the scan differs from it (the test
+// above must then scan)
+extern "C" long long gammahip_emu_shock_shift_checks[3];
+long long gammahip_emu_shock_shift_checks[3] = {0, 0, 0};
+#endif
 
 template <int ND>
 GH_HD void rg_move_cell(const Dev& d, long long kd, long long ks) {
@@ -1327,6 +1341,7 @@ GH_HD void body_rg_apply(const Dev& d, int j, int i0) {
     int a = rg_check<ND, NQ>(d, j, i, iS);
     int nt = d.ntrack[j];
     bool acted = false;
+    int delta = 0;
     if (a == RG_SPLIT && nt < d.nax - 1) {
       int isplit = i;
       if (d.circ) {
@@ -1338,6 +1353,7 @@ GH_HD void body_rg_apply(const Dev& d, int j, int i0) {
       rg_split<ND, NQ>(d, j, isplit);
       d.nsplit[j]++;
       acted = true;
+      delta = 1;
       last_mod = i + 1;
     }
     if (a == RG_MERGE && nt > 2 * d.ngst + 2) {
@@ -1349,10 +1365,19 @@ GH_HD void body_rg_apply(const Dev& d, int j, int i0) {
         d.nsplit[j]++;
       }
       acted = true;
+      delta = -1;
       last_mod = i;
     }
     if (acted) {
-      if (bm) iS = rg_shock_index(d, j);
+#ifdef GH_EMU
+      if (bm && !d.circ) {
+        double fresh = rg_shock_index(d, j);
+        long long* ck = gammahip_emu_shock_shift_checks;
+        if (rg_shock_shifts(iS, delta, last_mod)) { ck[0]++; if (iS + delta != fresh) ck[1]++; }
+        else if (iS + delta == last_mod && fresh != iS + delta) ck[2]++;
+      }
+#endif
+      if (bm) iS = !d.circ && rg_shock_shifts(iS, delta, last_mod) ? iS + delta : rg_shock_index(d, j);
       if (d.circ) rg_victims<ND, NQ>(d, j, &is, &ib);
     }
   }
@@ -1965,7 +1990,7 @@ __global__ void __launch_bounds__(RGB) k_rg_apply(Dev d) {
     __syncthreads();
     if (first == RGB) { i += RGB; continue; }
     i += first;
-    int act = s_act;
+    int act = s_act, delta = 0;
     __syncthreads();
     // is, ib and i are the same on every lane: a clash returns the whole workgroup
     if (act == 1) {
@@ -1980,6 +2005,7 @@ __global__ void __launch_bounds__(RGB) k_rg_apply(Dev d) {
       }
       wg_split<ND, NQ>(d, j, b, isplit);
       last_mod = i + 1;
+      delta = 1;
     } else {
       if (d.circ && (ib < 0 || ib == i)) {
         if (threadIdx.x == 0) GH_ATOMIC_ADD(cnt_at(d, CNT_RGCLASH), 1ull);
@@ -1988,8 +2014,9 @@ __global__ void __launch_bounds__(RGB) k_rg_apply(Dev d) {
       wg_merge<ND, NQ>(d, j, b, i, &s_act);
       if (d.circ) wg_split<ND, NQ>(d, j, b, ib > i ? ib - 1 : ib);
       last_mod = i;
+      delta = -1;
     }
-    if (bm) iS = wg_shock_index(d, j, red);
+    if (bm) iS = !d.circ && rg_shock_shifts(iS, delta, last_mod) ? iS + delta : wg_shock_index(d, j, red);
     if (d.circ) wg_victims<ND, NQ>(d, j, rv, red, &is, &ib);
     ++i;
   }

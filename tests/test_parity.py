@@ -428,3 +428,26 @@ def test_emu_fused_options_are_bitwise(emu_lib, case, fuse):
     for f in fields:
         assert np.array_equal(a.cells(f), b.cells(f), equal_nan=True), f
     assert a.stats()["splits"] == b.stats()["splits"] and a.stats()["merges"] == b.stats()["merges"]
+
+
+def test_emu_shifted_shock_index_equals_a_scan(emu_lib):
+    """After a split or merge the BM1D rule's shock index is the old one
+    shifted by the action when that cell lies past the cells the action wrote
+    (the cells after it are unchanged); the CPU build checks every such shift
+    against a new scan of the track (gammahip_emu_shock_shift_checks): the
+    shifts are taken and none differs."""
+    import ctypes
+    runs = [(setups.blandford_mckee_1d(400), 200), (setups.expanding_tube_1d(200), 300),
+            (setups.bm_jet_2d(12, 0.3, 40, 160), 60), (setups.ensemble_1d(8, sizes=(64, 512))[0], 100)]
+    taken = 0
+    for s, n in runs:
+        g = hip.Grid(s, lib_path=emu_lib)
+        ck = (ctypes.c_longlong * 3).in_dll(g.L, "gammahip_emu_shock_shift_checks")
+        before = list(ck)
+        g.step(n)
+        sp, mg = g.regrid_counts()
+        assert sp.sum() + mg.sum() > 0, s.name
+        assert ck[1] == before[1], f"{s.name}: a shifted shock index differs from the scan"
+        taken += ck[0] - before[0]
+        g.close()
+    assert taken > 0
