@@ -1329,7 +1329,9 @@ GH_HD void body_rg_apply(const Dev& d, int j, int i0) {
   int i = d.tfirst[j];
   if (i == INT_MAX) return;
   bool bm = d.trg[j] == RG_BM1D;
-  double iS = bm ? rg_shock_index(d, j) : 0.;
+  // the pass's shock index as body_rg_shockpos left it (the same maximum as
+  // rg_shock_index: no cell has changed since)
+  double iS = bm ? (double)d.tiS[j] : 0.;
   int is = -1, ib = -1;
   if (d.circ) rg_victims<ND, NQ>(d, j, &is, &ib);
   RgStop stop = rg_stop_init(d, j, iS);
@@ -1949,7 +1951,7 @@ __global__ void __launch_bounds__(RGB) k_rg_apply(Dev d) {
   if (i == INT_MAX) return;
   bool bm = d.trg[j] == RG_BM1D;
   long long b = CI(d, j, 0);
-  double iS = bm ? wg_shock_index(d, j, red) : 0.;
+  double iS = bm ? (double)d.tiS[j] : 0.;    // as body_rg_apply
   int is = -1, ib = -1;
   if (d.circ) wg_victims<ND, NQ>(d, j, rv, red, &is, &ib);
   RgStop stop = rg_stop_init(d, j, iS);
