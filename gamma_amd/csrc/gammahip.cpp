@@ -1799,7 +1799,7 @@ __global__ void __launch_bounds__(KB) k_f1_scan(Dev d) {
 // in chunks of RGB cells, each chunk read into registers before any of it is
 // written, chunks taken from the far end towards the action, which is the
 // order in which the sequential loops copy.
-constexpr int RGB = 256;
+constexpr int RGB = 512;
 template <int ND, int NQ>
 struct CellRec {
   static constexpr int NS = ND == 1 ? NQ - 1 : NQ;
