@@ -1253,11 +1253,19 @@ GH_HD void body_rg_init(const Dev& d, int j, int i) {
   d.tlast[j] = -1;
   d.tiS[j] = d.ngst;        // iLbnd + 1
 }
+// In the plain launch (a thread for every slot of the track) the threads take
+// the cells from the far end of the track (cell nax-1-i), so that the
+// wavefronts dispatched first hold the largest indices; a thread whose
+// candidate does not exceed the maximum already stored (it only grows; a
+// stale read is lower) leaves out its atomic: the same maximum with far fewer
+// same-address atomics per track.  The packed launch walks the live cells
+// only, in their own order.
 GH_HD void body_rg_shockpos(const Dev& d, int j, int i) {
   if (!active_track(d, j) || d.trg[j] != RG_BM1D) return;
+  if (!d.packed) i = d.nax - 1 - i;
   int nt = d.ntrack[j], iL = d.ngst - 1, iR = nt - d.ngst;
   if (i < iL + 1 || i > iR - 1) return;
-  if (d.prim[PPP * d.cs + CI(d, j, i)] > 1.5 * d.rg[3]) GH_ATOMIC_MAX(&d.tiS[j], i);
+  if (d.prim[PPP * d.cs + CI(d, j, i)] > 1.5 * d.rg[3] && i > d.tiS[j]) GH_ATOMIC_MAX(&d.tiS[j], i);
 }
 template <int ND, int NQ>
 GH_HD void body_rg_screen(const Dev& d, int j, int i) {
